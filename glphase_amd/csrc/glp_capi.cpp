@@ -1,0 +1,561 @@
+// glp_capi.cpp -- host side of the C ABI (include/glphase_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glphase_amd.h"
+#include "glp_common.hpp"
+#include "glp_internal.hpp"
+
+namespace glp {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *msg) {
+  g_err = msg ? msg : "";
+  return code;
+}
+
+int hip_check(hipError_t err, const char *what) {
+  if (err == hipSuccess) return GLP_OK;
+  g_err = std::string(what) + ": " + hipGetErrorString(err);
+  return GLP_ERR_HIP;
+}
+
+#define GLP_TRY(expr)            \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != GLP_OK) return rc_; \
+  } while (0)
+
+// FillEmit (hmmLike.cu:39-44; same order as insti.cpp:1376-1378):
+// E[c][i] = pc[i][0]*GL0 + pc[i][1]*GL1 + pc[i][2]*GL1 + pc[i][3]*GL2 with
+// GL2 = max(1 - GL0 - GL1, 0) (hmmLike.cu:27).
+void emit_table(const float *codebook, unsigned ncode, const float *pc, float *E) {
+  for (unsigned c = 0; c < ncode; ++c) {
+    const float g0 = codebook[2 * c], g1 = codebook[2 * c + 1];
+    const float g2r = 1 - g0 - g1;
+    const float g2 = g2r > 0.0f ? g2r : 0.0f;
+    for (unsigned i = 0; i < 4; ++i)
+      E[4 * c + i] = pc[4 * i + 0] * g0 + pc[4 * i + 1] * g1 + pc[4 * i + 2] * g1 +
+                     pc[4 * i + 3] * g2;
+  }
+}
+
+// (code, nib) -> (e00, e01, e10, e11), nib = f0 | f1<<1 | m0<<2 | m1<<3,
+// e_xy = E[code][(f_x << 1) | m_y].
+void nibble_table(const float *E, unsigned ncode, float *out) {
+  for (unsigned c = 0; c < ncode; ++c)
+    for (unsigned nib = 0; nib < 16; ++nib) {
+      const unsigned f0 = nib & 1u, f1 = (nib >> 1) & 1u, m0 = (nib >> 2) & 1u,
+                     m1 = (nib >> 3) & 1u;
+      float *o = out + 4 * (16 * (size_t)c + nib);
+      o[0] = E[4 * c + ((f0 << 1) | m0)];
+      o[1] = E[4 * c + ((f0 << 1) | m1)];
+      o[2] = E[4 * c + ((f1 << 1) | m0)];
+      o[3] = E[4 * c + ((f1 << 1) | m1)];
+    }
+}
+
+static float norm_value() { return powf(FLT_MIN, 2.0f / 3.0f); }
+
+static bool valid_bits(unsigned bits) { return bits == 4 || bits == 8; }
+
+// grow-only device buffer
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= n) return GLP_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    GLP_TRY(hip_check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc"));
+    n = bytes;
+    return GLP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+}  // namespace glp
+
+using namespace glp;
+
+// ---------------------------------------------------------------------------
+// batched plan
+// ---------------------------------------------------------------------------
+struct glp_plan {
+  int dev = 0;
+  KArgs a{};
+  unsigned nwin = 0, ntasks = 0, Q = 1, grid = 1;
+  DevBuf wins, tasks, ctrs;
+};
+
+static unsigned auto_lanes(unsigned ntasks, unsigned C, int num_cu) {
+  // enough lanes for ~16 waves per CU, never more than a task can use
+  const double target = (double)num_cu * 16.0 * 64.0;
+  unsigned Q = 1;
+  while (Q < 64 && (double)ntasks * Q < target && Q < C + 1) Q *= 2;
+  return Q;
+}
+
+static int fill_args(const glp_mh_desc *d, KArgs &a) {
+  if (!d || !d->windows || d->num_windows == 0)
+    return set_error(GLP_ERR_ARG, "descriptor has no windows");
+  if (!valid_bits(d->bits_per_code)) return set_error(GLP_ERR_ARG, "bits_per_code must be 4 or 8");
+  if (!d->emit_lut) return set_error(GLP_ERR_STATE, "emit_lut not built");
+  if (d->num_samples == 0) return set_error(GLP_ERR_ARG, "num_samples is 0");
+  if (d->prop_mode == GLP_PROP_UNIFORM && d->num_haps < 2 * d->num_samples)
+    return set_error(GLP_ERR_ARG, "num_haps < 2 * num_samples");
+  if (d->prop_mode == GLP_PROP_KLIST && (!d->klist || d->K == 0))
+    return set_error(GLP_ERR_ARG, "klist mode needs klist and K > 0");
+  if (d->prop_mode == GLP_PROP_HOST && d->num_cycles > 0 && !d->extra)
+    return set_error(GLP_ERR_ARG, "host proposal mode needs extra");
+  if (d->prop_mode < 0 || d->prop_mode > 2) return set_error(GLP_ERR_ARG, "bad prop_mode");
+  if (d->precision < 0 || d->precision > 1)
+    return set_error(GLP_ERR_ARG, "precision must be GLP_PREC_F32 or GLP_PREC_MIXED");
+  if (d->k0 + (uint64_t)d->num_cycles >= 0x80000000ull)
+    return set_error(GLP_ERR_ARG, "proposal counters must stay below 2^31");
+  for (unsigned w = 0; w < d->num_windows; ++w) {
+    const glp_window &W = d->windows[w];
+    if (W.num_sites == 0) return set_error(GLP_ERR_ARG, "window with 0 sites");
+    if (W.words_per_hap != (W.num_sites + 63) / 64)
+      return set_error(GLP_ERR_ARG, "words_per_hap must be ceil(num_sites / 64)");
+    if (!W.panel || !W.packed || !W.tran || !W.parents || !W.like)
+      return set_error(GLP_ERR_ARG, "window with a null pointer");
+  }
+  a.N = d->num_samples;
+  a.H = d->num_haps;
+  a.C = d->num_cycles;
+  a.k0 = d->k0;
+  a.pen = d->pen;
+  a.norm = norm_value();
+  a.seed = d->seed;
+  a.gen = d->gen;
+  a.prop_mode = d->prop_mode;
+  a.klist = d->klist;
+  a.K = d->K;
+  a.extra = d->extra;
+  a.trace_like = d->trace_like;
+  a.trace_acc = d->trace_acc;
+  a.lut = d->emit_lut;
+  a.bits = d->bits_per_code;
+  a.lut_entries = 16u << d->bits_per_code;
+  a.precision = d->precision;
+  return GLP_OK;
+}
+
+// Plain-path launches need the window table on the device; one short-lived
+// copy per call keeps the calls stateless.
+static int plain_launch(const glp_mh_desc *desc, unsigned step, void *stream) {
+  KArgs a;
+  GLP_TRY(fill_args(desc, a));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  glp_window *d_wins = nullptr;
+  const size_t bytes = sizeof(glp_window) * desc->num_windows;
+  GLP_TRY(hip_check(hipMallocAsync((void **)&d_wins, bytes, s), "hipMallocAsync"));
+  int rc = hip_check(hipMemcpyAsync(d_wins, desc->windows, bytes, hipMemcpyHostToDevice, s),
+                     "window table copy");
+  if (rc == GLP_OK) rc = launch_plain(a, d_wins, desc->num_windows, step, s);
+  (void)hipFreeAsync(d_wins, s);
+  return rc;
+}
+
+extern "C" {
+
+const char *glp_last_error(void) { return g_err.c_str(); }
+const char *glp_version(void) { return "glphase_amd 0.1 (gfx950)"; }
+
+size_t glp_emit_lut_bytes(unsigned bits, int precision) {
+  (void)precision;
+  if (!valid_bits(bits)) return 0;
+  return (size_t)16 * 16 << bits;
+}
+
+int glp_build_emit_lut(const float *codebook, unsigned n_codes, const float *pc16,
+                       int precision, void *lut_dev) {
+  if (!codebook || !pc16 || !lut_dev) return set_error(GLP_ERR_ARG, "null argument");
+  if (n_codes != 16 && n_codes != 256) return set_error(GLP_ERR_ARG, "n_codes must be 16 or 256");
+  if (precision < 0 || precision > 1) return set_error(GLP_ERR_ARG, "bad precision");
+  std::vector<float> E(4 * (size_t)n_codes), T(64 * (size_t)n_codes);
+  emit_table(codebook, n_codes, pc16, E.data());
+  nibble_table(E.data(), n_codes, T.data());
+  return hip_check(hipMemcpy(lut_dev, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice),
+                   "emit table copy");
+}
+
+int glp_plain_init(const glp_mh_desc *desc, void *stream) {
+  return plain_launch(desc, 0, stream);
+}
+
+int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream) {
+  if (desc && c >= desc->num_cycles) return set_error(GLP_ERR_ARG, "cycle index >= num_cycles");
+  return plain_launch(desc, c + 1, stream);
+}
+
+int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan **out) {
+  if (!out) return set_error(GLP_ERR_ARG, "null out");
+  *out = nullptr;
+  KArgs a;
+  GLP_TRY(fill_args(desc, a));
+  if (lanes_per_chain != 0 &&
+      (lanes_per_chain > 64 || (lanes_per_chain & (lanes_per_chain - 1)) != 0))
+    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 or a power of two <= 64");
+  glp_plan *p = new glp_plan();
+  p->a = a;
+  GLP_TRY(hip_check(hipGetDevice(&p->dev), "hipGetDevice"));
+  int num_cu = 0;
+  GLP_TRY(hip_check(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, p->dev),
+                    "device attribute"));
+  p->nwin = desc->num_windows;
+  // tasks: every (window, sample), longest windows first
+  std::vector<unsigned> order(p->nwin);
+  for (unsigned w = 0; w < p->nwin; ++w) order[w] = w;
+  std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) {
+    return desc->windows[x].num_sites > desc->windows[y].num_sites;
+  });
+  std::vector<uint2> tasks;
+  tasks.reserve((size_t)p->nwin * a.N);
+  for (unsigned w : order)
+    for (unsigned i = 0; i < a.N; ++i) tasks.push_back(make_uint2(w, i));
+  p->ntasks = (unsigned)tasks.size();
+  p->Q = lanes_per_chain ? lanes_per_chain : auto_lanes(p->ntasks, a.C, num_cu);
+  int bpc = 0;
+  int rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, p->Q, (size_t)a.lut_entries * 16, &bpc);
+  if (rc != GLP_OK) {
+    delete p;
+    return rc;
+  }
+  if (bpc < 1) bpc = 1;
+  const unsigned slots_per_block = 4 * (64 / p->Q);
+  const unsigned need = (p->ntasks + slots_per_block - 1) / slots_per_block;
+  p->grid = std::max(1u, std::min((unsigned)(bpc * num_cu), need));
+  rc = p->wins.ensure(sizeof(glp_window) * p->nwin);
+  if (rc == GLP_OK) rc = p->tasks.ensure(sizeof(uint2) * tasks.size());
+  if (rc == GLP_OK) rc = p->ctrs.ensure(4 * sizeof(unsigned long long));
+  if (rc == GLP_OK)
+    rc = hip_check(hipMemcpy(p->wins.p, desc->windows, sizeof(glp_window) * p->nwin,
+                             hipMemcpyHostToDevice), "window table copy");
+  if (rc == GLP_OK)
+    rc = hip_check(hipMemcpy(p->tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
+                             hipMemcpyHostToDevice), "task list copy");
+  if (rc != GLP_OK) {
+    glp_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return GLP_OK;
+}
+
+int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  KArgs a = p->a;
+  a.gen = gen;
+  a.pen = pen;
+  a.seed = seed;
+  return launch_batched(a, p->wins.as<glp_window>(), p->tasks.as<uint2>(), p->ntasks,
+                        p->ctrs.as<unsigned long long>(), p->Q, p->grid,
+                        static_cast<hipStream_t>(stream));
+}
+
+int glp_plan_stats(glp_plan *p, uint64_t *evals, uint64_t *useful) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  unsigned long long h[4];
+  GLP_TRY(hip_check(hipDeviceSynchronize(), "synchronize"));
+  GLP_TRY(hip_check(hipMemcpy(h, p->ctrs.p, sizeof(h), hipMemcpyDeviceToHost), "stats copy"));
+  if (evals) *evals = h[1];
+  if (useful) *useful = h[2];
+  return GLP_OK;
+}
+
+int glp_plan_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  KArgs a = p->a;
+  a.gen = gen;
+  a.pen = pen;
+  a.seed = seed;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (unsigned step = 0; step <= a.C; ++step)
+    GLP_TRY(launch_plain(a, p->wins.as<glp_window>(), p->nwin, step, s));
+  return GLP_OK;
+}
+
+int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  if (lanes) *lanes = p->Q;
+  if (grid) *grid = p->grid;
+  return GLP_OK;
+}
+
+int glp_plan_destroy(glp_plan *p) {
+  if (!p) return GLP_OK;
+  p->wins.release();
+  p->tasks.release();
+  p->ctrs.release();
+  delete p;
+  return GLP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// reference-compatible layer
+// ---------------------------------------------------------------------------
+struct glp_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<float> tran, pc, codebook;
+  unsigned bits = 0;
+  float pen = 1.0f;
+  bool lut_dirty = true;
+  DevBuf lut, tran_d, packed, panel, extra, parents, like, samples, evalp, out;
+  size_t packed_words = 0;
+};
+
+static int ensure_lut(glp_ctx *c) {
+  if (c->codebook.empty()) return set_error(GLP_ERR_STATE, "codebook not set");
+  if (c->pc.size() != 16) return set_error(GLP_ERR_STATE, "mutation matrix not set");
+  if (!c->lut_dirty) return GLP_OK;
+  const unsigned ncode = (unsigned)(c->codebook.size() / 2);
+  unsigned bits = 0;
+  while ((1u << bits) < ncode) ++bits;
+  if (bits != 4 && bits != 8) return set_error(GLP_ERR_ARG, "codebook size must be 16 or 256");
+  GLP_TRY(c->lut.ensure(glp_emit_lut_bytes(bits, GLP_PREC_F32)));
+  GLP_TRY(glp_build_emit_lut(c->codebook.data(), ncode, c->pc.data(), GLP_PREC_F32, c->lut.p));
+  c->lut_dirty = false;
+  return GLP_OK;
+}
+
+static int ctx_window(glp_ctx *c, const uint64_t *hap_panel, size_t n_panel_words,
+                      unsigned num_sites, unsigned num_samples, glp_window &W) {
+  if (!hap_panel || num_sites == 0 || num_samples == 0)
+    return set_error(GLP_ERR_ARG, "null panel or zero sites/samples");
+  if (c->tran.size() < 3 * (size_t)num_sites)
+    return set_error(GLP_ERR_STATE, "transition matrix has fewer than 3 * num_sites entries");
+  if (c->bits == 0) return set_error(GLP_ERR_STATE, "packed GLs not set");
+  const unsigned per = 32 / c->bits;
+  // hmmLike.cu:407-414
+  const size_t expect = (size_t)num_samples * ((num_sites + per - 1) / per);
+  if (c->packed_words != expect)
+    return set_error(GLP_ERR_ARG, "packed GLs have the wrong size for num_sites x num_samples");
+  const unsigned wn = (num_sites + 63) / 64;
+  if (n_panel_words % wn != 0) return set_error(GLP_ERR_ARG, "panel size not a multiple of words per hap");
+  GLP_TRY(ensure_lut(c));
+  if (c->codebook.size() != 2 * ((size_t)1 << c->bits))
+    return set_error(GLP_ERR_STATE, "codebook size does not match bits_per_code of the packed GLs");
+  GLP_TRY(c->panel.ensure(n_panel_words * 8));
+  GLP_TRY(hip_check(hipMemcpyAsync(c->panel.p, hap_panel, n_panel_words * 8, hipMemcpyHostToDevice,
+                                   c->stream), "panel copy"));
+  GLP_TRY(c->parents.ensure((size_t)num_samples * 16));
+  GLP_TRY(c->like.ensure((size_t)num_samples * 8));
+  W.num_sites = num_sites;
+  W.words_per_hap = wn;
+  W.panel = c->panel.as<uint64_t>();
+  W.packed = c->packed.as<uint32_t>();
+  W.tran = c->tran_d.as<float>();
+  W.parents = c->parents.as<unsigned>();
+  W.like = c->like.as<double>();
+  return GLP_OK;
+}
+
+extern "C" {
+
+int glp_check_device(int *num_devices) {
+  int n = 0;
+  hipError_t err = hipGetDeviceCount(&n);
+  if (num_devices) *num_devices = (err == hipSuccess) ? n : 0;
+  if (err != hipSuccess || n == 0) return set_error(GLP_ERR_NODEV, "no HIP device");
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess &&
+        std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      return GLP_OK;
+  }
+  return set_error(GLP_ERR_NODEV, "no gfx950 device");
+}
+
+int glp_create(int dev_id, glp_ctx **out) {
+  if (!out) return set_error(GLP_ERR_ARG, "null out");
+  *out = nullptr;
+  if (dev_id < 0) return set_error(GLP_ERR_ARG, "negative device id");
+  int n = 0;
+  GLP_TRY(glp_check_device(&n));
+  if (dev_id >= n) return set_error(GLP_ERR_ARG, "device id out of range");
+  hipDeviceProp_t prop;
+  GLP_TRY(hip_check(hipGetDeviceProperties(&prop, dev_id), "device properties"));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(GLP_ERR_NODEV, "device is not gfx950");
+  GLP_TRY(hip_check(hipSetDevice(dev_id), "hipSetDevice"));
+  glp_ctx *c = new glp_ctx();
+  c->dev = dev_id;
+  int rc = hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream create");
+  if (rc != GLP_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return GLP_OK;
+}
+
+int glp_destroy(glp_ctx *c) {
+  if (!c) return GLP_OK;
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf *b : {&c->lut, &c->tran_d, &c->packed, &c->panel, &c->extra, &c->parents, &c->like,
+                    &c->samples, &c->evalp, &c->out})
+    b->release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GLP_OK;
+}
+
+int glp_set_tran(glp_ctx *c, const float *tran, size_t n) {
+  if (!c || !tran || n == 0 || n % 3 != 0) return set_error(GLP_ERR_ARG, "tran must hold 3 floats per site");
+  GLP_TRY(hip_check(hipSetDevice(c->dev), "hipSetDevice"));
+  c->tran.assign(tran, tran + n);
+  GLP_TRY(c->tran_d.ensure(n * sizeof(float)));
+  return hip_check(hipMemcpy(c->tran_d.p, tran, n * sizeof(float), hipMemcpyHostToDevice), "tran copy");
+}
+
+int glp_set_mutation_mat(glp_ctx *c, const float *pc16) {
+  if (!c || !pc16) return set_error(GLP_ERR_ARG, "null argument");
+  c->pc.assign(pc16, pc16 + 16);
+  c->lut_dirty = true;
+  return GLP_OK;
+}
+
+int glp_set_codebook(glp_ctx *c, const float *pairs, unsigned n_codes) {
+  if (!c || !pairs) return set_error(GLP_ERR_ARG, "null argument");
+  if (n_codes != 16 && n_codes != 256) return set_error(GLP_ERR_ARG, "codebook must hold 16 or 256 codes");
+  c->codebook.assign(pairs, pairs + 2 * (size_t)n_codes);
+  c->lut_dirty = true;
+  return GLP_OK;
+}
+
+int glp_set_packed_gls(glp_ctx *c, const uint32_t *packed, size_t n_words, unsigned bits) {
+  if (!c || !packed || n_words == 0) return set_error(GLP_ERR_ARG, "null or empty packed GLs");
+  if (!valid_bits(bits)) return set_error(GLP_ERR_ARG, "bits_per_code must be 4 or 8");
+  GLP_TRY(hip_check(hipSetDevice(c->dev), "hipSetDevice"));
+  GLP_TRY(c->packed.ensure(n_words * 4));
+  c->packed_words = n_words;
+  c->bits = bits;
+  return hip_check(hipMemcpy(c->packed.p, packed, n_words * 4, hipMemcpyHostToDevice), "packed copy");
+}
+
+int glp_set_pen(glp_ctx *c, float pen) {
+  if (!c) return set_error(GLP_ERR_ARG, "null ctx");
+  if (!(pen >= 0.0f)) return set_error(GLP_ERR_ARG, "pen must be >= 0");
+  c->pen = pen;
+  return GLP_OK;
+}
+
+int glp_run_hmm_on_device(glp_ctx *c, const uint64_t *hap_panel, size_t n_panel_words,
+                          const unsigned *extra_prop_haps, unsigned num_sites,
+                          unsigned num_samples, unsigned num_cycles, unsigned *hap_idxs,
+                          unsigned long seed) {
+  if (!c || !hap_idxs || (num_cycles > 0 && !extra_prop_haps))
+    return set_error(GLP_ERR_ARG, "null argument");
+  GLP_TRY(hip_check(hipSetDevice(c->dev), "hipSetDevice"));
+  glp_window W;
+  GLP_TRY(ctx_window(c, hap_panel, n_panel_words, num_sites, num_samples, W));
+  const unsigned wn = W.words_per_hap;
+  const size_t nhaps = n_panel_words / wn;
+  // parents: reference SoA [4][N] -> engine AoS [N][4]
+  std::vector<unsigned> aos(4 * (size_t)num_samples);
+  for (unsigned i = 0; i < num_samples; ++i)
+    for (unsigned q = 0; q < 4; ++q) {
+      const unsigned h = hap_idxs[(size_t)q * num_samples + i];
+      if (h >= nhaps) return set_error(GLP_ERR_ARG, "initial parent index outside the panel");
+      aos[4 * (size_t)i + q] = h;
+    }
+  for (size_t e = 0; e < (size_t)num_cycles * num_samples; ++e)
+    if (extra_prop_haps[e] >= nhaps) return set_error(GLP_ERR_ARG, "proposal index outside the panel");
+  GLP_TRY(hip_check(hipMemcpyAsync(W.parents, aos.data(), aos.size() * 4, hipMemcpyHostToDevice,
+                                   c->stream), "parents copy"));
+  if (num_cycles > 0) {
+    GLP_TRY(c->extra.ensure((size_t)num_cycles * num_samples * 4));
+    GLP_TRY(hip_check(hipMemcpyAsync(c->extra.p, extra_prop_haps, (size_t)num_cycles * num_samples * 4,
+                                     hipMemcpyHostToDevice, c->stream), "proposal copy"));
+  }
+  glp_mh_desc d{};
+  d.num_windows = 1;
+  d.windows = &W;
+  d.num_samples = num_samples;
+  d.num_haps = (unsigned)nhaps;
+  d.bits_per_code = c->bits;
+  d.emit_lut = c->lut.p;
+  d.num_cycles = num_cycles;
+  d.pen = c->pen;
+  d.seed = (uint64_t)seed;
+  d.prop_mode = GLP_PROP_HOST;
+  d.extra = c->extra.as<unsigned>();
+  d.precision = GLP_PREC_F32;
+  glp_plan *p = nullptr;
+  GLP_TRY(glp_plan_create(&d, 0, &p));
+  int rc = glp_plan_run(p, 0, c->pen, d.seed, c->stream);
+  if (rc == GLP_OK)
+    rc = hip_check(hipMemcpyAsync(aos.data(), W.parents, aos.size() * 4, hipMemcpyDeviceToHost,
+                                  c->stream), "parents copy back");
+  if (rc == GLP_OK) rc = hip_check(hipStreamSynchronize(c->stream), "synchronize");
+  glp_plan_destroy(p);
+  if (rc != GLP_OK) return rc;
+  for (unsigned i = 0; i < num_samples; ++i)
+    for (unsigned q = 0; q < 4; ++q) hap_idxs[(size_t)q * num_samples + i] = aos[4 * (size_t)i + q];
+  return GLP_OK;
+}
+
+int glp_hmm_like(glp_ctx *c, const uint64_t *hap_panel, size_t n_panel_words, unsigned num_sites,
+                 unsigned num_samples, unsigned n_evals, const unsigned *samples,
+                 const unsigned *parents, float *out) {
+  if (!c || !samples || !parents || !out) return set_error(GLP_ERR_ARG, "null argument");
+  if (n_evals == 0) return GLP_OK;
+  GLP_TRY(hip_check(hipSetDevice(c->dev), "hipSetDevice"));
+  glp_window W;
+  GLP_TRY(ctx_window(c, hap_panel, n_panel_words, num_sites, num_samples, W));
+  const size_t nhaps = n_panel_words / W.words_per_hap;
+  for (unsigned e = 0; e < n_evals; ++e) {
+    if (samples[e] >= num_samples) return set_error(GLP_ERR_ARG, "sample index out of range");
+    for (unsigned q = 0; q < 4; ++q)
+      if (parents[4 * (size_t)e + q] >= nhaps) return set_error(GLP_ERR_ARG, "parent index outside the panel");
+  }
+  GLP_TRY(c->samples.ensure((size_t)n_evals * 4));
+  GLP_TRY(c->evalp.ensure((size_t)n_evals * 16));
+  GLP_TRY(c->out.ensure((size_t)n_evals * 4));
+  GLP_TRY(hip_check(hipMemcpyAsync(c->samples.p, samples, (size_t)n_evals * 4, hipMemcpyHostToDevice,
+                                   c->stream), "samples copy"));
+  GLP_TRY(hip_check(hipMemcpyAsync(c->evalp.p, parents, (size_t)n_evals * 16, hipMemcpyHostToDevice,
+                                   c->stream), "parents copy"));
+  glp_window *d_win = nullptr;
+  GLP_TRY(hip_check(hipMallocAsync((void **)&d_win, sizeof(glp_window), c->stream), "hipMallocAsync"));
+  int rc = hip_check(hipMemcpyAsync(d_win, &W, sizeof(W), hipMemcpyHostToDevice, c->stream), "window copy");
+  KArgs a{};
+  a.N = num_samples;
+  a.norm = norm_value();
+  a.lut = c->lut.p;
+  a.bits = c->bits;
+  a.lut_entries = 16u << c->bits;
+  if (rc == GLP_OK)
+    rc = launch_eval(a, d_win, c->samples.as<unsigned>(), c->evalp.as<unsigned>(), n_evals,
+                     c->out.as<float>(), c->stream);
+  (void)hipFreeAsync(d_win, c->stream);
+  if (rc == GLP_OK)
+    rc = hip_check(hipMemcpyAsync(out, c->out.p, (size_t)n_evals * 4, hipMemcpyDeviceToHost, c->stream),
+                   "result copy");
+  if (rc == GLP_OK) rc = hip_check(hipStreamSynchronize(c->stream), "synchronize");
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// glp_common.hpp -- definitions shared by host and device code of the
+// GLPhase MH surrogate-parent engine for gfx950.
+//
+// The random streams here replace curand XORWOW (hmmLike.cu:163,188,196,
+// 220-226) and the host GSL draws of HMMLike::RunHMMOnSamples
+// (hmmLike.cpp:91-102) with a counter-based Philox4x32-10 generator, so a
+// proposal's draws depend only on (seed, generation, window, sample, proposal)
+// and not on the order in which lanes evaluate proposals.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GLP_HD __host__ __device__ __forceinline__
+
+namespace glp {
+
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+GLP_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umulhi(a, b);
+#else
+  return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+GLP_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += kPhiloxW0;
+      k1 += kPhiloxW1;
+    }
+    const uint32_t hi0 = mulhi32(kPhiloxM0, c.x), lo0 = kPhiloxM0 * c.x;
+    const uint32_t hi1 = mulhi32(kPhiloxM1, c.z), lo1 = kPhiloxM1 * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+// Counter layout: {k, sample, window, generation}; key = 64-bit seed.
+// k < 2^31 is MH proposal k; kInitTag draws the 4 fresh parents of a
+// generation; kWorkTag | (site >> 2) draws hmm_work's per-site uniforms.
+constexpr uint32_t kInitTag = 0xFFFFFFFFu;
+constexpr uint32_t kWorkTag = 0x80000000u;
+
+GLP_HD U4 draw(uint64_t seed, uint32_t k, uint32_t sample, uint32_t window,
+               uint32_t gen) {
+  return philox4x32_10(U4{k, sample, window, gen}, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
+}
+
+// floor(x * n / 2^32): a uniform integer of [0, n)
+GLP_HD uint32_t uniform_int(uint32_t x, uint32_t n) { return mulhi32(x, n); }
+
+// UnifSampler::SampleHap (sampler.cpp:29-39): uniform over [0, H) minus the
+// sample's own haplotypes 2s and 2s+1, in closed form.
+GLP_HD uint32_t unif_excl(uint32_t x, uint32_t nhaps, uint32_t sample) {
+  const uint32_t r = uniform_int(x, nhaps - 2u);
+  return r + (r >= 2u * sample ? 2u : 0u);
+}
+
+// a double of [0, 1) with 32 random bits (the resolution of gsl_rng_uniform)
+GLP_HD double u01(uint32_t x) { return (double)x * (1.0 / 4294967296.0); }
+
+// Proposal-selection modes.
+enum PropMode : int {
+  kPropUniform = 0,  // candidates uniform over all other haplotypes
+  kPropKList = 1,    // candidates uniform over a per-sample list of K haps
+  kPropHost = 2,     // initial parents and candidates supplied by the host
+};
+
+enum Precision : int {
+  kPrecF32 = 0,    // FP32 recurrence and FP32 log accumulation (reference)
+  kPrecMixed = 1,  // FP32 recurrence, FP64 log accumulation and MH ratio
+  kPrecF16Lut = 2, // as kPrecMixed with an FP16 emission table
+};
+
+}  // namespace glp

@@ -1,0 +1,55 @@
+// glp_internal.hpp -- declarations shared by the kernel file and the host
+// side of the C ABI.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/glphase_amd.h"
+
+namespace glp {
+
+// Kernel arguments common to every launch (passed by value).
+struct KArgs {
+  unsigned N;   // samples per window
+  unsigned H;   // haplotypes per panel
+  unsigned C;   // MH proposals per sample
+  unsigned k0;  // Philox counter of proposal 0
+  float pen;
+  float norm;   // powf(FLT_MIN, 2/3) (impute.cpp:11-13)
+  uint64_t seed;
+  unsigned gen;
+  int prop_mode;
+  const unsigned *klist;
+  unsigned K;
+  const unsigned *extra;
+  double *trace_like;
+  uint8_t *trace_acc;
+  const void *lut;       // device emission table
+  unsigned lut_entries;  // 16 * 2^bits
+  unsigned bits;
+  int precision;
+};
+
+int set_error(int code, const char *msg);
+int hip_check(hipError_t err, const char *what);
+
+int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
+                 unsigned step, hipStream_t stream);
+int launch_batched(const KArgs &a, const glp_window *d_wins,
+                   const uint2 *d_tasks, unsigned ntasks,
+                   unsigned long long *d_ctrs, unsigned Q, unsigned grid,
+                   hipStream_t stream);
+int batched_occupancy(unsigned bits, bool mixed, unsigned Q, size_t smem,
+                      int *blocks_per_cu);
+int launch_eval(const KArgs &a, const glp_window *d_wins,
+                const unsigned *d_samples, const unsigned *d_parents,
+                unsigned n, float *d_out, hipStream_t stream);
+
+// Host emission tables (compiled with -ffp-contract=off, so the values equal
+// the CPU reference's float arithmetic).
+void emit_table(const float *codebook, unsigned ncode, const float *pc16,
+                float *E);
+void nibble_table(const float *E, unsigned ncode, float *out);
+
+}  // namespace glp

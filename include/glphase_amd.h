@@ -1,0 +1,192 @@
+/*
+ * glphase_amd.h -- C ABI of the gfx950 GLPhase MH surrogate-parent engine.
+ *
+ * Plain pointers and sizes only.  Every call returns GLP_OK (0) or a
+ * GLP_ERR_* code; glp_last_error() gives the calling thread's last message.
+ * No call exits the process (the reference's device layer calls
+ * exit(EXIT_FAILURE), hmmLike.cu:240,252,261,272,286,476,544).
+ *
+ * Two layers:
+ *  1. The reference-compatible layer mirrors namespace HMMLikeCUDA
+ *     (hmmLike.h:23-38): host buffers in, synchronous, one context per
+ *     HMMLike object instead of process-global device state
+ *     (hmmLike.cu:8-10,357-383).
+ *  2. The device-resident engine takes device pointers and a hipStream_t
+ *     (passed as void*), runs asynchronously and is what the benchmark times.
+ *
+ * Conventions shared with the reference:
+ *  - haplotype h, site s is bit (s & 63) of word h*wn + (s >> 6), with
+ *    wn = ceil(M / 64) (impute.hpp:53-66, insti.cpp:1204);
+ *  - packed GL codes are site-major, sample-minor uint32 words, MSB-first,
+ *    32/bits codes per word (glPack.cpp:112-192);
+ *  - parent slots 0,1 are the father pair and 2,3 the mother pair
+ *    (hmmLike.cu:61-63, impute.cpp:331-332);
+ *  - tran is [M][3] = ((1-r)^2, r(1-r), r^2) per site, tran[0..2] unused
+ *    (insti.cpp:1277-1300);
+ *  - pc is the 4x4 mutation matrix, row-major pc[i][j] (insti.cpp:1307-1315).
+ */
+#ifndef GLPHASE_AMD_H
+#define GLPHASE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GLP_OK = 0,
+  GLP_ERR_ARG = 1,   /* invalid argument or size mismatch */
+  GLP_ERR_HIP = 2,   /* HIP runtime error */
+  GLP_ERR_STATE = 3, /* call made before the state it needs was set */
+  GLP_ERR_NODEV = 4  /* no usable gfx950 device */
+};
+
+/* proposal modes (glp_mh_desc.prop_mode) */
+enum { GLP_PROP_UNIFORM = 0, GLP_PROP_KLIST = 1, GLP_PROP_HOST = 2 };
+/* arithmetic (glp_mh_desc.precision) */
+enum { GLP_PREC_F32 = 0, GLP_PREC_MIXED = 1, GLP_PREC_F16LUT = 2 };
+
+const char *glp_last_error(void);
+const char *glp_version(void);
+
+typedef struct glp_ctx glp_ctx;
+
+/* ===== 1. reference-compatible layer ===================================== */
+
+/* HMMLikeCUDA::CheckDevice (hmmLike.h:24, hmmLike.cu:266-320): counts
+ * devices and requires at least one gfx950 device. */
+int glp_check_device(int *num_devices);
+
+/* HMMLikeCUDA::SetDevice (hmmLike.h:25, hmmLike.cu:244-264) plus the state
+ * the reference keeps in process globals; norm = powf(FLT_MIN, 2/3)
+ * (hmmLike.cu:233). */
+int glp_create(int dev_id, glp_ctx **out);
+
+/* HMMLikeCUDA::Cleanup (hmmLike.h:37, hmmLike.cu:385-397); frees this
+ * context only (no device reset). */
+int glp_destroy(glp_ctx *ctx);
+
+/* HMMLikeCUDA::CopyTranToDevice (hmmLike.h:26, hmmLike.cu:322-337).
+ * n = 3 * num_sites floats. */
+int glp_set_tran(glp_ctx *ctx, const float *tran, size_t n);
+
+/* HMMLikeCUDA::CopyMutationMatToDevice (hmmLike.h:27, hmmLike.cu:339-355).
+ * pc16 is row-major pc[i][j]. */
+int glp_set_mutation_mat(glp_ctx *ctx, const float *pc16);
+
+/* HMMLikeCUDA::CopyCodeBookToDevice (hmmLike.h:29-30, hmmLike.cu:369-383).
+ * pairs = n_codes (RR, Het) float pairs; n_codes = 2^bits, bits in 1,2,4,8. */
+int glp_set_codebook(glp_ctx *ctx, const float *pairs, unsigned n_codes);
+
+/* HMMLikeCUDA::CopyPackedGLsToDevice (hmmLike.h:28, hmmLike.cu:357-366). */
+int glp_set_packed_gls(glp_ctx *ctx, const uint32_t *packed, size_t n_words,
+                       unsigned bits_per_code);
+
+/* MH penalty for the accept test u < exp((prop - curr) * pen).  The
+ * reference CUDA kernel fixes it at 1 (hmmLike.cu:159,196); its CPU MH loop
+ * uses the annealing penalty (insti.cpp:1670-1671, MHSampler.hpp:92).
+ * Default 1. */
+int glp_set_pen(glp_ctx *ctx, float pen);
+
+/* HMMLikeCUDA::RunHMMOnDevice (hmmLike.h:31-35, hmmLike.cu:399-572).
+ * hap_idxs: in = initial parents, out = chosen parents, both [4][num_samples]
+ * (slot-major, as the reference).  extra_prop_haps: [num_cycles][num_samples]
+ * candidate per cycle.  The replace slot and the accept uniform of cycle c of
+ * sample i are Philox draws keyed by seed, counter {c, i, 0, 0}. */
+int glp_run_hmm_on_device(glp_ctx *ctx, const uint64_t *hap_panel,
+                          size_t n_panel_words, const unsigned *extra_prop_haps,
+                          unsigned num_sites, unsigned num_samples,
+                          unsigned num_cycles, unsigned *hap_idxs,
+                          unsigned long seed);
+
+/* Impute::hmm_like / Insti::hmm_like (impute.hpp:80, impute.cpp:327-375,
+ * insti.cpp:97), batched: out[e] = log-likelihood of sample samples[e] given
+ * the 4 parents parents[4e..4e+3], with emissions decoded from the packed
+ * codes through the codebook (hmmLike.cu:53-142). */
+int glp_hmm_like(glp_ctx *ctx, const uint64_t *hap_panel, size_t n_panel_words,
+                 unsigned num_sites, unsigned num_samples, unsigned n_evals,
+                 const unsigned *samples, const unsigned *parents, float *out);
+
+/* ===== 2. device-resident engine ========================================= */
+
+/* One window (a run of sites scored together).  All pointers are device
+ * pointers. */
+typedef struct {
+  unsigned num_sites;     /* M */
+  unsigned words_per_hap; /* ceil(M / 64) */
+  const uint64_t *panel;  /* [num_haps][words_per_hap] */
+  const uint32_t *packed; /* [ceil(M * bits / 32)][num_samples] */
+  const float *tran;      /* [M][3] */
+  unsigned *parents;      /* [num_samples][4]; in (GLP_PROP_HOST) and out */
+  double *like;           /* [num_samples] out: log-likelihood of parents */
+} glp_window;
+
+typedef struct {
+  unsigned num_windows;
+  const glp_window *windows; /* host array of num_windows */
+  unsigned num_samples;      /* samples per window */
+  unsigned num_haps;         /* haplotypes per panel (2N + reference) */
+  unsigned bits_per_code;    /* 4 or 8 */
+  const void *emit_lut;      /* device table built by glp_build_emit_lut */
+  unsigned num_cycles;       /* MH proposals per sample (C) */
+  unsigned k0;               /* proposal c uses Philox counter k0 + c */
+  float pen;
+  uint64_t seed;
+  unsigned gen;
+  int prop_mode;           /* GLP_PROP_* */
+  const unsigned *klist;   /* GLP_PROP_KLIST: device [num_windows][N][K] */
+  unsigned K;
+  const unsigned *extra;   /* GLP_PROP_HOST: device [num_windows][C][N] */
+  int precision;           /* GLP_PREC_* */
+  double *trace_like;      /* optional device [num_windows][N][C+1] */
+  uint8_t *trace_acc;      /* optional device [num_windows][N][C] */
+} glp_mh_desc;
+
+/* Bytes of the device emission table for codes of `bits` bits. */
+size_t glp_emit_lut_bytes(unsigned bits, int precision);
+
+/* Builds the emission table FillEmit(codebook[c]) (hmmLike.cu:12-44) for
+ * every code and every combination of the 4 parents' alleles, and copies it
+ * to lut_dev (device, glp_emit_lut_bytes bytes).  codebook and pc16 are host
+ * arrays. */
+int glp_build_emit_lut(const float *codebook, unsigned n_codes,
+                       const float *pc16, int precision, void *lut_dev);
+
+/* Plain path, step 0: draws 4 fresh parents per sample (insti.cpp:1493-1494;
+ * kept from the input under GLP_PROP_HOST) and scores them. */
+int glp_plain_init(const glp_mh_desc *desc, void *stream);
+
+/* Plain path: one launch = one MH iteration = proposal c of every sample of
+ * every window (findHapSet's cycle body, hmmLike.cu:185-207, with the CPU
+ * accept rule of MHSampler.hpp:82-96). */
+int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
+
+/* Batched form: a persistent work-queue kernel runs init + all C proposals of
+ * every (window, sample) chain; lanes_per_chain lanes evaluate consecutive
+ * proposals of one chain speculatively from the same state, and a sequential
+ * walk keeps exactly the serial MH result.  lanes_per_chain is one of
+ * 1,2,4,8,16,32,64 (0 = automatic). */
+typedef struct glp_plan glp_plan;
+int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain,
+                    glp_plan **out);
+/* Runs one generation with the given gen / pen / seed (other fields from
+ * create).  Stream-ordered; capturable into a HIP graph. */
+int glp_plan_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
+                 void *stream);
+/* Counters of the last completed run: forward passes evaluated (useful and
+ * speculative) and rounds.  Synchronises the plan's stream. */
+int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
+/* Lanes per chain and grid size the plan launches with. */
+int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
+/* Plain path over the plan's windows: one launch for the fresh parents, then
+ * one launch per MH iteration (num_cycles launches), all stream-ordered. */
+int glp_plan_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
+                       void *stream);
+int glp_plan_destroy(glp_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

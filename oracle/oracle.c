@@ -1,0 +1,720 @@
+/*
+ * oracle.c -- CPU restatement of GLPhase's MH surrogate-parent update.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the timed CPU
+ * baseline.  Compiled by oracle/Makefile with -ffp-contract=off.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10, from Salmon, Moraes, Dror & Shaw, "Parallel random numbers: */
+/* as easy as 1, 2, 3", SC'11 (the Random123 family).  Replaces curand XORWOW */
+/* (hmmLike.cu:220-226) and the GSL streams (insti.cpp:1506, MHSampler.hpp:92) */
+/* ------------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += PHILOX_W0;
+      k1 += PHILOX_W1;
+    }
+    uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+void orc_draw(uint64_t seed, uint32_t k, uint32_t sample, uint32_t window,
+              uint32_t gen, uint32_t out[4]) {
+  const uint32_t ctr[4] = {k, sample, window, gen};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  orc_philox(ctr, key, out);
+}
+
+uint32_t orc_uniform_int(uint32_t x, uint32_t n) {
+  return (uint32_t)(((uint64_t)x * n) >> 32);
+}
+
+/* UnifSampler::SampleHap (sampler.cpp:29-39) draws uniformly over [0, H)
+ * rejecting h/2 == sample; the same distribution in closed form. */
+uint32_t orc_unif_excl(uint32_t x, uint32_t nhaps, uint32_t sample) {
+  uint32_t r = orc_uniform_int(x, nhaps - 2);
+  return r + (r >= 2 * sample ? 2u : 0u);
+}
+
+static double orc_u01(uint32_t x) { return (double)x * (1.0 / 4294967296.0); }
+
+/* ------------------------------------------------------------------------ */
+/* decode + emission                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* impute.cpp:11-13 and hmmLike.cu:233 */
+float orc_norm(void) { return powf(FLT_MIN, 2.0f / 3.0f); }
+
+/* hmmLike.cu:12-28: code j of a word is MSB-first; GL2 = max(1-GL0-GL1, 0) */
+void orc_unpack_gl(uint32_t word, const float *codebook, unsigned bits,
+                   unsigned slot, float gl[3]) {
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint32_t code = (word >> (32u - bits * slot - bits)) & mask;
+  gl[0] = codebook[2 * code];
+  gl[1] = codebook[2 * code + 1];
+  float g2 = 1 - gl[0] - gl[1];
+  gl[2] = g2 > 0.0f ? g2 : 0.0f;
+}
+
+/* hmmLike.cu:39-44 (FillEmit with mutationMat[i + 4j] = pc[i][j],
+ * hmmLike.cu:341-344); the het GL is counted twice (insti.cpp:1376-1378). */
+void orc_emit_lut(const float *codebook, unsigned ncode, const float *pc,
+                  float *lut) {
+  for (unsigned c = 0; c < ncode; ++c) {
+    float gl[3];
+    gl[0] = codebook[2 * c];
+    gl[1] = codebook[2 * c + 1];
+    float g2 = 1 - gl[0] - gl[1];
+    gl[2] = g2 > 0.0f ? g2 : 0.0f;
+    for (unsigned i = 0; i < 4; ++i)
+      lut[4 * c + i] = pc[4 * i + 0] * gl[0] + pc[4 * i + 1] * gl[1] +
+                       pc[4 * i + 2] * gl[1] + pc[4 * i + 3] * gl[2];
+  }
+}
+
+/* hmmLike.cu:100-104: word (site / k) of sample i sits at [site/k][N] */
+void orc_sample_emit(const uint32_t *packed, unsigned nsamp_packed,
+                     unsigned sample, unsigned M, unsigned bits,
+                     const float *lut, float *emit) {
+  const unsigned k = 32u / bits;
+  const uint32_t mask = (1u << bits) - 1u;
+  for (unsigned m = 0; m < M; ++m) {
+    uint32_t w = packed[(size_t)(m / k) * nsamp_packed + sample];
+    unsigned slot = m % k;
+    uint32_t code = (w >> (32u - bits * slot - bits)) & mask;
+    memcpy(emit + 4 * (size_t)m, lut + 4 * (size_t)code, 4 * sizeof(float));
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* forward pass                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* impute.hpp:64-66 / hmmLike.cu:47-49: site s of a hap is bit s&63 of word s>>6 */
+static inline unsigned hbit(const uint64_t *h, unsigned s) {
+  return (unsigned)((h[s >> 6] >> (s & 63)) & 1u);
+}
+
+/* The reference calls logf (impute.cpp:349,373).  glibc's logf is not
+ * correctly rounded (it differs by 1 ulp on ~0.03% of inputs), so the oracle
+ * fixes logf as the correctly rounded natural log: (float)log((double)x),
+ * except for the 20 positive floats whose log lies so close to a float
+ * rounding midpoint that a 2-ulp change of the double log flips the float
+ * (found by exhaustive search; values from an 80-digit decimal log). */
+static const uint32_t orc_logf_hard[20][2] = {
+    {0x0dc8bba4u, 0xc289bac4u}, {0x111c87f8u, 0xc28085dfu},
+    {0x1a8446cbu, 0xc24d0a45u}, {0x1f116ab8u, 0xc233b53eu},
+    {0x29fd22f8u, 0xc1ee8859u}, {0x2c4c24b7u, 0xc1d48710u},
+    {0x38dcbe38u, 0xc1128ba4u}, {0x3bf86ef0u, 0xc09c399eu},
+    {0x3c413d3au, 0xc08e158fu}, {0x41178febu, 0x400fe5e7u},
+    {0x464d5b2bu, 0x4117bcf1u}, {0x4665a9a6u, 0x41198725u},
+    {0x4c5d65a5u, 0x418f034bu}, {0x4d604ebeu, 0x419a352cu},
+    {0x4e85f412u, 0x41a6b811u}, {0x5ee8984eu, 0x422e4a21u},
+    {0x65d890d3u, 0x4254d1f9u}, {0x66a8c860u, 0x42595e46u},
+    {0x6f31a8ecu, 0x42845a89u}, {0x79e7ec37u, 0x42a1ffb7u}};
+
+float orc_logf_cr(float x) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  for (int i = 0; i < 20; ++i)
+    if (orc_logf_hard[i][0] == b) {
+      float r;
+      memcpy(&r, &orc_logf_hard[i][1], 4);
+      return r;
+    }
+  return (float)log((double)x);
+}
+
+/* Impute::hmm_like, impute.cpp:327-375, in its own operation order. */
+float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                       const float *emit, const float *tran, unsigned M) {
+  const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
+                 *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
+  const float norm = orc_norm();
+  const float *e = emit, *t = tran;
+  float sum, score = 0;
+  float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
+        l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
+  float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
+        l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
+  float b00, b01, b10, b11;
+  e += 4;
+  t += 3;
+  for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
+    b00 = l00 * t[0] + (l01 + l10) * t[1] + l11 * t[2];
+    b01 = l01 * t[0] + (l00 + l11) * t[1] + l10 * t[2];
+    b10 = l10 * t[0] + (l00 + l11) * t[1] + l01 * t[2];
+    b11 = l11 * t[0] + (l01 + l10) * t[1] + l00 * t[2];
+    l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
+    l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
+    l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
+    l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
+    if ((sum = l00 + l01 + l10 + l11) < norm) {
+      sum = 1.0f / sum;
+      score -= orc_logf_cr(sum);
+      l00 *= sum;
+      l01 *= sum;
+      l10 *= sum;
+      l11 *= sum;
+    }
+  }
+  return score + orc_logf_cr(l00 + l01 + l10 + l11);
+}
+
+/* The same FP32 recurrence; the log of the scale factors and the final log
+ * are accumulated in FP64. */
+double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
+                          const unsigned P[4], const float *emit,
+                          const float *tran, unsigned M) {
+  const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
+                 *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
+  const float norm = orc_norm();
+  const float *e = emit, *t = tran;
+  float sum;
+  double score = 0;
+  float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
+        l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
+  float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
+        l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
+  float b00, b01, b10, b11;
+  e += 4;
+  t += 3;
+  for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
+    b00 = l00 * t[0] + (l01 + l10) * t[1] + l11 * t[2];
+    b01 = l01 * t[0] + (l00 + l11) * t[1] + l10 * t[2];
+    b10 = l10 * t[0] + (l00 + l11) * t[1] + l01 * t[2];
+    b11 = l11 * t[0] + (l01 + l10) * t[1] + l00 * t[2];
+    l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
+    l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
+    l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
+    l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
+    if ((sum = l00 + l01 + l10 + l11) < norm) {
+      sum = 1.0f / sum;
+      score -= log((double)sum);
+      l00 *= sum;
+      l01 *= sum;
+      l10 *= sum;
+      l11 *= sum;
+    }
+  }
+  return score + log((double)(l00 + l01 + l10 + l11));
+}
+
+/* FP64 forward oracle: the same model, normalised at every site. */
+double orc_hmm_like_f64(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                        const float *emit, const float *tran, unsigned M) {
+  const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
+                 *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
+  double l00 = 0.25 * emit[(hbit(f0, 0) << 1) | hbit(m0, 0)];
+  double l01 = 0.25 * emit[(hbit(f0, 0) << 1) | hbit(m1, 0)];
+  double l10 = 0.25 * emit[(hbit(f1, 0) << 1) | hbit(m0, 0)];
+  double l11 = 0.25 * emit[(hbit(f1, 0) << 1) | hbit(m1, 0)];
+  double s = l00 + l01 + l10 + l11, score = log(s);
+  l00 /= s;
+  l01 /= s;
+  l10 /= s;
+  l11 /= s;
+  for (unsigned m = 1; m < M; m++) {
+    const float *e = emit + 4 * (size_t)m, *t = tran + 3 * (size_t)m;
+    double t0 = t[0], t1 = t[1], t2 = t[2];
+    double b00 = l00 * t0 + (l01 + l10) * t1 + l11 * t2;
+    double b01 = l01 * t0 + (l00 + l11) * t1 + l10 * t2;
+    double b10 = l10 * t0 + (l00 + l11) * t1 + l01 * t2;
+    double b11 = l11 * t0 + (l01 + l10) * t1 + l00 * t2;
+    l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
+    l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
+    l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
+    l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
+    s = l00 + l01 + l10 + l11;
+    score += log(s);
+    l00 /= s;
+    l01 /= s;
+    l10 /= s;
+    l11 /= s;
+  }
+  return score;
+}
+
+/* Sum over all 4^M hidden paths in FP64.  State xy = (father x, mother y);
+ * a step keeps both (tran[0]), switches one (tran[1]) or both (tran[2]),
+ * the model of hmmLike.h:18-20 and hmmLike.cu:111-122. */
+double orc_hmm_like_brute(const uint64_t *haps, unsigned wn,
+                          const unsigned P[4], const float *emit,
+                          const float *tran, unsigned M) {
+  if (M == 0 || M > 12)
+    return NAN;
+  const uint64_t *par[4];
+  for (int i = 0; i < 4; ++i)
+    par[i] = haps + (size_t)P[i] * wn;
+  const uint64_t npaths = 1ull << (2 * M);
+  double total = 0;
+  for (uint64_t path = 0; path < npaths; ++path) {
+    double p = 0.25;
+    unsigned prev = 0;
+    for (unsigned m = 0; m < M; ++m) {
+      unsigned st = (unsigned)((path >> (2 * m)) & 3u);
+      unsigned x = st >> 1, y = st & 1u;
+      unsigned idx = (hbit(par[x], m) << 1) | hbit(par[2 + y], m);
+      if (m) {
+        unsigned px = prev >> 1, py = prev & 1u;
+        unsigned nsw = (px != x) + (py != y);
+        p *= (double)tran[3 * (size_t)m + nsw];
+      }
+      p *= (double)emit[4 * (size_t)m + idx];
+      prev = st;
+    }
+    total += p;
+  }
+  return log(total);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MH loop: Insti::solve (insti.cpp:1478-1551) with MHSampler::SampleHap      */
+/* (MHSampler.hpp:82-96) and fresh parents each generation (insti.cpp:1493).  */
+/* Accept when prop >= curr, else when u < exp((prop - curr) * pen) in FP64.  */
+/* ------------------------------------------------------------------------ */
+static double like_of(const orc_window *w, const unsigned P[4],
+                      const float *emit, int precision) {
+  if (precision == ORC_PREC_MIXED)
+    return orc_hmm_like_mixed(w->haps, w->wn, P, emit, w->tran, w->M);
+  return (double)orc_hmm_like_f32(w->haps, w->wn, P, emit, w->tran, w->M);
+}
+
+static void mh_one(const orc_window *w, const orc_mh_args *a, unsigned i,
+                   float *emit, unsigned *out_parents, double *out_curr,
+                   double *trace_like, uint8_t *trace_acc) {
+  unsigned P[4];
+  uint32_t x[4];
+  orc_sample_emit(w->packed, w->nsamp, i, w->M, w->bits, w->lut, emit);
+  if (a->prop_mode == ORC_PROP_HOST) {
+    for (int j = 0; j < 4; ++j)
+      P[j] = a->init_parents[4 * (size_t)i + j];
+  } else {
+    orc_draw(a->seed, ORC_INIT_TAG, i, a->window_id, a->gen, x);
+    for (int j = 0; j < 4; ++j)
+      P[j] = a->prop_mode == ORC_PROP_KLIST
+                 ? a->klist[(size_t)i * a->K + orc_uniform_int(x[j], a->K)]
+                 : orc_unif_excl(x[j], w->nhaps, i);
+  }
+  double curr = like_of(w, P, emit, a->precision);
+  if (trace_like)
+    trace_like[(size_t)i * (a->ncycles + 1)] = curr;
+  const double pen = (double)a->pen;
+  for (unsigned n = 0; n < a->ncycles; ++n) {
+    orc_draw(a->seed, a->k0 + n, i, a->window_id, a->gen, x);
+    unsigned rp = x[0] & 3u, oh = P[rp];
+    unsigned cand;
+    if (a->prop_mode == ORC_PROP_HOST)
+      cand = a->extra[(size_t)n * w->nsamp + i];
+    else if (a->prop_mode == ORC_PROP_KLIST)
+      cand = a->klist[(size_t)i * a->K + orc_uniform_int(x[1], a->K)];
+    else
+      cand = orc_unif_excl(x[1], w->nhaps, i);
+    P[rp] = cand;
+    double prop = like_of(w, P, emit, a->precision);
+    int acc = prop >= curr || orc_u01(x[2]) < exp((prop - curr) * pen);
+    if (acc)
+      curr = prop;
+    else
+      P[rp] = oh;
+    if (trace_like)
+      trace_like[(size_t)i * (a->ncycles + 1) + 1 + n] = prop;
+    if (trace_acc)
+      trace_acc[(size_t)i * a->ncycles + n] = (uint8_t)acc;
+  }
+  for (int j = 0; j < 4; ++j)
+    out_parents[4 * (size_t)i + j] = P[j];
+  if (out_curr)
+    out_curr[i] = curr;
+}
+
+void orc_mh_window(const orc_window *w, const orc_mh_args *a, unsigned s0,
+                   unsigned s1, unsigned *out_parents, double *out_curr,
+                   double *trace_like, uint8_t *trace_acc) {
+  float *emit = (float *)malloc(sizeof(float) * 4 * (size_t)w->M);
+  for (unsigned i = s0; i < s1; ++i)
+    mh_one(w, a, i, emit, out_parents, out_curr, trace_like, trace_acc);
+  free(emit);
+}
+
+void orc_mh_window_omp(const orc_window *w, const orc_mh_args *a, unsigned s0,
+                       unsigned s1, unsigned *out_parents, double *out_curr,
+                       int nthreads) {
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+  {
+    float *emit = (float *)malloc(sizeof(float) * 4 * (size_t)w->M);
+#pragma omp for schedule(dynamic, 4)
+    for (long i = (long)s0; i < (long)s1; ++i)
+      mh_one(w, a, (unsigned)i, emit, out_parents, out_curr, NULL, NULL);
+    free(emit);
+  }
+#else
+  (void)nthreads;
+  orc_mh_window(w, a, s0, s1, out_parents, out_curr, NULL, NULL);
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* GL vector quantisation: GLVQ (glVQ.cpp:26-165).  The GSL draws of the     */
+/* reference are replaced by Philox draws under key `seed`, counter          */
+/* {n, tag, 0, 0}: tag 1 = random initial codewords, 2 = training subsample,  */
+/* 3 = re-seeding of empty clusters.                                         */
+/* ------------------------------------------------------------------------ */
+
+/* EuclidianDist (glVQ.hpp:23-34): sqrt(pow(float diff, 2) + ...) in FP64;
+ * std::min_element keeps the first of equal distances. */
+static inline double vq_dist(float cr, float ch, float RR, float Het) {
+  double dr = (double)(cr - RR), dh = (double)(ch - Het);
+  return sqrt(dr * dr + dh * dh);
+}
+
+unsigned orc_vq_find(const float *codebook, unsigned ncode, float RR,
+                     float Het) {
+  unsigned best = 0;
+  double bd = vq_dist(codebook[0], codebook[1], RR, Het);
+  for (unsigned c = 1; c < ncode; ++c) {
+    double d = vq_dist(codebook[2 * c], codebook[2 * c + 1], RR, Het);
+    if (d < bd) {
+      bd = d;
+      best = c;
+    }
+  }
+  return best;
+}
+
+static void vq_draw(uint64_t seed, uint32_t n, uint32_t tag, uint32_t out[4]) {
+  const uint32_t ctr[4] = {n, tag, 0, 0};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  orc_philox(ctr, key, out);
+}
+
+/* AssignPoints (glVQ.cpp:149-163) */
+static double vq_assign(const float *pts, size_t n, const float *cb,
+                        unsigned ncode, uint8_t *lab) {
+  double var = 0;
+  for (size_t p = 0; p < n; ++p) {
+    float RR = pts[2 * p], Het = pts[2 * p + 1];
+    unsigned c = orc_vq_find(cb, ncode, RR, Het);
+    lab[p] = (uint8_t)c;
+    double dr = (double)(RR - cb[2 * c]), dh = (double)(Het - cb[2 * c + 1]);
+    var += dr * dr + dh * dh;
+  }
+  return var;
+}
+
+/* UpdateCodeBook (glVQ.cpp:165-211): float sums in point order */
+static void vq_update(const float *pts, size_t n, const uint8_t *lab,
+                      unsigned ncode, uint64_t seed, uint32_t *reseed,
+                      float *cb) {
+  float *s = (float *)calloc(2 * (size_t)ncode, sizeof(float));
+  size_t *cnt = (size_t *)calloc(ncode, sizeof(size_t));
+  for (size_t p = 0; p < n; ++p) {
+    s[2 * lab[p]] += pts[2 * p];
+    s[2 * lab[p] + 1] += pts[2 * p + 1];
+    ++cnt[lab[p]];
+  }
+  for (unsigned c = 0; c < ncode; ++c) {
+    if (cnt[c] > 0) {
+      cb[2 * c] = s[2 * c] / (float)cnt[c];
+      cb[2 * c + 1] = s[2 * c + 1] / (float)cnt[c];
+    } else {
+      uint32_t x[4];
+      vq_draw(seed, (*reseed)++, 3, x);
+      size_t p = (size_t)(((uint64_t)x[0] * n) >> 32);
+      cb[2 * c] = pts[2 * p];
+      cb[2 * c + 1] = pts[2 * p + 1];
+    }
+  }
+  free(s);
+  free(cnt);
+}
+
+/* BuildCodeBook (glVQ.cpp:26-146).  Returns 0, or -1 when Lloyd iterations
+ * exceed 1000 (the reference throws). */
+int orc_vq_train(const float *gls, size_t ngl, unsigned ncode, uint64_t seed,
+                 size_t training_size, float *cb, unsigned *iters) {
+  if (ncode < 4 || ncode > 256 || ngl == 0)
+    return -2;
+  cb[0] = 0;
+  cb[1] = 1;
+  cb[2] = 0;
+  cb[3] = 0;
+  cb[4] = 1;
+  cb[5] = 0;
+  cb[6] = (float)(1.0 / 3);
+  cb[7] = (float)(1.0 / 3);
+  for (unsigned c = 4; c < ncode; ++c) {
+    uint32_t x[4];
+    vq_draw(seed, c, 1, x);
+    const float RR = (float)orc_u01(x[0]);
+    const float Het = (float)(orc_u01(x[1]) * (1 - RR));
+    cb[2 * c] = RR;
+    cb[2 * c + 1] = Het;
+  }
+  size_t n = training_size && training_size < ngl ? training_size : ngl;
+  float *pts = (float *)malloc(sizeof(float) * 2 * n);
+  for (size_t it = 0; it < n; ++it) {
+    size_t g = it;
+    if (n < ngl) {
+      uint32_t x[4];
+      vq_draw(seed, (uint32_t)it, 2, x);
+      g = (size_t)(((uint64_t)x[0] * ngl) >> 32);
+    }
+    float RR = gls[3 * g], Het = gls[3 * g + 1];
+    float sum = gls[3 * g + 2] + RR + Het;
+    RR /= sum;
+    Het /= sum;
+    pts[2 * it] = RR;
+    pts[2 * it + 1] = Het;
+  }
+  uint8_t *lab = (uint8_t *)malloc(n);
+  uint32_t reseed = 0;
+  double prev = DBL_MAX, cur = vq_assign(pts, n, cb, ncode, lab);
+  unsigned iter = 0;
+  int rc = 0;
+  while (fabs(cur - prev) > cur / 100) {
+    if (++iter > 1000) {
+      rc = -1;
+      break;
+    }
+    prev = cur;
+    vq_update(pts, n, lab, ncode, seed, &reseed, cb);
+    cur = vq_assign(pts, n, cb, ncode, lab);
+  }
+  if (iters)
+    *iters = iter;
+  free(pts);
+  free(lab);
+  return rc;
+}
+
+/* GLPack::GLs2VQChar (glPack.cpp:195-219): normalise (sum = RR + Het + AA)
+ * then FindGLCode */
+void orc_vq_codes(const float *gls, size_t ngl, const float *codebook,
+                  unsigned ncode, uint8_t *codes) {
+  for (size_t g = 0; g < ngl; ++g) {
+    float RR = gls[3 * g], Het = gls[3 * g + 1];
+    float sum = RR + Het + gls[3 * g + 2];
+    RR /= sum;
+    Het /= sum;
+    codes[g] = (uint8_t)orc_vq_find(codebook, ncode, RR, Het);
+  }
+}
+
+/* GLPack::GetPackedGLs + GLIdxs2uint32_t (glPack.cpp:32-85,87-139): site-major,
+ * sample-minor words, MSB-first codes; a site past the end takes GL index 0,
+ * i.e. the code of sample 0, site 0. codes are [nsamp][M]. */
+void orc_pack_codes(const uint8_t *codes, unsigned nsamp, unsigned M,
+                    unsigned bits, unsigned s0, unsigned s1, uint32_t *out) {
+  (void)nsamp;
+  const unsigned k = 32u / bits;
+  const unsigned nw = (M + k - 1) / k, ns = s1 - s0;
+  for (unsigned w = 0; w < nw; ++w)
+    for (unsigned i = s0; i < s1; ++i) {
+      uint32_t word = 0;
+      for (unsigned j = 0; j < k; ++j) {
+        unsigned m = w * k + j;
+        uint32_t c = m < M ? codes[(size_t)i * M + m] : codes[0];
+        word |= c << (32u - bits * (j + 1));
+      }
+      out[(size_t)w * ns + (i - s0)] = word;
+    }
+}
+
+/* GLPack::GLTrio2Char / GL2HalfChar (glPack.cpp:221-243): two 4-bit halves */
+static unsigned half_char(float gl) {
+  gl = gl - FLT_EPSILON;
+  gl = gl > 0.0f ? gl : 0.0f;
+  return (unsigned)floorf(gl * 16u);
+}
+
+static unsigned trio_char(const float *g) {
+  float sum = g[0] + g[1] + g[2];
+  return (half_char(g[0] / sum) << 4) | half_char(g[1] / sum);
+}
+
+void orc_pack_nonvq(const float *gls, unsigned nsamp, unsigned M, unsigned s0,
+                    unsigned s1, uint32_t *out) {
+  (void)nsamp;
+  const unsigned nw = (M + 3) / 4, ns = s1 - s0;
+  for (unsigned w = 0; w < nw; ++w)
+    for (unsigned i = s0; i < s1; ++i) {
+      uint32_t word = 0;
+      for (unsigned j = 0; j < 4; ++j) {
+        unsigned m = w * 4 + j;
+        const float *g = m < M ? gls + 3 * ((size_t)i * M + m) : gls;
+        word ^= trio_char(g) << (24u - 8u * j);
+      }
+      out[(size_t)w * ns + (i - s0)] = word;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* parameters: insti.cpp:1254-1315                                          */
+/* ------------------------------------------------------------------------ */
+float orc_mu(unsigned hn) {
+  float mu = 0;
+  for (unsigned i = 1; i < hn; i++)
+    mu += 1.0 / i;
+  mu = 1 / mu;
+  return mu / (hn + mu);
+}
+
+void orc_pc(float mu, float *pc) {
+  const float d = (1 - mu) * (1 - mu), o = mu * (1 - mu), a = mu * mu;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      int flips = ((i ^ j) & 1) + (((i ^ j) >> 1) & 1);
+      pc[4 * i + j] = flips == 0 ? d : (flips == 1 ? o : a);
+    }
+}
+
+/* insti.cpp:1277-1300 with r from a genetic map (Morgans); tran[0..2] unused */
+void orc_tran_from_morgans(const double *r_morgans, unsigned M,
+                           float inflation, float *tran) {
+  tran[0] = tran[1] = tran[2] = 0;
+  for (unsigned m = M - 1; m; m--) {
+    float r = (float)r_morgans[m];
+    r *= inflation;
+    r = r > 1 ? 1 : r;
+    tran[m * 3] = (1 - r) * (1 - r);
+    tran[m * 3 + 1] = r * (1 - r);
+    tran[m * 3 + 2] = r * r;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* hmm_work: Impute::hmm_work (impute.cpp:380-485).  The per-site uniform of   */
+/* gsl_rng_uniform becomes Philox counter {0x80000000 | m>>2, sample, window, */
+/* gen}, word m & 3.  Bits of ha/hb past M are left zero.                     */
+/* ------------------------------------------------------------------------ */
+static inline void set_bit(uint64_t *h, unsigned s, unsigned v) {
+  if (v)
+    h[s >> 6] |= (uint64_t)1 << (s & 63);
+  else
+    h[s >> 6] &= ~((uint64_t)1 << (s & 63));
+}
+
+void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                  const float *emit, const float *tran, const float *prob,
+                  const float *pc, float S, unsigned M, uint64_t seed,
+                  uint32_t sample, uint32_t window, uint32_t gen, uint64_t *ha,
+                  uint64_t *hb) {
+  const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
+                 *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
+  float *beta = (float *)malloc(sizeof(float) * 4 * (size_t)M);
+  const float *e = emit + 4 * (size_t)(M - 1), *t = tran + 3 * (size_t)(M - 1);
+  float *b = beta + 4 * (size_t)(M - 1), sum;
+  float l00 = 0, l01 = 0, l10 = 0, l11 = 0, b00, b01, b10, b11;
+  b[0] = b[1] = b[2] = b[3] = 1;
+  for (unsigned m = M - 1; m; m--, e -= 4, t -= 3) {
+    b00 = b[0] * e[(hbit(f0, m) << 1) | hbit(m0, m)];
+    b01 = b[1] * e[(hbit(f0, m) << 1) | hbit(m1, m)];
+    b10 = b[2] * e[(hbit(f1, m) << 1) | hbit(m0, m)];
+    b11 = b[3] * e[(hbit(f1, m) << 1) | hbit(m1, m)];
+    b -= 4;
+    b[0] = b00 * t[0] + (b01 + b10) * t[1] + b11 * t[2];
+    b[1] = b01 * t[0] + (b00 + b11) * t[1] + b10 * t[2];
+    b[2] = b10 * t[0] + (b00 + b11) * t[1] + b01 * t[2];
+    b[3] = b11 * t[0] + (b01 + b10) * t[1] + b00 * t[2];
+    sum = 1.0f / (b[0] + b[1] + b[2] + b[3]);
+    b[0] *= sum;
+    b[1] *= sum;
+    b[2] *= sum;
+    b[3] *= sum;
+  }
+  for (unsigned w = 0; w < wn; ++w)
+    ha[w] = hb[w] = 0;
+  const float *p = prob;
+  e = emit;
+  t = tran;
+  b = beta;
+  uint32_t x[4] = {0, 0, 0, 0};
+  for (unsigned m = 0; m < M; m++, e += 4, t += 3, p += 3, b += 4) {
+    unsigned s00 = (hbit(f0, m) << 1) | hbit(m0, m);
+    unsigned s01 = (hbit(f0, m) << 1) | hbit(m1, m);
+    unsigned s10 = (hbit(f1, m) << 1) | hbit(m0, m);
+    unsigned s11 = (hbit(f1, m) << 1) | hbit(m1, m);
+    if (m) {
+      b00 = l00 * t[0] + l01 * t[1] + l10 * t[1] + l11 * t[2],
+      b01 = l00 * t[1] + l01 * t[0] + l10 * t[2] + l11 * t[1];
+      b10 = l00 * t[1] + l01 * t[2] + l10 * t[0] + l11 * t[1],
+      b11 = l00 * t[2] + l01 * t[1] + l10 * t[1] + l11 * t[0];
+      l00 = b00 * e[s00];
+      l01 = b01 * e[s01];
+      l10 = b10 * e[s10];
+      l11 = b11 * e[s11];
+    } else {
+      l00 = 0.25f * e[s00];
+      l01 = 0.25f * e[s01];
+      l10 = 0.25f * e[s10];
+      l11 = 0.25f * e[s11];
+    }
+    sum = 1.0f / (l00 + l01 + l10 + l11);
+    l00 *= sum;
+    l01 *= sum;
+    l10 *= sum;
+    l11 *= sum;
+    float p00 = l00 * b[0], p01 = l01 * b[1], p10 = l10 * b[2],
+          p11 = l11 * b[3];
+    float c00 = powf(p[0] * (p00 * pc[4 * s00 + 0] + p01 * pc[4 * s01 + 0] +
+                             p10 * pc[4 * s10 + 0] + p11 * pc[4 * s11 + 0]),
+                     S);
+    float c01 = powf(p[1] * (p00 * pc[4 * s00 + 1] + p01 * pc[4 * s01 + 1] +
+                             p10 * pc[4 * s10 + 1] + p11 * pc[4 * s11 + 1]),
+                     S);
+    float c10 = powf(p[1] * (p00 * pc[4 * s00 + 2] + p01 * pc[4 * s01 + 2] +
+                             p10 * pc[4 * s10 + 2] + p11 * pc[4 * s11 + 2]),
+                     S);
+    float c11 = powf(p[2] * (p00 * pc[4 * s00 + 3] + p01 * pc[4 * s01 + 3] +
+                             p10 * pc[4 * s10 + 3] + p11 * pc[4 * s11 + 3]),
+                     S);
+    if ((m & 3u) == 0)
+      orc_draw(seed, 0x80000000u | (m >> 2), sample, window, gen, x);
+    sum = (float)(orc_u01(x[m & 3u]) * (c00 + c01 + c10 + c11));
+    if (sum < c00) {
+      set_bit(ha, m, 0);
+      set_bit(hb, m, 0);
+    } else if (sum < c00 + c01) {
+      set_bit(ha, m, 0);
+      set_bit(hb, m, 1);
+    } else if (sum < c00 + c01 + c10) {
+      set_bit(ha, m, 1);
+      set_bit(hb, m, 0);
+    } else {
+      set_bit(ha, m, 1);
+      set_bit(hb, m, 1);
+    }
+  }
+  free(beta);
+}
