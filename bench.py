@@ -1,0 +1,188 @@
+"""Benchmark of the GLPhase MH surrogate-parent update on one MI355X.
+
+A step is one MH generation over every sample: 4 fresh parents scored, then
+C proposals each scored and accepted or rejected (Insti::estimate's per-
+generation work, insti.cpp:1668-1705).  Site-evals (SE) per step =
+N * (C + 1) * M: one forward-recurrence step of one chain at one site.
+
+Default workload: 5,000 samples x 4,096 sites, 64 proposals per sample,
+synthetic founder-mosaic haplotypes and Poisson(2)-depth GLs, 8-bit VQ codes,
+batched persistent kernel with FP32 recurrence and FP64 log accumulation.
+The plain path (one launch per MH iteration) runs on the same inputs, and the
+CPU oracle (C, -O3) is timed on a bounded sample of the same chains.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_SE = 29      # 2 pair sums + 4 x (3 mul + 2 add) + 4 emission mul + 3 adds
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(N, M, C, depth, seed):
+    from glphase_amd import host, synth
+    t0 = time.time()
+    d = synth.make_dataset(N, M, depth=depth, seed=seed)
+    hb = synth.initial_haps(d["gls"], seed=seed + 1)
+    panel = synth.pack_haps(hb)
+    tran, pc, mu = host.model_params(2 * N, d["r_morgans"])
+    cb, iters = host.vq_train(d["gls"], 256, key=seed, training_size=1 << 18)
+    codes = host.vq_codes(d["gls"], cb)
+    packed = host.pack_codes(codes, 8)
+    log(f"workload N={N} M={M} C={C}: generated in {time.time() - t0:.1f}s (VQ {iters} iterations)")
+    return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, N=N, M=M,
+                H=panel.shape[0])
+
+
+def time_steps(torch, fn, steps, warmup, stream_events=True):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / 1e3, wall
+
+
+def cpu_baseline(w, C, pen, seed, budget_s=10.0, precision=1):
+    """The oracle (C restatement of Insti::solve) on this host, OpenMP over
+    samples on all cores and single-threaded; bounded sample of chains."""
+    from oracle import bind as orc
+    lut = orc.emit_lut(w["codebook"], w["pc"])
+    mh = orc.MH(w["panel"], w["packed"], 8, lut, w["tran"], w["N"])
+    se_per_chain = (C + 1) * w["M"]
+    # single core
+    n1 = 4
+    t0 = time.perf_counter()
+    mh.run(C, pen, seed, precision=precision, s0=0, s1=n1)
+    dt1 = time.perf_counter() - t0
+    n1 = int(min(w["N"], max(4, budget_s / 3 / max(dt1 / n1, 1e-6))))
+    t0 = time.perf_counter()
+    mh.run(C, pen, seed, precision=precision, s0=0, s1=n1)
+    dt1 = time.perf_counter() - t0
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
+    nall = int(min(w["N"], max(cores, n1 * cores * (budget_s * 2 / 3) / max(dt1, 1e-6) / 2)))
+    t0 = time.perf_counter()
+    mh.run(C, pen, seed, precision=precision, s0=0, s1=nall, nthreads=cores)
+    dta = time.perf_counter() - t0
+    return {
+        "value": se_per_chain * nall / dta, "unit": "site-evals/s", "cores": cores, "kind": "port",
+        "sample": f"{nall} of {w['N']} chains x {C + 1} evaluations x {w['M']} sites "
+                  f"({dta:.1f}s, OpenMP over samples)",
+        "single_core": {"value": se_per_chain * n1 / dt1, "cores": 1,
+                        "sample": f"{n1} chains ({dt1:.1f}s)"},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--samples", type=int, default=5000)
+    ap.add_argument("--sites", type=int, default=4096)
+    ap.add_argument("--cycles", type=int, default=64)
+    ap.add_argument("--depth", type=float, default=2.0)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per chain (0 = auto)")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from glphase_amd import PREC_MIXED
+    from glphase_amd.engine import Job
+
+    N, M, C = args.samples, args.sites, args.cycles
+    pen, seed = 1.0, 20240601
+    w = make_workload(N, M, C, args.depth, args.seed)
+    win = dict(panel=w["panel"], packed=w["packed"], tran=w["tran"])
+    job = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C,
+              precision=PREC_MIXED)
+    lanes, grid = job.make_plan(args.lanes)
+    se_step = N * (C + 1) * M
+    gen = [0]
+
+    def step():
+        job.run_batched(gen=gen[0], pen=pen, seed=seed)
+        gen[0] += 1
+
+    log(f"batched: lanes/chain={lanes} grid={grid}")
+    secs, wall = time_steps(torch, step, args.steps, args.warmup)
+    ms = secs / args.steps * 1e3
+    evals, useful = job.stats()
+    value = se_step * args.steps / secs
+    log(f"batched: {ms:.3f} ms/step, {value:.3e} SE/s (wall {wall:.3f}s); "
+        f"evaluated {evals} chain passes for {useful} useful")
+
+    # plain path on the same inputs: one launch per MH iteration
+    job_p = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED)
+    job_p.make_plan(1)
+    gp = [0]
+
+    def pstep():
+        job_p.run_plain(gen=gp[0], pen=pen, seed=seed)
+        gp[0] += 1
+
+    psecs, _ = time_steps(torch, pstep, args.plain_steps, 1)
+    pms = psecs / args.plain_steps * 1e3
+    pvalue = se_step * args.plain_steps / psecs
+    log(f"plain: {pms:.3f} ms/step, {pvalue:.3e} SE/s")
+
+    achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
+    out = {
+        "metric": "HMM site-evals/s (sample x proposal x site), batched MH generation",
+        "value": value,
+        "unit": "site-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 recurrence, fp64 log accumulation",
+        "data": "synthetic founder-mosaic haplotypes and GLs",
+        "config": {"workload": f"{N} samples x {M} sites, {C} proposals/sample, 8-bit VQ GLs, "
+                               "batched persistent kernel (mixed precision)",
+                   "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes},
+        "mh_iterations_per_s": 1e3 / ms,
+        "proposals_per_s": N * C / (ms / 1e3),
+        "plain": {"value": pvalue, "ms_per_step": pms, "launches_per_step": C + 1,
+                  "batched_over_plain": value / pvalue},
+        "speculation": {"chain_passes": evals, "useful_passes": useful},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_site_eval": FLOP_PER_SE},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,14 +15,14 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglphase_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["glp_kernels.hip", "glp_capi.cpp"]
+SOURCES = ["glp_kernels.hip", "glp_capi.cpp", "glp_host.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math",
     # packed FP32 VALU ops stay off: they have given silently wrong results
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
+    "-Wall", "-Wno-unused-result", "-Wno-pass-failed", "-I", os.path.join(ROOT, "include"),
 ]
 
 
@@ -40,6 +40,17 @@ def build(verbose=False):
     for o in objs:
         os.remove(o)
     return LIB
+
+
+def build_cpp_tests(verbose=False):
+    """The C++ drop-in test program tests/cpp/test_hmmlike (links the
+    library with an rpath to its directory)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_hmmlike.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_hmmlike")
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+           src, "-o", out, "-L", HERE, "-lglphase_amd", "-Wl,-rpath," + HERE]
+    _run(cmd, verbose)
+    return out
 
 
 def _run(cmd, verbose):

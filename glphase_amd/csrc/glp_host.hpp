@@ -55,6 +55,9 @@ class GLVQ {
   // BuildCodeBook (glVQ.cpp:26-146); training_size 0 = every GL (reference).
   GLVQ(const std::vector<float> &inGLs, Rng &rng, unsigned codeBookSize,
        size_t training_size = 0);
+  // the same with the Philox key given directly
+  GLVQ(const float *gls, size_t numGLs, uint64_t key, unsigned codeBookSize,
+       size_t training_size = 0);
   // FindGLCode (glVQ.cpp:156-165): first index of the smallest Euclidean
   // distance (compared in FP64 after sqrt).
   unsigned char FindGLCode(float RR, float Het) const;
@@ -64,7 +67,12 @@ class GLVQ {
  private:
   std::vector<std::pair<float, float>> m_codeBook;
   unsigned m_iters = 0;
+  void build(const float *gls, size_t numGLs, uint64_t key, unsigned codeBookSize,
+             size_t training_size);
 };
+
+unsigned nearest(const std::vector<std::pair<float, float>> &cb, float RR, float Het);
+unsigned trio_char(const float *gl);
 
 // ---- GL packing: glPack.hpp / glPack.cpp ------------------------------------
 struct GLPackInit {
@@ -126,6 +134,7 @@ class UnifSampler : public Sampler {
  public:
   UnifSampler(Rng &rng, unsigned numSamples, unsigned numHaps)
       : Sampler(rng, numSamples, numHaps) {}
+  using Sampler::SampleHap;
   unsigned SampleHap(unsigned excludeInd) override;
 };
 
@@ -134,6 +143,7 @@ class KListSampler : public Sampler {
  public:
   KListSampler(Rng &rng, unsigned numSamples, unsigned numHaps,
                std::vector<std::vector<unsigned>> neighbors);
+  using Sampler::SampleHap;
   unsigned SampleHap(unsigned excludeInd) override;
 
  private:

@@ -1,0 +1,178 @@
+"""Device-resident MH engine: inputs uploaded once into HBM, then the plain
+path (one launch per MH iteration) or the batched persistent kernel run on a
+HIP stream.  torch provides device memory and streams; all compute is the
+HIP code of libglphase_amd.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import GlpMhDesc, GlpWindow, c_f32p, c_u32p, c_u64p, check, lib, ptr
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.GlpError("no GPU visible: the glphase_amd engine runs only on a gfx950 device")
+    return torch
+
+
+def _dev(torch, a, device):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint64:
+        a = a.view(np.int64)
+    elif a.dtype == np.uint32:
+        a = a.view(np.int32)
+    return torch.from_numpy(a).to(device)
+
+
+class Job:
+    """MH update of N samples over W windows.
+
+    windows: list of dicts with 'panel' [H][wn] uint64, 'packed'
+    [ceil(M*bits/32)][N] uint32 and 'tran' [M][3] float32 (host arrays).
+    """
+
+    def __init__(self, windows, n_samples, n_haps, codebook, pc, bits=8, num_cycles=1,
+                 prop_mode=_lib.PROP_UNIFORM, precision=_lib.PREC_F32, klist=None, extra=None,
+                 init_parents=None, trace=False, k0=0, device="cuda:0"):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device(device)
+        self.W = len(windows)
+        self.N = n_samples
+        self.H = n_haps
+        self.C = num_cycles
+        self.bits = bits
+        self.keep = []
+        self.M = [int(w["tran"].shape[0]) for w in windows]
+        wins = (GlpWindow * self.W)()
+        self.parents_d, self.like_d = [], []
+        for i, w in enumerate(windows):
+            M = self.M[i]
+            wn = (M + 63) // 64
+            assert w["panel"].shape[1] == wn and w["panel"].shape[0] == n_haps
+            per = 32 // bits
+            assert w["packed"].shape == ((M + per - 1) // per, n_samples)
+            panel = _dev(torch, w["panel"], self.device)
+            packed = _dev(torch, w["packed"], self.device)
+            tran = _dev(torch, np.asarray(w["tran"], np.float32), self.device)
+            par = torch.zeros((n_samples, 4), dtype=torch.int32, device=self.device)
+            if init_parents is not None:
+                par.copy_(_dev(torch, np.asarray(init_parents[i], np.uint32), self.device))
+            like = torch.zeros(n_samples, dtype=torch.float64, device=self.device)
+            self.keep += [panel, packed, tran]
+            self.parents_d.append(par)
+            self.like_d.append(like)
+            wins[i] = GlpWindow(M, wn, panel.data_ptr(), packed.data_ptr(), tran.data_ptr(),
+                                par.data_ptr(), like.data_ptr())
+        self.wins = wins
+        ncode = 1 << bits
+        cb = np.ascontiguousarray(codebook, np.float32).reshape(ncode, 2)
+        pcm = np.ascontiguousarray(pc, np.float32).reshape(16)
+        nbytes = lib().glp_emit_lut_bytes(bits, precision)
+        self.lut = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        check(lib().glp_build_emit_lut(ptr(cb, c_f32p), ncode, ptr(pcm, c_f32p), precision,
+                                       self.lut.data_ptr()), "glp_build_emit_lut")
+        d = GlpMhDesc()
+        d.num_windows = self.W
+        d.windows = ctypes.cast(wins, ctypes.POINTER(GlpWindow))
+        d.num_samples = n_samples
+        d.num_haps = n_haps
+        d.bits_per_code = bits
+        d.emit_lut = self.lut.data_ptr()
+        d.num_cycles = num_cycles
+        d.k0 = k0
+        d.pen = 1.0
+        d.seed = 0
+        d.gen = 0
+        d.prop_mode = prop_mode
+        if klist is not None:
+            self.klist_d = _dev(torch, np.asarray(klist, np.uint32), self.device)
+            d.klist = self.klist_d.data_ptr()
+            d.K = int(np.asarray(klist).shape[-1])
+        if extra is not None:
+            self.extra_d = _dev(torch, np.asarray(extra, np.uint32), self.device)
+            d.extra = self.extra_d.data_ptr()
+        d.precision = precision
+        self.trace_like = self.trace_acc = None
+        if trace:
+            self.trace_like = torch.full((self.W, n_samples, num_cycles + 1), float("nan"),
+                                         dtype=torch.float64, device=self.device)
+            self.trace_acc = torch.full((self.W, n_samples, max(num_cycles, 1)), 255,
+                                        dtype=torch.uint8, device=self.device)
+            d.trace_like = self.trace_like.data_ptr()
+            d.trace_acc = self.trace_acc.data_ptr()
+        self.desc = d
+        self.plan = None
+
+    # -- execution ----------------------------------------------------------
+    def stream_handle(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def make_plan(self, lanes=0):
+        if self.plan is not None:
+            check(lib().glp_plan_destroy(self.plan), "glp_plan_destroy")
+        p = ctypes.c_void_p()
+        check(lib().glp_plan_create(ctypes.byref(self.desc), lanes, ctypes.byref(p)), "glp_plan_create")
+        self.plan = p
+        return self.lanes()
+
+    def lanes(self):
+        q, g = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().glp_plan_lanes(self.plan, ctypes.byref(q), ctypes.byref(g)), "glp_plan_lanes")
+        return q.value, g.value
+
+    def run_batched(self, gen=0, pen=1.0, seed=0):
+        if self.plan is None:
+            self.make_plan()
+        check(lib().glp_plan_run(self.plan, gen, pen, seed, self.stream_handle()), "glp_plan_run")
+
+    def run_plain(self, gen=0, pen=1.0, seed=0):
+        if self.plan is None:
+            self.make_plan()
+        check(lib().glp_plan_run_plain(self.plan, gen, pen, seed, self.stream_handle()),
+              "glp_plan_run_plain")
+
+    def run_plain_desc(self, gen=0, pen=1.0, seed=0):
+        """The plain path through the stateless glp_plain_init / glp_plain_step."""
+        self.desc.gen, self.desc.pen, self.desc.seed = gen, pen, seed
+        s = self.stream_handle()
+        check(lib().glp_plain_init(ctypes.byref(self.desc), s), "glp_plain_init")
+        for c in range(self.C):
+            check(lib().glp_plain_step(ctypes.byref(self.desc), c, s), "glp_plain_step")
+
+    def stats(self):
+        e, u = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().glp_plan_stats(self.plan, ctypes.byref(e), ctypes.byref(u)), "glp_plan_stats")
+        return e.value, u.value
+
+    # -- results ------------------------------------------------------------
+    def parents(self):
+        self.torch.cuda.synchronize(self.device)
+        return np.stack([p.cpu().numpy().view(np.uint32) for p in self.parents_d])
+
+    def like(self):
+        self.torch.cuda.synchronize(self.device)
+        return np.stack([l.cpu().numpy() for l in self.like_d])
+
+    def trace(self):
+        self.torch.cuda.synchronize(self.device)
+        return self.trace_like.cpu().numpy(), self.trace_acc.cpu().numpy()
+
+    def set_panel(self, w, panel):
+        """Replace window w's haplotype panel (e.g. after a generation)."""
+        t = self.keep[3 * w]
+        t.copy_(_dev(self.torch, panel, self.device))
+
+    def close(self):
+        if self.plan is not None:
+            lib().glp_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -186,6 +186,40 @@ int glp_plan_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                        void *stream);
 int glp_plan_destroy(glp_plan *plan);
 
+/* ===== 3. host-side helpers of the drop-in ================================ */
+
+/* Insti::initialize parameter builders (insti.cpp:1254-1315): mu from the
+ * harmonic number of hn haplotypes, the 4x4 mutation matrix pc (row-major)
+ * and tran[M][3] from per-site recombination distances in Morgans
+ * (r_morgans[m] between sites m-1 and m; geneticMap.cpp:105-111) times
+ * `inflation`, capped at 1.  Any output pointer may be NULL. */
+int glp_model_params(unsigned hn, unsigned num_sites, const double *r_morgans,
+                     float inflation, float *tran_out, float *pc16_out,
+                     float *mu_out);
+
+/* GLVQ::BuildCodeBook (glVQ.cpp:26-146) on n_gl GL trios: Lloyd iterations
+ * from the 4 fixed codewords plus Philox-drawn ones (key), until the
+ * variance changes by <= 1%.  training_size 0 trains on every GL, as the
+ * reference; otherwise on a Philox-drawn subsample of that size. */
+int glp_vq_train(const float *gls, size_t n_gl, unsigned n_codes, uint64_t key,
+                 size_t training_size, float *codebook_out, unsigned *iters);
+
+/* GLPack::GLs2VQChar + GLVQ::FindGLCode (glPack.cpp:195-219,
+ * glVQ.cpp:156-165) for every GL trio. */
+int glp_vq_codes(const float *gls, size_t n_gl, const float *codebook,
+                 unsigned n_codes, uint8_t *codes_out);
+
+/* GLPack::GetPackedGLs (glPack.cpp:32-192) for samples [s0, s1) of codes
+ * [n_samples][num_sites]; out = [ceil(num_sites*bits/32)][s1-s0]. */
+int glp_pack_codes(const uint8_t *codes, unsigned n_samples,
+                   unsigned num_sites, unsigned bits, unsigned s0, unsigned s1,
+                   uint32_t *out);
+
+/* The same without VQ: two 4-bit halves floor((GL - eps) * 16) per trio
+ * (glPack.cpp:221-243). */
+int glp_pack_nonvq(const float *gls, unsigned n_samples, unsigned num_sites,
+                   unsigned s0, unsigned s1, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -618,6 +618,11 @@ void orc_tran_from_morgans(const double *r_morgans, unsigned M,
 /* gsl_rng_uniform becomes Philox counter {0x80000000 | m>>2, sample, window, */
 /* gen}, word m & 3.  Bits of ha/hb past M are left zero.                     */
 /* ------------------------------------------------------------------------ */
+/* powf as the correctly rounded float of the FP64 power (hmm_work's
+ * annealing exponent, impute.cpp:457-469); libm powf differs between
+ * libraries in the last bit. */
+static float orc_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+
 static inline void set_bit(uint64_t *h, unsigned s, unsigned v) {
   if (v)
     h[s >> 6] |= (uint64_t)1 << (s & 63);
@@ -687,16 +692,16 @@ void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
     l11 *= sum;
     float p00 = l00 * b[0], p01 = l01 * b[1], p10 = l10 * b[2],
           p11 = l11 * b[3];
-    float c00 = powf(p[0] * (p00 * pc[4 * s00 + 0] + p01 * pc[4 * s01 + 0] +
+    float c00 = orc_powf(p[0] * (p00 * pc[4 * s00 + 0] + p01 * pc[4 * s01 + 0] +
                              p10 * pc[4 * s10 + 0] + p11 * pc[4 * s11 + 0]),
                      S);
-    float c01 = powf(p[1] * (p00 * pc[4 * s00 + 1] + p01 * pc[4 * s01 + 1] +
+    float c01 = orc_powf(p[1] * (p00 * pc[4 * s00 + 1] + p01 * pc[4 * s01 + 1] +
                              p10 * pc[4 * s10 + 1] + p11 * pc[4 * s11 + 1]),
                      S);
-    float c10 = powf(p[1] * (p00 * pc[4 * s00 + 2] + p01 * pc[4 * s01 + 2] +
+    float c10 = orc_powf(p[1] * (p00 * pc[4 * s00 + 2] + p01 * pc[4 * s01 + 2] +
                              p10 * pc[4 * s10 + 2] + p11 * pc[4 * s11 + 2]),
                      S);
-    float c11 = powf(p[2] * (p00 * pc[4 * s00 + 3] + p01 * pc[4 * s01 + 3] +
+    float c11 = orc_powf(p[2] * (p00 * pc[4 * s00 + 3] + p01 * pc[4 * s01 + 3] +
                              p10 * pc[4 * s10 + 3] + p11 * pc[4 * s11 + 3]),
                      S);
     if ((m & 3u) == 0)

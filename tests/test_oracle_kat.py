@@ -1,0 +1,187 @@
+"""Pins the CPU oracle against the known answers of the reference's own unit
+tests (tests/golden/reference_kats.json) and against textbook values."""
+import json
+import os
+import struct
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def test_philox_known_answers(orc):
+    for c in KATS["philox4x32_10"]["cases"]:
+        assert list(orc.philox(c["ctr"], c["key"])) == c["out"]
+
+
+def test_draw_counter_layout(orc):
+    # draw(seed, k, sample, window, gen) = Philox({k, sample, window, gen}, seed)
+    seed = 0x0123456789ABCDEF
+    got = orc.draw(seed, 7, 11, 3, 5)
+    want = orc.philox([7, 11, 3, 5], [seed & 0xFFFFFFFF, seed >> 32])
+    assert list(got) == list(want)
+
+
+def test_uniform_int_and_exclusion(orc):
+    L = orc.L()
+    assert L.orc_uniform_int(0xFFFFFFFF, 10) == 9
+    assert L.orc_uniform_int(0, 10) == 0
+    # closed form of UnifSampler::SampleHap: never the sample's own two haps
+    H, n = 20, 20000
+    for s in (0, 4, 9):
+        seen = set()
+        for x in np.linspace(0, 2**32 - 1, n).astype(np.uint64):
+            h = L.orc_unif_excl(int(x), H, s)
+            assert h // 2 != s and h < H
+            seen.add(h)
+        assert len(seen) == H - 2
+
+
+def codebook4():
+    # cuda_tests.cpp:70-75: cb[i] = (i/10, 2i/100) in float
+    return np.array([[f32(i) / f32(10), f32(i) * f32(2) / f32(100)] for i in range(16)], np.float32)
+
+
+def test_unpack_known_answers(orc):
+    cb = codebook4()
+    g = orc.unpack_gl(1, cb, 4, 7)
+    assert g[0] == f32(0.1) and g[1] == f32(0.02)
+    assert abs(float(g[2]) - 0.88) < 4 * np.spacing(f32(0.88))
+    for i in range(8):
+        g = orc.unpack_gl(19088743, cb, 4, i)
+        assert g[0] == cb[i, 0] and g[1] == cb[i, 1]
+        assert g[2] == np.float32(np.float32(1.0) - cb[i, 0] - cb[i, 1])
+
+
+def test_fill_emit_ones(orc):
+    rng = np.random.default_rng(5)
+    cb = rng.random((16, 2)).astype(np.float32) * 0.5
+    lut = orc.emit_lut(cb, np.ones((4, 4), np.float32))
+    for c in range(16):
+        g0, g1 = cb[c]
+        g2 = max(f32(1) - g0 - g1, f32(0))
+        want = np.float32(g0 + f32(2) * g1 + g2)
+        for i in range(4):
+            assert abs(lut[c, i] - want) <= 4 * np.spacing(want)
+
+
+def test_mutation_matrix_orientation(orc):
+    # pc[i][j] = i + 4j (cuda_tests.cpp:46-51): unit GLs read out one column
+    pc = np.array([[i + 4 * j for j in range(4)] for i in range(4)], np.float32)
+    cb = np.array([[1, 0], [0, 1], [0, 0], [0.5, 0.5]], np.float32)
+    lut = orc.emit_lut(cb, pc)
+    for i in range(4):
+        assert lut[0, i] == pc[i, 0]
+        assert lut[1, i] == pc[i, 1] + pc[i, 2]
+        assert lut[2, i] == pc[i, 3]
+
+
+def test_pack_nonvq_known_answers(orc):
+    k = KATS["pack_nonvq"]
+    g = np.array(k["one_site"]["gls"], np.float32).reshape(5, 1, 3)
+    w = orc.pack_nonvq(g, 0, 3)
+    assert w.shape == (1, 3)
+    assert [int(x) >> 24 for x in w[0]] == k["one_site"]["first_bytes"]
+    assert orc.pack_nonvq(g, 3, 5).shape == (1, 2)
+    g2 = np.array(k["two_sites"]["gls"], np.float32)
+    w = orc.pack_nonvq(g2, 0, 2)
+    b = k["two_sites"]["bytes"]
+    assert int(w[0, 0]) >> 24 == b["s0_site0"] and (int(w[0, 0]) >> 16) & 255 == b["s0_site1"]
+    assert int(w[0, 1]) >> 24 == b["s1_site0"] and (int(w[0, 1]) >> 16) & 255 == b["s1_site1"]
+    w = orc.pack_nonvq(g2, 2, 3)
+    assert int(w[0, 0]) >> 24 == b["s2_site0"] and (int(w[0, 0]) >> 16) & 255 == b["s2_site1"]
+    # sites past the end repeat GL index 0 (sample 0, site 0): glPack.cpp:66-73
+    assert (int(w[0, 0]) >> 8) & 255 == b["s0_site0"]
+
+
+def test_euclidian_dist(orc):
+    k = KATS["euclidian_dist"]
+    cb = np.array([k["a"], k["b"]], np.float32)
+    assert orc.vq_find(cb, *k["point"]) == 0
+    assert orc.vq_find(cb[::-1].copy(), *k["point"]) == 1
+    # ties keep the first codeword (std::min_element)
+    assert orc.vq_find(np.array([[0, 0], [0, 0]], np.float32), 0.3, 0.3) == 0
+
+
+@pytest.mark.parametrize("seed", [111, 112, 7])
+def test_vq_finds_every_training_point(orc, seed):
+    k = KATS["vq_find_points"]
+    g = np.array(k["gls"], np.float32)
+    rc, cb, iters = orc.vq_train(g, k["codebook_size"], seed)
+    assert rc == 0 and iters >= 1
+    for RR, Het, AA in g:
+        s = f32(RR) + f32(Het) + f32(AA)
+        rr, het = f32(RR) / s, f32(Het) / s
+        near = [c for c in range(len(cb)) if abs(cb[c, 0] - rr) < 0.01 and abs(cb[c, 1] - het) < 0.01]
+        assert near, (RR, Het)
+        assert orc.vq_find(cb, rr, het) == near[0]
+
+
+def test_pack_codes_layout(orc):
+    rng = np.random.default_rng(0)
+    for bits in (4, 8):
+        N, M = 5, 37
+        codes = rng.integers(0, 1 << bits, (N, M)).astype(np.uint8)
+        w = orc.pack_codes(codes, bits)
+        per = 32 // bits
+        assert w.shape == ((M + per - 1) // per, N)
+        for i in range(N):
+            for m in range(((M + per - 1) // per) * per):
+                c = (int(w[m // per, i]) >> (32 - bits * (m % per + 1))) & ((1 << bits) - 1)
+                assert c == (codes[i, m] if m < M else codes[0, 0])
+
+
+def _decimal_log_round(x):
+    getcontext().prec = 80
+    L = Decimal(float(x)).ln()
+    c = np.float32(float(L))
+    cands = [np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))]
+    return min(cands, key=lambda v: abs(Decimal(float(v)) - L))
+
+
+def test_logf_correctly_rounded(orc):
+    L = orc.L()
+    # the 20 hard inputs of the exception table
+    hard = [0x0dc8bba4, 0x111c87f8, 0x1a8446cb, 0x1f116ab8, 0x29fd22f8, 0x2c4c24b7, 0x38dcbe38,
+            0x3bf86ef0, 0x3c413d3a, 0x41178feb, 0x464d5b2b, 0x4665a9a6, 0x4c5d65a5, 0x4d604ebe,
+            0x4e85f412, 0x5ee8984e, 0x65d890d3, 0x66a8c860, 0x6f31a8ec, 0x79e7ec37]
+    rng = np.random.default_rng(3)
+    rand = rng.integers(0x00800000, 0x7f7fffff, 400).tolist()
+    for b in hard + rand:
+        x = struct.unpack("<f", struct.pack("<I", b))[0]
+        assert np.float32(L.orc_logf_cr(x)) == _decimal_log_round(x), hex(b)
+
+
+def test_norm_value(orc):
+    # powf(FLT_MIN, 2.0f / 3.0f) (impute.cpp:11-13): the exponent is the float 2/3
+    n = orc.L().orc_norm()
+    want = 2.0 ** (-126 * float(np.float32(2.0) / np.float32(3.0)))
+    assert abs(n - want) / n < 1e-6
+
+
+def test_model_params(orc):
+    # mu: 1/H_{hn-1} then mu/(hn+mu), in float as insti.cpp:1254-1270
+    for hn in (4, 200, 10000):
+        mu = 0.0
+        for i in range(1, hn):
+            mu = float(np.float32(mu + 1.0 / i))
+        mu = np.float32(np.float32(1) / np.float32(mu))
+        mu = np.float32(mu / np.float32(np.float32(hn) + mu))
+        assert orc.mu(hn) == mu
+    pc = orc.pc(np.float32(0.01))
+    assert pc[0, 0] == pc[3, 3] == np.float32(np.float32(0.99) * np.float32(0.99))
+    assert pc[0, 3] == pc[1, 2] == np.float32(np.float32(0.01) * np.float32(0.01))
+    assert pc[0, 1] == pc[2, 3] == pc[1, 0]
+    # rows of pc are the allele-flip probabilities: each row sums to 1
+    assert np.allclose(pc.sum(1), 1.0, atol=1e-6)
+    r = np.array([0, 1e-6, 0.3, 2.0])
+    t = orc.tran_from_morgans(r)
+    assert (t[0] == 0).all()
+    assert t[3, 0] == 0 and t[3, 2] == 1  # r capped at 1 (insti.cpp:1293)
+    assert np.allclose(t[1:, 0] + 2 * t[1:, 1] + t[1:, 2], 1.0, atol=1e-6)
