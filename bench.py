@@ -124,6 +124,11 @@ def main():
     job = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C,
               precision=PREC_MIXED)
     lanes, grid = job.make_plan(args.lanes)
+    if not args.lanes:
+        t0 = time.time()
+        tuned = job.autotune(gen=0, pen=pen, seed=seed)
+        lanes, grid = job.lanes()
+        log(f"autotune: lanes/chain {list(tuned)} in {time.time() - t0:.2f}s")
     se_step = N * (C + 1) * M
     gen = [0]
 

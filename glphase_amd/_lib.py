@@ -87,6 +87,9 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p]),
     ("glp_plan_stats", ctypes.c_int, [ctypes.c_void_p, c_u64p, c_u64p]),
     ("glp_plan_lanes", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p]),
+    ("glp_plan_set_lanes", ctypes.c_int, [ctypes.c_void_p, c_u32p]),
+    ("glp_plan_autotune", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("glp_model_params", ctypes.c_int,
      [ctypes.c_uint, ctypes.c_uint, c_f64p, ctypes.c_float, c_f32p, c_f32p, c_f32p]),
@@ -110,6 +113,12 @@ def lib():
     """The loaded library; raises when it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime; loading it first makes this
+        # library bind to the same libamdhip64 instance in one process
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         if not os.path.exists(LIB_PATH):
             raise GlpError(
                 "libglphase_amd.so is not built (run __graft_entry__.build() or "

@@ -98,18 +98,35 @@ using namespace glp;
 // batched plan
 // ---------------------------------------------------------------------------
 struct glp_plan {
-  int dev = 0;
+  int dev = 0, num_cu = 1, bpc = 1;
   KArgs a{};
-  unsigned nwin = 0, ntasks = 0, Q = 1, grid = 1;
-  DevBuf wins, tasks, ctrs;
+  unsigned nwin = 0, grid = 1;
+  std::vector<glp_window> wins_h;
+  std::vector<unsigned> order, q;  // windows longest first; lanes per chain slot
+  DevBuf wins, order_d, qtab, wctr, ctrs;
 };
 
-static unsigned auto_lanes(unsigned ntasks, unsigned C, int num_cu) {
-  // enough lanes for ~16 waves per CU, never more than a task can use
-  const double target = (double)num_cu * 16.0 * 64.0;
-  unsigned Q = 1;
-  while (Q < 64 && (double)ntasks * Q < target && Q < C + 1) Q *= 2;
+// Lanes per chain slot when none is given: one MH round decides ~6 proposals
+// at the ~15% acceptance of sampling generations, so depth 8 wastes little;
+// deeper only when there are too few chains to fill the device.
+static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu) {
+  const double target = (double)num_cu * 4.0 * 64.0;  // one wave per SIMD
+  unsigned Q = 8;
+  while (Q < 64 && (double)nchains * Q < target) Q *= 2;
+  while (Q > 1 && Q / 2 >= C + 1) Q /= 2;
   return Q;
+}
+
+static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
+  double lanes = 0;
+  for (unsigned w : wins) lanes += (double)p->a.N * p->q[w];
+  const unsigned blocks = (unsigned)std::ceil(lanes / 256.0);
+  return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
+}
+
+static int plan_upload_q(glp_plan *p) {
+  return hip_check(hipMemcpy(p->qtab.p, p->q.data(), 4 * p->nwin, hipMemcpyHostToDevice),
+                   "lanes table copy");
 }
 
 static int fill_args(const glp_mh_desc *d, KArgs &a) {
@@ -216,42 +233,38 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
     return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 or a power of two <= 64");
   glp_plan *p = new glp_plan();
   p->a = a;
-  GLP_TRY(hip_check(hipGetDevice(&p->dev), "hipGetDevice"));
-  int num_cu = 0;
-  GLP_TRY(hip_check(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, p->dev),
-                    "device attribute"));
-  p->nwin = desc->num_windows;
-  // tasks: every (window, sample), longest windows first
-  std::vector<unsigned> order(p->nwin);
-  for (unsigned w = 0; w < p->nwin; ++w) order[w] = w;
-  std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) {
-    return desc->windows[x].num_sites > desc->windows[y].num_sites;
-  });
-  std::vector<uint2> tasks;
-  tasks.reserve((size_t)p->nwin * a.N);
-  for (unsigned w : order)
-    for (unsigned i = 0; i < a.N; ++i) tasks.push_back(make_uint2(w, i));
-  p->ntasks = (unsigned)tasks.size();
-  p->Q = lanes_per_chain ? lanes_per_chain : auto_lanes(p->ntasks, a.C, num_cu);
-  int bpc = 0;
-  int rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, p->Q, (size_t)a.lut_entries * 16, &bpc);
+  int rc = hip_check(hipGetDevice(&p->dev), "hipGetDevice");
+  if (rc == GLP_OK)
+    rc = hip_check(hipDeviceGetAttribute(&p->num_cu, hipDeviceAttributeMultiprocessorCount, p->dev),
+                   "device attribute");
+  if (rc == GLP_OK) rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, lds_bytes(a), &p->bpc);
   if (rc != GLP_OK) {
     delete p;
     return rc;
   }
-  if (bpc < 1) bpc = 1;
-  const unsigned slots_per_block = 4 * (64 / p->Q);
-  const unsigned need = (p->ntasks + slots_per_block - 1) / slots_per_block;
-  p->grid = std::max(1u, std::min((unsigned)(bpc * num_cu), need));
+  if (p->bpc < 1) p->bpc = 1;
+  p->nwin = desc->num_windows;
+  p->wins_h.assign(desc->windows, desc->windows + p->nwin);
+  p->order.resize(p->nwin);
+  for (unsigned w = 0; w < p->nwin; ++w) p->order[w] = w;
+  std::stable_sort(p->order.begin(), p->order.end(), [&](unsigned x, unsigned y) {
+    return desc->windows[x].num_sites > desc->windows[y].num_sites;
+  });
+  const unsigned Q = lanes_per_chain ? lanes_per_chain : auto_lanes(a.N * p->nwin, a.C, p->num_cu);
+  p->q.assign(p->nwin, Q);
+  p->grid = plan_grid(p, p->order);
   rc = p->wins.ensure(sizeof(glp_window) * p->nwin);
-  if (rc == GLP_OK) rc = p->tasks.ensure(sizeof(uint2) * tasks.size());
+  if (rc == GLP_OK) rc = p->order_d.ensure(4 * p->nwin);
+  if (rc == GLP_OK) rc = p->qtab.ensure(4 * p->nwin);
+  if (rc == GLP_OK) rc = p->wctr.ensure(4 * p->nwin);
   if (rc == GLP_OK) rc = p->ctrs.ensure(4 * sizeof(unsigned long long));
   if (rc == GLP_OK)
     rc = hip_check(hipMemcpy(p->wins.p, desc->windows, sizeof(glp_window) * p->nwin,
                              hipMemcpyHostToDevice), "window table copy");
   if (rc == GLP_OK)
-    rc = hip_check(hipMemcpy(p->tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
-                             hipMemcpyHostToDevice), "task list copy");
+    rc = hip_check(hipMemcpy(p->order_d.p, p->order.data(), 4 * p->nwin, hipMemcpyHostToDevice),
+                   "window order copy");
+  if (rc == GLP_OK) rc = plan_upload_q(p);
   if (rc != GLP_OK) {
     glp_plan_destroy(p);
     return rc;
@@ -266,9 +279,106 @@ int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stre
   a.gen = gen;
   a.pen = pen;
   a.seed = seed;
-  return launch_batched(a, p->wins.as<glp_window>(), p->tasks.as<uint2>(), p->ntasks,
-                        p->ctrs.as<unsigned long long>(), p->Q, p->grid,
-                        static_cast<hipStream_t>(stream));
+  return launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
+                        p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
+                        p->ctrs.as<unsigned long long>(), p->grid, static_cast<hipStream_t>(stream));
+}
+
+int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
+  if (!p || !lanes_per_window) return set_error(GLP_ERR_ARG, "null argument");
+  for (unsigned w = 0; w < p->nwin; ++w) {
+    const unsigned Q = lanes_per_window[w];
+    if (Q == 0 || Q > 64 || (Q & (Q - 1)) != 0)
+      return set_error(GLP_ERR_ARG, "lanes per chain must be a power of two <= 64");
+  }
+  p->q.assign(lanes_per_window, lanes_per_window + p->nwin);
+  p->grid = plan_grid(p, p->order);
+  return plan_upload_q(p);
+}
+
+int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream,
+                      unsigned *lanes_out) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KArgs a = p->a;
+  a.gen = gen;
+  a.pen = pen;
+  a.seed = seed;
+  // trials overwrite the windows' parents and log-likelihoods (inputs under
+  // GLP_PROP_HOST): keep a copy and restore it after every trial
+  const size_t pbytes = 16 * (size_t)a.N, lbytes = 8 * (size_t)a.N;
+  DevBuf save, sub;
+  GLP_TRY(save.ensure((pbytes + lbytes) * p->nwin));
+  GLP_TRY(sub.ensure(4 * p->nwin));
+  auto copy_all = [&](bool to_save) -> int {
+    for (unsigned w = 0; w < p->nwin; ++w) {
+      char *base = save.as<char>() + (pbytes + lbytes) * w;
+      const glp_window &W = p->wins_h[w];
+      void *dp = to_save ? (void *)base : (void *)W.parents;
+      const void *sp = to_save ? (const void *)W.parents : (const void *)base;
+      GLP_TRY(hip_check(hipMemcpyAsync(dp, sp, pbytes, hipMemcpyDeviceToDevice, s), "save copy"));
+      dp = to_save ? (void *)(base + pbytes) : (void *)W.like;
+      sp = to_save ? (const void *)W.like : (const void *)(base + pbytes);
+      GLP_TRY(hip_check(hipMemcpyAsync(dp, sp, lbytes, hipMemcpyDeviceToDevice, s), "save copy"));
+    }
+    return GLP_OK;
+  };
+  GLP_TRY(copy_all(true));
+  hipEvent_t e0, e1;
+  GLP_TRY(hip_check(hipEventCreate(&e0), "event"));
+  GLP_TRY(hip_check(hipEventCreate(&e1), "event"));
+  // window-size buckets: ceil(log2(M))
+  std::vector<std::vector<unsigned>> buckets;
+  std::vector<int> bucket_of(64, -1);
+  for (unsigned w : p->order) {
+    unsigned lg = 0;
+    while ((1u << lg) < p->wins_h[w].num_sites) ++lg;
+    if (bucket_of[lg] < 0) {
+      bucket_of[lg] = (int)buckets.size();
+      buckets.emplace_back();
+    }
+    buckets[bucket_of[lg]].push_back(w);
+  }
+  int rc = GLP_OK;
+  for (auto &b : buckets) {
+    rc = hip_check(hipMemcpy(sub.p, b.data(), 4 * b.size(), hipMemcpyHostToDevice), "bucket copy");
+    if (rc != GLP_OK) break;
+    unsigned best = p->q[b[0]];
+    float best_ms = 1e30f;
+    for (unsigned Q = 1; Q <= 64 && rc == GLP_OK; Q *= 2) {
+      if (Q > 1 && Q / 2 >= a.C + 1) break;
+      for (unsigned w : b) p->q[w] = Q;
+      rc = plan_upload_q(p);
+      const unsigned grid = plan_grid(p, b);
+      float ms = 0;
+      for (int rep = 0; rep < 2 && rc == GLP_OK; ++rep) {
+        rc = hip_check(hipEventRecord(e0, s), "event record");
+        if (rc == GLP_OK)
+          rc = launch_batched(a, p->wins.as<glp_window>(), sub.as<unsigned>(), p->qtab.as<unsigned>(),
+                              (unsigned)b.size(), p->wctr.as<unsigned>(),
+                              p->ctrs.as<unsigned long long>(), grid, s);
+        if (rc == GLP_OK) rc = hip_check(hipEventRecord(e1, s), "event record");
+        if (rc == GLP_OK) rc = hip_check(hipEventSynchronize(e1), "event sync");
+        float t = 0;
+        if (rc == GLP_OK) rc = hip_check(hipEventElapsedTime(&t, e0, e1), "event time");
+        ms = rep == 0 ? t : std::min(ms, t);
+        if (rc == GLP_OK) rc = copy_all(false);
+      }
+      if (rc == GLP_OK && ms < best_ms) {
+        best_ms = ms;
+        best = Q;
+      }
+    }
+    for (unsigned w : b) p->q[w] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == GLP_OK) rc = plan_upload_q(p);
+  if (rc == GLP_OK) rc = hip_check(hipStreamSynchronize(s), "synchronize");
+  p->grid = plan_grid(p, p->order);
+  if (rc == GLP_OK && lanes_out)
+    for (unsigned w = 0; w < p->nwin; ++w) lanes_out[w] = p->q[w];
+  return rc;
 }
 
 int glp_plan_stats(glp_plan *p, uint64_t *evals, uint64_t *useful) {
@@ -295,16 +405,14 @@ int glp_plan_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void
 
 int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {
   if (!p) return set_error(GLP_ERR_ARG, "null plan");
-  if (lanes) *lanes = p->Q;
+  if (lanes) *lanes = p->q.empty() ? 0 : p->q[p->order[0]];
   if (grid) *grid = p->grid;
   return GLP_OK;
 }
 
 int glp_plan_destroy(glp_plan *p) {
   if (!p) return GLP_OK;
-  p->wins.release();
-  p->tasks.release();
-  p->ctrs.release();
+  for (DevBuf *b : {&p->wins, &p->order_d, &p->qtab, &p->wctr, &p->ctrs}) b->release();
   delete p;
   return GLP_OK;
 }

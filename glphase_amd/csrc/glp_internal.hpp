@@ -36,12 +36,11 @@ int hip_check(hipError_t err, const char *what);
 
 int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
                  unsigned step, hipStream_t stream);
-int launch_batched(const KArgs &a, const glp_window *d_wins,
-                   const uint2 *d_tasks, unsigned ntasks,
-                   unsigned long long *d_ctrs, unsigned Q, unsigned grid,
-                   hipStream_t stream);
-int batched_occupancy(unsigned bits, bool mixed, unsigned Q, size_t smem,
-                      int *blocks_per_cu);
+size_t lds_bytes(const KArgs &a);
+int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
+                   const unsigned *d_qtab, unsigned nwin, unsigned *d_wctr,
+                   unsigned long long *d_ctrs, unsigned grid, hipStream_t stream);
+int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu);
 int launch_eval(const KArgs &a, const glp_window *d_wins,
                 const unsigned *d_samples, const unsigned *d_parents,
                 unsigned n, float *d_out, hipStream_t stream);

@@ -23,172 +23,300 @@
 
 namespace glp {
 
+#define GLP_TRY_K(expr)              \
+  do {                               \
+    int rc_ = (expr);                \
+    if (rc_ != GLP_OK) return rc_;   \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // correctly rounded logf
 // ---------------------------------------------------------------------------
 // (float)log((double)x) is the correctly rounded logf of x for every positive
-// float except these 20, whose log lies so close to a float rounding midpoint
-// that a 2-ulp change of the double log flips the float (exhaustive search
-// over all positive floats; values from an 80-digit decimal log).
-__constant__ uint32_t c_logf_hard[20][2] = {
-    {0x0dc8bba4u, 0xc289bac4u}, {0x111c87f8u, 0xc28085dfu},
-    {0x1a8446cbu, 0xc24d0a45u}, {0x1f116ab8u, 0xc233b53eu},
-    {0x29fd22f8u, 0xc1ee8859u}, {0x2c4c24b7u, 0xc1d48710u},
-    {0x38dcbe38u, 0xc1128ba4u}, {0x3bf86ef0u, 0xc09c399eu},
-    {0x3c413d3au, 0xc08e158fu}, {0x41178febu, 0x400fe5e7u},
-    {0x464d5b2bu, 0x4117bcf1u}, {0x4665a9a6u, 0x41198725u},
-    {0x4c5d65a5u, 0x418f034bu}, {0x4d604ebeu, 0x419a352cu},
-    {0x4e85f412u, 0x41a6b811u}, {0x5ee8984eu, 0x422e4a21u},
-    {0x65d890d3u, 0x4254d1f9u}, {0x66a8c860u, 0x42595e46u},
-    {0x6f31a8ecu, 0x42845a89u}, {0x79e7ec37u, 0x42a1ffb7u}};
+// float except 20, whose log lies so close to a float rounding midpoint that
+// a 2-ulp change of the double log flips the float (exhaustive search over
+// all positive floats; values from an 80-digit decimal log).  They sit in a
+// 32-slot table under the perfect hash (bits * 0xc2eba581) >> 27.
+__constant__ uint32_t c_logf_hard[32][2] = {
+    {0x4c5d65a5u, 0x418f034bu}, {0x3bf86ef0u, 0xc09c399eu},
+    {0xffffffffu, 0x00000000u}, {0xffffffffu, 0x00000000u},
+    {0xffffffffu, 0x00000000u}, {0x41178febu, 0x400fe5e7u},
+    {0x66a8c860u, 0x42595e46u}, {0xffffffffu, 0x00000000u},
+    {0xffffffffu, 0x00000000u}, {0x1f116ab8u, 0xc233b53eu},
+    {0x4d604ebeu, 0x419a352cu}, {0x29fd22f8u, 0xc1ee8859u},
+    {0xffffffffu, 0x00000000u}, {0x79e7ec37u, 0x42a1ffb7u},
+    {0x1a8446cbu, 0xc24d0a45u}, {0xffffffffu, 0x00000000u},
+    {0x4665a9a6u, 0x41198725u}, {0x464d5b2bu, 0x4117bcf1u},
+    {0x3c413d3au, 0xc08e158fu}, {0x4e85f412u, 0x41a6b811u},
+    {0xffffffffu, 0x00000000u}, {0x5ee8984eu, 0x422e4a21u},
+    {0xffffffffu, 0x00000000u}, {0x65d890d3u, 0x4254d1f9u},
+    {0x111c87f8u, 0xc28085dfu}, {0x2c4c24b7u, 0xc1d48710u},
+    {0xffffffffu, 0x00000000u}, {0x0dc8bba4u, 0xc289bac4u},
+    {0xffffffffu, 0x00000000u}, {0xffffffffu, 0x00000000u},
+    {0x6f31a8ecu, 0x42845a89u}, {0x38dcbe38u, 0xc1128ba4u}};
 
-__device__ __noinline__ float cr_logf(float x) {
+__device__ __forceinline__ float cr_logf(float x) {
   const uint32_t b = __float_as_uint(x);
-  float r = (float)log((double)x);
-#pragma unroll
-  for (int q = 0; q < 20; ++q)
-    if (c_logf_hard[q][0] == b) r = __uint_as_float(c_logf_hard[q][1]);
-  return r;
+  const uint32_t slot = (b * 0xc2eba581u) >> 27;
+  const float r = (float)log((double)x);
+  return c_logf_hard[slot][0] == b ? __uint_as_float(c_logf_hard[slot][1]) : r;
 }
+
+// ---------------------------------------------------------------------------
+// LDS tables
+// ---------------------------------------------------------------------------
+// Emission table: the 16-byte entry at byte (code << 8) | (nib << 4) holds
+// (e00, e01, e10, e11) of a site with GL code `code` whose 4 parent alleles
+// are nib = f0 | f1 << 1 | m0 << 2 | m1 << 3, e_xy = FillEmit(code)[(f_x << 1)
+// | m_y] (hmmLike.cu:39-44,124-127).
+// Spread table: for parent p and an allele byte b of 8 consecutive sites,
+// entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
+// in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
+constexpr unsigned kSpreadBytes = 4 * 256 * 8;
+
+__device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
+                                         unsigned lut_entries) {
+  float4 *lut = reinterpret_cast<float4 *>(smem);
+  for (unsigned q = threadIdx.x; q < lut_entries; q += blockDim.x) lut[q] = g_lut[q];
+  uint2 *spread = reinterpret_cast<uint2 *>(smem + 16u * lut_entries);
+  for (unsigned q = threadIdx.x; q < 1024u; q += blockDim.x) {
+    const unsigned p = q >> 8, b = q & 255u;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (unsigned i = 0; i < 4; ++i) {
+      lo |= ((b >> i) & 1u) << (8 * i + 4 + p);
+      hi |= ((b >> (i + 4)) & 1u) << (8 * i + 4 + p);
+    }
+    spread[q] = make_uint2(lo, hi);
+  }
+  __syncthreads();
+}
+
+typedef const __attribute__((address_space(4))) float cfloat;
 
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
-// Emission table in LDS: entry (code << 4) | nib holds the 4 emissions
-// (e00, e01, e10, e11) of a site whose code is `code` and whose 4 parent
-// alleles are nib = f0 | f1 << 1 | m0 << 2 | m1 << 3, where
-// e_xy = FillEmit(code)[(f_x << 1) | m_y] (hmmLike.cu:124-127).
+// Forward state of one lane.  A rescale (impute.cpp:367-372) scales the
+// probabilities at once but only queues the scale factor; the logs of queued
+// factors are taken in their original order at chunk ends (or when the
+// 4-entry queue is full), so the score is the same sum of the same terms as
+// the serial code while the wavefront's lanes, which rescale at different
+// sites, share one pass of log evaluations.
 struct FwdState {
   float l00, l01, l10, l11;
   float fscore;
   double dscore;
+  float q0, q1, q2, q3;  // queued factors, q0 newest
+  unsigned nq;
 };
 
-// One 32-site chunk of the recurrence.  h[] are the 32-bit halves of the 4
-// parents' haplotype words covering the chunk; codes points at the chunk's
-// first code word of this sample (stride = words between sites groups);
-// tr = tran + 3 * (first site of the chunk).
+template <bool MIXED>
+__device__ __forceinline__ void add_log(FwdState &st, float inv) {
+  if (MIXED)
+    st.dscore -= log((double)inv);
+  else
+    st.fscore -= cr_logf(inv);
+}
+
+// Takes the logs of the queued factors, oldest first.
+template <bool MIXED>
+__device__ __forceinline__ void flush_logs(FwdState &st) {
+  if (__any(st.nq >= 4u) && st.nq >= 4u) add_log<MIXED>(st, st.q3);
+  if (__any(st.nq >= 3u) && st.nq >= 3u) add_log<MIXED>(st, st.q2);
+  if (__any(st.nq >= 2u) && st.nq >= 2u) add_log<MIXED>(st, st.q1);
+  if (__any(st.nq >= 1u) && st.nq >= 1u) add_log<MIXED>(st, st.q0);
+  st.nq = 0;
+}
+
+// One site of the recurrence (impute.cpp:354-372) with emissions e.
+template <bool MIXED>
+__device__ __forceinline__ void site_step(FwdState &st, const float4 e, float t0, float t1,
+                                          float t2, float norm) {
+  const float l00 = st.l00, l01 = st.l01, l10 = st.l10, l11 = st.l11;
+  const float b00 = l00 * t0 + (l01 + l10) * t1 + l11 * t2;
+  const float b01 = l01 * t0 + (l00 + l11) * t1 + l10 * t2;
+  const float b10 = l10 * t0 + (l00 + l11) * t1 + l01 * t2;
+  const float b11 = l11 * t0 + (l01 + l10) * t1 + l00 * t2;
+  float n00 = b00 * e.x, n01 = b01 * e.y, n10 = b10 * e.z, n11 = b11 * e.w;
+  float sum = n00 + n01 + n10 + n11;
+  if (sum < norm) {
+    sum = 1.0f / sum;
+    if (st.nq == 4u) {  // queue full: drain it in order (rare)
+      add_log<MIXED>(st, st.q3);
+      add_log<MIXED>(st, st.q2);
+      add_log<MIXED>(st, st.q1);
+      add_log<MIXED>(st, st.q0);
+      st.nq = 0;
+    }
+    st.q3 = st.q2;
+    st.q2 = st.q1;
+    st.q1 = st.q0;
+    st.q0 = sum;
+    ++st.nq;
+    n00 *= sum;
+    n01 *= sum;
+    n10 *= sum;
+    n11 *= sum;
+  }
+  st.l00 = n00;
+  st.l01 = n01;
+  st.l10 = n10;
+  st.l11 = n11;
+}
+
+// Byte address of site s's emission entry: byte 0 = (nib << 4) from the
+// spread word, byte 1 = the site's 8-bit code.
+template <int BITS>
+__device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned s) {
+  if (BITS == 8) {
+    // v_perm_b32: result byte 0 <- nbw byte (s & 3), byte 1 <- cw byte
+    // (3 - (s & 3)) (codes are MSB-first), bytes 2, 3 <- 0
+    const uint32_t sel = 0x0c0c0000u | ((7u - (s & 3u)) << 8) | (s & 3u);
+    return __builtin_amdgcn_perm(cw, nbw, sel);
+  } else {
+    const uint32_t code = (cw >> (28u - 4u * (s & 7u))) & 15u;
+    return (code << 8) | ((nbw >> (8u * (s & 3u))) & 0xF0u);
+  }
+}
+
+// 32 sites starting at chunk site 0 (FIRST: the window's site 0, which only
+// initialises, impute.cpp:344-347).  h = the 4 parents' 32-bit allele words,
+// cw = the chunk's code words, tr = tran of the chunk's first site
+// (wave-uniform), n = sites in this chunk.
 template <int BITS, bool MIXED, bool FIRST, bool TAIL>
 __device__ __forceinline__ void fwd_chunk(FwdState &st, const uint32_t (&h)[4],
-                                          const uint32_t *__restrict__ codes,
-                                          size_t stride,
-                                          const float *__restrict__ tr,
-                                          unsigned n, const float4 *lut,
+                                          const uint32_t (&cw)[BITS], const float *tr, unsigned n,
+                                          const unsigned char *smem, const uint2 *spread,
                                           float norm) {
-  constexpr int CPW = 32 / BITS;  // codes per word
-  constexpr int NW = 32 / CPW;    // code words per chunk
-  constexpr uint32_t MASK = (1u << BITS) - 1u;
-  uint32_t cw[NW];
 #pragma unroll
-  for (int q = 0; q < NW; ++q)
-    cw[q] = (!TAIL || (unsigned)(q * CPW) < n) ? codes[q * stride] : 0u;
+  for (int g = 0; g < 4; ++g) {
+    if (TAIL && (unsigned)(8 * g) >= n) break;
+    const uint2 s0 = spread[0 * 256 + ((h[0] >> (8 * g)) & 255u)];
+    const uint2 s1 = spread[1 * 256 + ((h[1] >> (8 * g)) & 255u)];
+    const uint2 s2 = spread[2 * 256 + ((h[2] >> (8 * g)) & 255u)];
+    const uint2 s3 = spread[3 * 256 + ((h[3] >> (8 * g)) & 255u)];
+    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x, s0.y | s1.y | s2.y | s3.y};
+    constexpr int CPW = 32 / BITS;
+    // emission entries and transitions of the group, loaded ahead of the
+    // recurrence (tr is the wave's LDS copy of the chunk's tran)
+    float4 e[8];
+    float4 t4[6];
 #pragma unroll
-  for (int s = 0; s < 32; ++s) {
-    if (TAIL && (unsigned)s >= n) break;
-    const uint32_t code = (cw[s / CPW] >> (32 - BITS * (s % CPW + 1))) & MASK;
-    const uint32_t nib = ((h[0] >> s) & 1u) | (((h[1] >> s) & 1u) << 1) |
-                         (((h[2] >> s) & 1u) << 2) | (((h[3] >> s) & 1u) << 3);
-    const float4 e = lut[(code << 4) | nib];
-    if (FIRST && s == 0) {
-      // impute.cpp:344-347
-      st.l00 = 0.25f * e.x;
-      st.l01 = 0.25f * e.y;
-      st.l10 = 0.25f * e.z;
-      st.l11 = 0.25f * e.w;
-      continue;
+    for (int q = 0; q < 6; ++q) t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
+    const float *tg = reinterpret_cast<const float *>(t4);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int s = 8 * g + k;
+      if (TAIL && (unsigned)s >= n) break;
+      const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
+      e[k] = *reinterpret_cast<const float4 *>(smem + a);
     }
-    const float t0 = tr[3 * s], t1 = tr[3 * s + 1], t2 = tr[3 * s + 2];
-    const float l00 = st.l00, l01 = st.l01, l10 = st.l10, l11 = st.l11;
-    // impute.cpp:354-366
-    const float b00 = l00 * t0 + (l01 + l10) * t1 + l11 * t2;
-    const float b01 = l01 * t0 + (l00 + l11) * t1 + l10 * t2;
-    const float b10 = l10 * t0 + (l00 + l11) * t1 + l01 * t2;
-    const float b11 = l11 * t0 + (l01 + l10) * t1 + l00 * t2;
-    float n00 = b00 * e.x, n01 = b01 * e.y, n10 = b10 * e.z, n11 = b11 * e.w;
-    float sum = n00 + n01 + n10 + n11;
-    // impute.cpp:367-372: rescale when the probabilities become too small
-    if (sum < norm) {
-      sum = 1.0f / sum;
-      if (MIXED)
-        st.dscore -= log((double)sum);
-      else
-        st.fscore -= cr_logf(sum);
-      n00 *= sum;
-      n01 *= sum;
-      n10 *= sum;
-      n11 *= sum;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int s = 8 * g + k;
+      if (TAIL && (unsigned)s >= n) break;
+      if (FIRST && s == 0) {
+        st.l00 = 0.25f * e[k].x;
+        st.l01 = 0.25f * e[k].y;
+        st.l10 = 0.25f * e[k].z;
+        st.l11 = 0.25f * e[k].w;
+        continue;
+      }
+      site_step<MIXED>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], norm);
     }
-    st.l00 = n00;
-    st.l01 = n01;
-    st.l10 = n10;
-    st.l11 = n11;
   }
 }
 
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
-// pair m0, m1), rows viewed as 32-bit words.
+// pair m0, m1; rows as 32-bit words).  Every lane of the wavefront must call
+// it together (M and tran wave-uniform): the lanes copy each 32-site chunk of
+// tran into the wave's LDS buffer wtr (96 floats), one chunk ahead.
 template <int BITS, bool MIXED>
-__device__ double forward_pass(const uint32_t *__restrict__ f0,
-                               const uint32_t *__restrict__ f1,
-                               const uint32_t *__restrict__ m0,
-                               const uint32_t *__restrict__ m1,
-                               const uint32_t *__restrict__ codes,
-                               size_t stride, const float *__restrict__ tran,
-                               unsigned M, const float4 *lut, float norm) {
+__device__ double forward_pass(const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                               const uint32_t *__restrict__ m0, const uint32_t *__restrict__ m1,
+                               const uint32_t *__restrict__ codes, size_t stride,
+                               const float *__restrict__ tran, unsigned M,
+                               const unsigned char *smem, unsigned lut_entries, float *wtr,
+                               float norm) {
   constexpr int NW = BITS;  // code words per 32 sites
+  constexpr int CPW = 32 / BITS;
+  const uint2 *spread = reinterpret_cast<const uint2 *>(smem + 16u * lut_entries);
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned ntr = 3u * M;
   FwdState st;
   st.fscore = 0.0f;
   st.dscore = 0.0;
-  const unsigned nfull = M >> 5, rem = M & 31u;
-  if (nfull > 0) {
+  st.q0 = st.q1 = st.q2 = st.q3 = 1.0f;
+  st.nq = 0;
+  const unsigned nch = (M + 31u) >> 5;
+  const unsigned nwords = (M + CPW - 1) / CPW;
+  uint32_t h[4] = {f0[0], f1[0], m0[0], m1[0]};
+  uint32_t cw[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) cw[q] = (unsigned)q < nwords ? codes[q * stride] : 0u;
+  // lane l < 48 carries floats 2l, 2l+1 of a chunk's 96 tran values
+  float2 tp = make_float2(0.0f, 0.0f);
+  if (lane < 48u && 2u * lane + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + 2u * lane);
+  for (unsigned ch = 0; ch < nch; ++ch) {
+    if (lane < 48u) reinterpret_cast<float2 *>(wtr)[lane] = tp;
+    // prefetch the next chunk's tran, allele and code words
+    const unsigned nx = ch + 1;
     {
-      const uint32_t h[4] = {f0[0], f1[0], m0[0], m1[0]};
-      fwd_chunk<BITS, MIXED, true, false>(st, h, codes, stride, tran, 32, lut,
-                                          norm);
+      const unsigned o = 96u * nx + 2u * lane;
+      tp = make_float2(0.0f, 0.0f);
+      if (lane < 48u && o + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + o);
     }
-    for (unsigned ch = 1; ch < nfull; ++ch) {
-      const uint32_t h[4] = {f0[ch], f1[ch], m0[ch], m1[ch]};
-      fwd_chunk<BITS, MIXED, false, false>(st, h, codes + ch * NW * stride,
-                                           stride, tran + 96 * ch, 32, lut,
-                                           norm);
+    uint32_t hn[4] = {0, 0, 0, 0}, cn[NW];
+    if (nx < nch) {
+      hn[0] = f0[nx];
+      hn[1] = f1[nx];
+      hn[2] = m0[nx];
+      hn[3] = m1[nx];
     }
-    if (rem) {
-      const uint32_t h[4] = {f0[nfull], f1[nfull], m0[nfull], m1[nfull]};
-      fwd_chunk<BITS, MIXED, false, true>(st, h, codes + nfull * NW * stride,
-                                          stride, tran + 96 * nfull, rem, lut,
-                                          norm);
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const unsigned wq = nx * NW + q;
+      cn[q] = wq < nwords ? codes[wq * stride] : 0u;
     }
-  } else {
-    const uint32_t h[4] = {f0[0], f1[0], m0[0], m1[0]};
-    fwd_chunk<BITS, MIXED, true, true>(st, h, codes, stride, tran, rem, lut,
-                                       norm);
+    const unsigned base = ch << 5;
+    const unsigned n = M - base < 32u ? M - base : 32u;
+    if (n == 32) {
+      if (ch == 0)
+        fwd_chunk<BITS, MIXED, true, false>(st, h, cw, wtr, 32, smem, spread, norm);
+      else
+        fwd_chunk<BITS, MIXED, false, false>(st, h, cw, wtr, 32, smem, spread, norm);
+    } else {
+      if (ch == 0)
+        fwd_chunk<BITS, MIXED, true, true>(st, h, cw, wtr, n, smem, spread, norm);
+      else
+        fwd_chunk<BITS, MIXED, false, true>(st, h, cw, wtr, n, smem, spread, norm);
+    }
+    flush_logs<MIXED>(st);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h[q] = hn[q];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) cw[q] = cn[q];
   }
   const float last = st.l00 + st.l01 + st.l10 + st.l11;
   if (MIXED) return st.dscore + log((double)last);
   return (double)(st.fscore + cr_logf(last));
 }
 
-__device__ __forceinline__ const uint32_t *hap_row(const glp_window &W,
-                                                   unsigned h) {
+__device__ __forceinline__ const uint32_t *hap_row(const glp_window &W, unsigned h) {
   return reinterpret_cast<const uint32_t *>(W.panel + (size_t)h * W.words_per_hap);
 }
 
 template <int BITS, bool MIXED>
-__device__ __forceinline__ double score_set(const glp_window &W,
-                                            const unsigned (&P)[4],
+__device__ __forceinline__ double score_set(const glp_window &W, const unsigned (&P)[4],
                                             unsigned sample, unsigned N,
-                                            const float4 *lut, float norm) {
-  return forward_pass<BITS, MIXED>(hap_row(W, P[0]), hap_row(W, P[1]),
-                                   hap_row(W, P[2]), hap_row(W, P[3]),
-                                   W.packed + sample, N, W.tran, W.num_sites,
-                                   lut, norm);
-}
-
-__device__ __forceinline__ void load_lut(float4 *s_lut,
-                                         const float4 *__restrict__ g_lut,
-                                         unsigned n) {
-  for (unsigned q = threadIdx.x; q < n; q += blockDim.x) s_lut[q] = g_lut[q];
-  __syncthreads();
+                                            const unsigned char *smem, unsigned lut_entries,
+                                            float norm) {
+  float *wtr = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) + 16u * lut_entries +
+                                         kSpreadBytes) + 96u * (threadIdx.x >> 6);
+  return forward_pass<BITS, MIXED>(hap_row(W, P[0]), hap_row(W, P[1]), hap_row(W, P[2]),
+                                   hap_row(W, P[3]), W.packed + sample, N, W.tran, W.num_sites,
+                                   smem, lut_entries, wtr, norm);
 }
 
 // MHSampler::SampleHap (MHSampler.hpp:82-96): accept a better or equal
@@ -240,17 +368,22 @@ __device__ __forceinline__ void proposal_draws(const KArgs &a, unsigned w,
 template <int BITS, bool MIXED>
 __global__ void __launch_bounds__(256)
     k_plain(const glp_window *__restrict__ wins, KArgs a, unsigned step) {
-  extern __shared__ float4 s_lut[];
-  load_lut(s_lut, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
+  extern __shared__ __align__(16) unsigned char smem[];
+  init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
   const unsigned w = blockIdx.y;
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
+  const unsigned ig = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = ig < a.N;
+  // the forward pass runs on whole wavefronts: lanes past N score a copy of
+  // sample N-1 and write nothing
+  if (__ballot(valid) == 0ull) return;
+  const unsigned i = valid ? ig : a.N - 1;
   const glp_window W = wins[w];
   const size_t row = (size_t)w * a.N + i;
   unsigned P[4];
   if (step == 0) {
     initial_parents(a, W, w, i, P);
-    const double v = score_set<BITS, MIXED>(W, P, i, a.N, s_lut, a.norm);
+    const double v = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+    if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) W.parents[4 * (size_t)i + q] = P[q];
     W.like[i] = v;
@@ -264,15 +397,17 @@ __global__ void __launch_bounds__(256)
   unsigned rp, cand;
   uint32_t ub;
   proposal_draws(a, w, i, c, rp, cand, ub);
-  const unsigned oh = P[rp];
-  P[rp] = cand;
-  const double prop = score_set<BITS, MIXED>(W, P, i, a.N, s_lut, a.norm);
+  P[0] = rp == 0 ? cand : P[0];
+  P[1] = rp == 1 ? cand : P[1];
+  P[2] = rp == 2 ? cand : P[2];
+  P[3] = rp == 3 ? cand : P[3];
+  const double prop = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+  if (!valid) return;
   const bool acc = mh_accept(prop, curr, (double)a.pen, ub);
   if (acc) {
     W.parents[4 * (size_t)i + rp] = cand;
     W.like[i] = prop;
   }
-  (void)oh;
   if (a.trace_like) a.trace_like[row * (a.C + 1) + 1 + c] = prop;
   if (a.trace_acc) a.trace_acc[row * a.C + c] = acc ? 1 : 0;
 }
@@ -280,54 +415,76 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // batched form: persistent work-queue kernel with speculative proposal windows
 // ---------------------------------------------------------------------------
-// Each wavefront holds 64/Q chain slots of Q lanes.  A slot owns one
-// (window, sample) task.  In a round, the slot's lanes score the next Q
-// undecided proposals (the first lane of a fresh task scores its initial
-// parents), all from the slot's current parent set S.  Then every lane of the
-// slot replays the serial MH walk over those Q values in proposal order.  A
-// value is the serial chain's value exactly when every slot of S changed in
-// this walk so far equals the proposal's own replace slot; the walk stops at
-// the first proposal for which that fails, and the next round re-scores from
-// there.  The result equals the serial loop of insti.cpp:1505-1539 draw for
-// draw; speculation only spends extra lane time.
-template <int BITS, bool MIXED, int Q>
+// Wavefronts walk the windows in `order` (longest first).  While on a window
+// a wavefront keeps its data (sites, tran, panel, codes) wave-uniform and
+// holds 64/Q chain slots of Q lanes (Q = qtab[window], a power of two).
+// Idle slots take the window's next samples from its task counter; when the
+// window runs dry and every slot is idle the wavefront moves on.
+//
+// A slot owns one (window, sample) task.  In a round the slot's lanes score
+// the next Q undecided proposals (the first lane of a fresh task scores its
+// initial parents), all from the slot's current parent set S.  Then every
+// lane of the slot replays the serial MH walk over those Q values in
+// proposal order.  A value is the serial chain's value exactly when every
+// slot of S changed in this walk so far equals the proposal's own replace
+// slot; the walk stops at the first proposal for which that fails, and the
+// next round re-scores from there.  The result equals the serial loop of
+// insti.cpp:1505-1539 draw for draw; speculation only spends lane time.
+template <int BITS, bool MIXED>
 __global__ void __launch_bounds__(256)
-    k_batched(const glp_window *__restrict__ wins,
-              const uint2 *__restrict__ tasks, unsigned ntasks,
+    k_batched(const glp_window *__restrict__ wins, const unsigned *__restrict__ order,
+              const unsigned *__restrict__ qtab, unsigned nwin, unsigned *__restrict__ wctr,
               unsigned long long *__restrict__ ctrs, KArgs a) {
-  extern __shared__ float4 s_lut[];
-  load_lut(s_lut, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
-  unsigned *queue = reinterpret_cast<unsigned *>(ctrs);
+  extern __shared__ __align__(16) unsigned char smem[];
+  init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
   const unsigned lane = threadIdx.x & 63u;
-  const unsigned j = lane % Q;
-  const unsigned sbase = lane - j;
+  const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double pen = (double)a.pen;
 
+  // wave-uniform window state
+  int wo = -1;  // position in `order`
+  unsigned bw = 0, Q = 1, j = 0, sbase = lane;
+  glp_window W{};
+  // slot state, replicated over the slot's lanes
   bool active = false, fresh = false;
-  unsigned w = 0, i = 0, kp = 0;
+  unsigned i = 0, kp = 0;
   unsigned S[4] = {0, 0, 0, 0};
   double cur = 0.0;
-  glp_window W{};
   unsigned long long n_eval = 0, n_useful = 0;
 
   for (;;) {
-    // 1. idle slots take the next task (longest windows first)
-    unsigned t = 0xFFFFFFFFu;
-    if (j == 0 && !active) t = atomicAdd(queue, 1u);
-    t = __shfl(t, sbase, 64);
-    if (!active && t < ntasks) {
-      const uint2 tk = tasks[t];
-      w = tk.x;
-      i = tk.y;
-      W = wins[w];
-      initial_parents(a, W, w, i, S);
-      active = true;
-      fresh = true;
-      kp = 0;
+    // 1. idle slots take the next samples of the current window
+    if (wo >= 0) {
+      const bool want = (j == 0) && !active;
+      const uint64_t wm = __ballot(want);
+      if (wm) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(&wctr[bw], (unsigned)__popcll(wm));
+        base = __builtin_amdgcn_readfirstlane(base);
+        const unsigned rank = (unsigned)__popcll(wm & lanes_below);
+        unsigned t = (want && base + rank < a.N) ? base + rank : 0xFFFFFFFFu;
+        t = __shfl(t, sbase, 64);
+        if (!active && t != 0xFFFFFFFFu) {
+          i = t;
+          initial_parents(a, W, bw, i, S);
+          active = true;
+          fresh = true;
+          kp = 0;
+        }
+      }
     }
-    if (!__any(active)) break;
+    if (!__any(active)) {
+      // 2. the window is done for this wavefront: move to the next one
+      if (++wo >= (int)nwin) break;
+      bw = __builtin_amdgcn_readfirstlane(order[wo]);
+      Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
+      j = lane & (Q - 1u);
+      sbase = lane - j;
+      W = wins[bw];
+      continue;
+    }
 
-    // 2. this lane's job: initial set or one speculative proposal
+    // 3. this lane's job: initial set or one speculative proposal
     bool work = false;
     unsigned rp = 4, cand = 0;
     uint32_t ub = 0;
@@ -338,29 +495,29 @@ __global__ void __launch_bounds__(256)
         work = true;
       } else if (kp + (unsigned)job < a.C) {
         work = true;
-        proposal_draws(a, w, i, kp + (unsigned)job, rp, cand, ub);
-        P[rp] = cand;
+        proposal_draws(a, bw, i, kp + (unsigned)job, rp, cand, ub);
+        P[0] = rp == 0 ? cand : P[0];
+        P[1] = rp == 1 ? cand : P[1];
+        P[2] = rp == 2 ? cand : P[2];
+        P[3] = rp == 3 ? cand : P[3];
       }
     }
-    double v = 0.0;
-    if (work) {
-      v = score_set<BITS, MIXED>(W, P, i, a.N, s_lut, a.norm);
-      ++n_eval;
-    }
+    // the whole wavefront runs the pass; lanes without work score their
+    // slot's current set and the value is ignored
+    double v = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+    if (work) ++n_eval;
 
-    // 3. serial MH walk over the slot's values (every lane of the slot
+    // 4. serial MH walk over the slot's values (every lane of the slot
     //    replays it, so the slot state stays replicated)
     const double v0 = __shfl(v, sbase, 64);
     const int jstart = fresh ? 1 : 0;
     if (active && fresh) {
       cur = v0;
-      if (j == 0 && a.trace_like)
-        a.trace_like[((size_t)w * a.N + i) * (a.C + 1)] = v0;
+      if (j == 0 && a.trace_like) a.trace_like[((size_t)bw * a.N + i) * (a.C + 1)] = v0;
     }
     unsigned changed = 0, decided = 0;
     bool stop = !active;
-#pragma unroll
-    for (int jj = 0; jj < Q; ++jj) {
+    for (int jj = 0; jj < (int)Q; ++jj) {
       const unsigned src = sbase + (unsigned)jj;
       const double vj = __shfl(v, src, 64);
       const unsigned rpj = __shfl(rp, src, 64);
@@ -382,7 +539,7 @@ __global__ void __launch_bounds__(256)
           }
           ++decided;
           if (j == (unsigned)jj) {
-            const size_t row = (size_t)w * a.N + i;
+            const size_t row = (size_t)bw * a.N + i;
             if (a.trace_like) a.trace_like[row * (a.C + 1) + 1 + kk] = vj;
             if (a.trace_acc) a.trace_acc[row * a.C + kk] = acc ? 1 : 0;
           }
@@ -414,8 +571,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void k_reset_ctrs(unsigned long long *ctrs) {
+// Clears the statistics and the per-window task counters before a run (a
+// kernel rather than a memset, so the run can be captured in a HIP graph).
+__global__ void k_reset_ctrs(unsigned long long *ctrs, unsigned *wctr, unsigned nwin) {
   if (threadIdx.x < 4) ctrs[threadIdx.x] = 0ull;
+  for (unsigned w = threadIdx.x; w < nwin; w += blockDim.x) wctr[w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -426,25 +586,37 @@ __global__ void __launch_bounds__(256)
     k_eval(const glp_window *__restrict__ wins, KArgs a,
            const unsigned *__restrict__ samples,
            const unsigned *__restrict__ parents, unsigned n, float *out) {
-  extern __shared__ float4 s_lut[];
-  load_lut(s_lut, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
-  const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
+  const unsigned eg = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = eg < n;
+  if (__ballot(valid) == 0ull) return;
+  const unsigned e = valid ? eg : n - 1;
   const glp_window W = wins[0];
   const unsigned P[4] = {parents[4 * (size_t)e], parents[4 * (size_t)e + 1],
                          parents[4 * (size_t)e + 2], parents[4 * (size_t)e + 3]};
-  out[e] = (float)score_set<BITS, false>(W, P, samples[e], a.N, s_lut, a.norm);
+  const float v = (float)score_set<BITS, false>(W, P, samples[e], a.N, smem, a.lut_entries, a.norm);
+  if (valid) out[e] = v;
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static size_t lut_smem(const KArgs &a) { return (size_t)a.lut_entries * 16; }
+size_t lds_bytes(const KArgs &a) { return (size_t)a.lut_entries * 16 + kSpreadBytes + 4 * 96 * 4; }
+
+static int allow_lds(const void *fn, size_t bytes) {
+  return hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+                   "dynamic LDS attribute");
+}
 
 int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
                  unsigned step, hipStream_t stream) {
   const dim3 block(256), grid((a.N + 255) / 256, nwin);
-  const size_t smem = lut_smem(a);
+  const size_t smem = lds_bytes(a);
+  const bool mixed_ = a.precision != kPrecF32;
+  const void *fn = a.bits == 8 ? (mixed_ ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>)
+                               : (mixed_ ? (const void *)&k_plain<4, true> : (const void *)&k_plain<4, false>);
+  GLP_TRY_K(allow_lds(fn, smem));
   const bool mixed = a.precision != kPrecF32;
   if (a.bits == 8) {
     if (mixed)
@@ -460,40 +632,26 @@ int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
   return hip_check(hipGetLastError(), "k_plain launch");
 }
 
-template <int BITS, bool MIXED>
-static void *batched_fn(unsigned Q) {
-  switch (Q) {
-    case 1: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 1>);
-    case 2: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 2>);
-    case 4: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 4>);
-    case 8: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 8>);
-    case 16: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 16>);
-    case 32: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 32>);
-    case 64: return reinterpret_cast<void *>(&k_batched<BITS, MIXED, 64>);
-  }
-  return nullptr;
+static const void *batched_kernel(unsigned bits, bool mixed) {
+  if (bits == 8) return mixed ? (const void *)&k_batched<8, true> : (const void *)&k_batched<8, false>;
+  return mixed ? (const void *)&k_batched<4, true> : (const void *)&k_batched<4, false>;
 }
 
-void *batched_kernel(unsigned bits, bool mixed, unsigned Q) {
-  if (bits == 8) return mixed ? batched_fn<8, true>(Q) : batched_fn<8, false>(Q);
-  return mixed ? batched_fn<4, true>(Q) : batched_fn<4, false>(Q);
-}
-
-int launch_batched(const KArgs &a, const glp_window *d_wins, const uint2 *d_tasks,
-                   unsigned ntasks, unsigned long long *d_ctrs, unsigned Q,
-                   unsigned grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_reset_ctrs, dim3(1), dim3(64), 0, stream, d_ctrs);
-  void *fn = batched_kernel(a.bits, a.precision != kPrecF32, Q);
-  if (!fn) return set_error(GLP_ERR_ARG, "lanes_per_chain must be a power of two <= 64");
-  void *args[] = {(void *)&d_wins, (void *)&d_tasks, (void *)&ntasks, (void *)&d_ctrs,
-                  (void *)&a};
-  hipError_t err = hipLaunchKernel(fn, dim3(grid), dim3(256), args, lut_smem(a), stream);
+int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
+                   const unsigned *d_qtab, unsigned nwin, unsigned *d_wctr,
+                   unsigned long long *d_ctrs, unsigned grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reset_ctrs, dim3(1), dim3(256), 0, stream, d_ctrs, d_wctr, nwin);
+  const void *fn = batched_kernel(a.bits, a.precision != kPrecF32);
+  void *args[] = {(void *)&d_wins, (void *)&d_order, (void *)&d_qtab, (void *)&nwin,
+                  (void *)&d_wctr, (void *)&d_ctrs, (void *)&a};
+  hipError_t err = hipLaunchKernel((void *)fn, dim3(grid), dim3(256), args, lds_bytes(a), stream);
   return hip_check(err, "k_batched launch");
 }
 
-int batched_occupancy(unsigned bits, bool mixed, unsigned Q, size_t smem, int *blocks_per_cu) {
-  void *fn = batched_kernel(bits, mixed, Q);
-  if (!fn) return set_error(GLP_ERR_ARG, "bad lanes_per_chain");
+int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu) {
+  const void *fn = batched_kernel(bits, mixed);
+  GLP_TRY_K(hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+                      "dynamic LDS attribute"));
   return hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 256, smem),
                    "occupancy query");
 }
@@ -501,11 +659,12 @@ int batched_occupancy(unsigned bits, bool mixed, unsigned Q, size_t smem, int *b
 int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samples,
                 const unsigned *d_parents, unsigned n, float *d_out, hipStream_t stream) {
   const dim3 block(256), grid((n + 255) / 256);
+  GLP_TRY_K(allow_lds(a.bits == 8 ? (const void *)&k_eval<8> : (const void *)&k_eval<4>, lds_bytes(a)));
   if (a.bits == 8)
-    hipLaunchKernelGGL((k_eval<8>), grid, block, lut_smem(a), stream, d_wins, a, d_samples,
+    hipLaunchKernelGGL((k_eval<8>), grid, block, lds_bytes(a), stream, d_wins, a, d_samples,
                        d_parents, n, d_out);
   else
-    hipLaunchKernelGGL((k_eval<4>), grid, block, lut_smem(a), stream, d_wins, a, d_samples,
+    hipLaunchKernelGGL((k_eval<4>), grid, block, lds_bytes(a), stream, d_wins, a, d_samples,
                        d_parents, n, d_out);
   return hip_check(hipGetLastError(), "k_eval launch");
 }

@@ -119,6 +119,19 @@ class Job:
         self.plan = p
         return self.lanes()
 
+    def autotune(self, gen=0, pen=1.0, seed=0):
+        """Runs the library's autotuner; returns lanes per chain per window."""
+        if self.plan is None:
+            self.make_plan()
+        out = np.zeros(self.W, np.uint32)
+        check(lib().glp_plan_autotune(self.plan, gen, pen, seed, self.stream_handle(),
+                                      ptr(out, c_u32p)), "glp_plan_autotune")
+        return out
+
+    def set_lanes(self, lanes):
+        q = np.ascontiguousarray(np.broadcast_to(np.asarray(lanes, np.uint32), (self.W,)))
+        check(lib().glp_plan_set_lanes(self.plan, ptr(q, c_u32p)), "glp_plan_set_lanes")
+
     def lanes(self):
         q, g = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().glp_plan_lanes(self.plan, ctypes.byref(q), ctypes.byref(g)), "glp_plan_lanes")

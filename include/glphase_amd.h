@@ -164,10 +164,11 @@ int glp_plain_init(const glp_mh_desc *desc, void *stream);
 int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
 
 /* Batched form: a persistent work-queue kernel runs init + all C proposals of
- * every (window, sample) chain; lanes_per_chain lanes evaluate consecutive
- * proposals of one chain speculatively from the same state, and a sequential
- * walk keeps exactly the serial MH result.  lanes_per_chain is one of
- * 1,2,4,8,16,32,64 (0 = automatic). */
+ * every (window, sample) chain, windows longest first; lanes_per_chain lanes
+ * evaluate consecutive proposals of one chain speculatively from the same
+ * state, and a sequential walk keeps exactly the serial MH result.
+ * lanes_per_chain is one of 1,2,4,8,16,32,64 for every window (0 = a
+ * default from the chain count; see glp_plan_autotune). */
 typedef struct glp_plan glp_plan;
 int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain,
                     glp_plan **out);
@@ -178,8 +179,19 @@ int glp_plan_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
 /* Counters of the last completed run: forward passes evaluated (useful and
  * speculative) and rounds.  Synchronises the plan's stream. */
 int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
-/* Lanes per chain and grid size the plan launches with. */
+/* Lanes per chain of the longest window and grid size the plan launches
+ * with. */
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
+/* Sets lanes per chain slot per window (powers of two <= 64). */
+int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
+/* Autotuner: for each window-size bucket (ceil(log2 sites)) runs one
+ * generation per candidate lanes-per-chain (1..64, at most the proposals + 1
+ * rounded up), times it with HIP events on `stream` and keeps the fastest.
+ * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
+ * trial.  lanes_out (optional) receives the choice per window.
+ * Synchronises `stream`. */
+int glp_plan_autotune(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
+                      void *stream, unsigned *lanes_out);
 /* Plain path over the plan's windows: one launch for the fresh parents, then
  * one launch per MH iteration (num_cycles launches), all stream-ordered. */
 int glp_plan_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,

@@ -120,6 +120,37 @@ def test_batched_mixed_precision(orc, lanes):
     compare_run(orc, p, job, C, pen, seed, PREC_MIXED)
 
 
+def test_autotune_keeps_parity_and_inputs(orc):
+    say("autotune")
+    from glphase_amd.engine import Job
+    rng = np.random.default_rng(5)
+    N = 48
+    sizes = [700, 300, 90]
+    probs = [helpers.synthetic(N, M, seed=200 + w) for w, M in enumerate(sizes)]
+    cb, pc = probs[0]["codebook"], probs[0]["pc"]
+    from glphase_amd import host
+    for q in probs:
+        q["codebook"], q["pc"] = cb, pc
+        q["packed"] = host.pack_codes(host.vq_codes(q["gls"], cb), 8)
+    wins = [dict(panel=q["panel"], packed=q["packed"], tran=q["tran"]) for q in probs]
+    C = 24
+    init = np.stack([rng.integers(0, probs[0]["H"], (N, 4)) for _ in probs]).astype(np.uint32)
+    extra = np.stack([rng.integers(0, probs[0]["H"], (C, N)) for _ in probs]).astype(np.uint32)
+    job = Job(wins, N, probs[0]["H"], cb, pc, bits=8, num_cycles=C, prop_mode=PROP_HOST,
+              init_parents=init, extra=extra, trace=True)
+    job.make_plan(0)
+    lanes = job.autotune(gen=1, pen=0.5, seed=3)
+    assert all(q in (1, 2, 4, 8, 16, 32) for q in lanes)
+    assert (job.parents() == init).all()  # host-mode inputs restored
+    job.run_batched(gen=1, pen=0.5, seed=3)
+    par = job.parents()
+    for w, q in enumerate(probs):
+        mh, _ = oracle_for(orc, q)
+        po, _ = mh.run(C, 0.5, 3, gen=1, window_id=w, prop_mode=2, init_parents=init[w],
+                       extra=extra[w])
+        assert (par[w] == po).all()
+
+
 def test_batched_zero_cycles(orc):
     say("batched C=0")
     p = helpers.synthetic(50, 64, seed=2)
