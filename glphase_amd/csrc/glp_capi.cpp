@@ -172,6 +172,13 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   a.bits = d->bits_per_code;
   a.lut_entries = 16u << d->bits_per_code;
   a.precision = d->precision;
+  // a wave walks every window once and serves each of its tasks in at most
+  // C + 1 rounds; bound the persistent loop well above that
+  {
+    const double bound = 2.0 * d->num_windows + 64.0 * (double)d->num_samples * d->num_windows *
+                                                    ((double)d->num_cycles + 2.0) + 1024.0;
+    a.max_rounds = bound > 4.0e9 ? 4000000000u : (unsigned)bound;
+  }
   return GLP_OK;
 }
 
@@ -388,6 +395,7 @@ int glp_plan_stats(glp_plan *p, uint64_t *evals, uint64_t *useful) {
   GLP_TRY(hip_check(hipMemcpy(h, p->ctrs.p, sizeof(h), hipMemcpyDeviceToHost), "stats copy"));
   if (evals) *evals = h[1];
   if (useful) *useful = h[2];
+  if (h[3]) return set_error(GLP_ERR_STATE, "batched kernel hit its round bound");
   return GLP_OK;
 }
 
@@ -618,6 +626,7 @@ int glp_run_hmm_on_device(glp_ctx *c, const uint64_t *hap_panel, size_t n_panel_
     rc = hip_check(hipMemcpyAsync(aos.data(), W.parents, aos.size() * 4, hipMemcpyDeviceToHost,
                                   c->stream), "parents copy back");
   if (rc == GLP_OK) rc = hip_check(hipStreamSynchronize(c->stream), "synchronize");
+  if (rc == GLP_OK) rc = glp_plan_stats(p, nullptr, nullptr);
   glp_plan_destroy(p);
   if (rc != GLP_OK) return rc;
   for (unsigned i = 0; i < num_samples; ++i)

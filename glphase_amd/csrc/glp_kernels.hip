@@ -233,7 +233,7 @@ __device__ __forceinline__ void fwd_chunk(FwdState &st, const uint32_t (&h)[4],
 // it together (M and tran wave-uniform): the lanes copy each 32-site chunk of
 // tran into the wave's LDS buffer wtr (96 floats), one chunk ahead.
 template <int BITS, bool MIXED>
-__device__ double forward_pass(const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+__device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
                                const uint32_t *__restrict__ m0, const uint32_t *__restrict__ m1,
                                const uint32_t *__restrict__ codes, size_t stride,
                                const float *__restrict__ tran, unsigned M,
@@ -451,8 +451,16 @@ __global__ void __launch_bounds__(256)
   unsigned S[4] = {0, 0, 0, 0};
   double cur = 0.0;
   unsigned long long n_eval = 0, n_useful = 0;
+  // every task needs at most C + 1 rounds; a wave can never run more than
+  // max_rounds (set by the host from the task count) -- if it does, it flags
+  // the run as failed and exits instead of spinning
+  unsigned rounds = 0;
 
   for (;;) {
+    if (++rounds > a.max_rounds) {
+      if (lane == 0) atomicOr(&ctrs[3], 1ull);
+      break;
+    }
     // 1. idle slots take the next samples of the current window
     if (wo >= 0) {
       const bool want = (j == 0) && !active;
