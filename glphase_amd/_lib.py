@@ -91,6 +91,9 @@ SIGNATURES = [
     ("glp_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("glp_selftest_recip", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.POINTER(ctypes.c_uint32)]),
     ("glp_model_params", ctypes.c_int,
      [ctypes.c_uint, ctypes.c_uint, c_f64p, ctypes.c_float, c_f32p, c_f32p, c_f32p]),
     ("glp_vq_train", ctypes.c_int,

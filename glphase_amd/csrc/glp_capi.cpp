@@ -47,8 +47,9 @@ void emit_table(const float *codebook, unsigned ncode, const float *pc, float *E
   }
 }
 
-// (code, nib) -> (e00, e01, e10, e11), nib = f0 | f1<<1 | m0<<2 | m1<<3,
-// e_xy = E[code][(f_x << 1) | m_y].
+// (code, nib) -> (e00, e11, e01, e10), nib = f0 | f1<<1 | m0<<2 | m1<<3,
+// e_xy = E[code][(f_x << 1) | m_y]: the states of the even lane of a lane
+// pair first, then those of the odd lane.
 void nibble_table(const float *E, unsigned ncode, float *out) {
   for (unsigned c = 0; c < ncode; ++c)
     for (unsigned nib = 0; nib < 16; ++nib) {
@@ -56,9 +57,9 @@ void nibble_table(const float *E, unsigned ncode, float *out) {
                      m1 = (nib >> 3) & 1u;
       float *o = out + 4 * (16 * (size_t)c + nib);
       o[0] = E[4 * c + ((f0 << 1) | m0)];
-      o[1] = E[4 * c + ((f0 << 1) | m1)];
-      o[2] = E[4 * c + ((f1 << 1) | m0)];
-      o[3] = E[4 * c + ((f1 << 1) | m1)];
+      o[1] = E[4 * c + ((f1 << 1) | m1)];
+      o[2] = E[4 * c + ((f0 << 1) | m1)];
+      o[3] = E[4 * c + ((f1 << 1) | m0)];
     }
 }
 
@@ -102,24 +103,28 @@ struct glp_plan {
   KArgs a{};
   unsigned nwin = 0, grid = 1;
   std::vector<glp_window> wins_h;
-  std::vector<unsigned> order, q;  // windows longest first; lanes per chain slot
+  std::vector<unsigned> order, q;  // windows longest first; passes per chain round
   DevBuf wins, order_d, qtab, wctr, ctrs;
 };
 
-// Lanes per chain slot when none is given: one MH round decides ~6 proposals
-// at the ~15% acceptance of sampling generations, so depth 8 wastes little;
-// deeper only when there are too few chains to fill the device.
+// Passes per chain round when none is given (each pass is a lane pair): one
+// MH round decides ~6 proposals at the ~15% acceptance of sampling
+// generations, so depth 8 wastes little; deeper only when there are too few
+// chains to fill the device.
+// a wavefront holds 32 lane pairs
+constexpr unsigned kMaxPasses = 32;
+
 static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu) {
-  const double target = (double)num_cu * 4.0 * 64.0;  // one wave per SIMD
+  const double target = (double)num_cu * 4.0 * 32.0;  // one wave per SIMD
   unsigned Q = 8;
-  while (Q < 64 && (double)nchains * Q < target) Q *= 2;
+  while (Q < kMaxPasses && (double)nchains * Q < target) Q *= 2;
   while (Q > 1 && Q / 2 >= C + 1) Q /= 2;
   return Q;
 }
 
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
   double lanes = 0;
-  for (unsigned w : wins) lanes += (double)p->a.N * p->q[w];
+  for (unsigned w : wins) lanes += 2.0 * p->a.N * p->q[w];
   const unsigned blocks = (unsigned)std::ceil(lanes / 256.0);
   return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
 }
@@ -236,8 +241,8 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   KArgs a;
   GLP_TRY(fill_args(desc, a));
   if (lanes_per_chain != 0 &&
-      (lanes_per_chain > 64 || (lanes_per_chain & (lanes_per_chain - 1)) != 0))
-    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 or a power of two <= 64");
+      (lanes_per_chain > kMaxPasses || (lanes_per_chain & (lanes_per_chain - 1)) != 0))
+    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 or a power of two <= 32");
   glp_plan *p = new glp_plan();
   p->a = a;
   int rc = hip_check(hipGetDevice(&p->dev), "hipGetDevice");
@@ -295,8 +300,8 @@ int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
   if (!p || !lanes_per_window) return set_error(GLP_ERR_ARG, "null argument");
   for (unsigned w = 0; w < p->nwin; ++w) {
     const unsigned Q = lanes_per_window[w];
-    if (Q == 0 || Q > 64 || (Q & (Q - 1)) != 0)
-      return set_error(GLP_ERR_ARG, "lanes per chain must be a power of two <= 64");
+    if (Q == 0 || Q > kMaxPasses || (Q & (Q - 1)) != 0)
+      return set_error(GLP_ERR_ARG, "lanes per chain must be a power of two <= 32");
   }
   p->q.assign(lanes_per_window, lanes_per_window + p->nwin);
   p->grid = plan_grid(p, p->order);
@@ -352,7 +357,7 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
     if (rc != GLP_OK) break;
     unsigned best = p->q[b[0]];
     float best_ms = 1e30f;
-    for (unsigned Q = 1; Q <= 64 && rc == GLP_OK; Q *= 2) {
+    for (unsigned Q = 1; Q <= kMaxPasses && rc == GLP_OK; Q *= 2) {
       if (Q > 1 && Q / 2 >= a.C + 1) break;
       for (unsigned w : b) p->q[w] = Q;
       rc = plan_upload_q(p);

@@ -66,13 +66,16 @@ __device__ __forceinline__ float cr_logf(float x) {
 // LDS tables
 // ---------------------------------------------------------------------------
 // Emission table: the 16-byte entry at byte (code << 8) | (nib << 4) holds
-// (e00, e01, e10, e11) of a site with GL code `code` whose 4 parent alleles
+// (e00, e11, e01, e10) of a site with GL code `code` whose 4 parent alleles
 // are nib = f0 | f1 << 1 | m0 << 2 | m1 << 3, e_xy = FillEmit(code)[(f_x << 1)
-// | m_y] (hmmLike.cu:39-44,124-127).
+// | m_y] (hmmLike.cu:39-44,124-127).  The even lane of a chain's lane pair
+// reads the first 8 bytes, the odd lane the last 8.
 // Spread table: for parent p and an allele byte b of 8 consecutive sites,
 // entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
 // in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
 constexpr unsigned kSpreadBytes = 4 * 256 * 8;
+constexpr unsigned kTranFloats = 96;   // per wave: tran of one 32-site chunk
+constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
 
 __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
                                          unsigned lut_entries) {
@@ -92,81 +95,102 @@ __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__re
   __syncthreads();
 }
 
-typedef const __attribute__((address_space(4))) float cfloat;
+// ---------------------------------------------------------------------------
+// lane pairs and the exact reciprocal
+// ---------------------------------------------------------------------------
+// A chain's forward pass runs on two adjacent lanes: the even lane carries
+// states 00 and 11, the odd lane 01 and 10.  The other lane's values arrive
+// through DPP quad permutations folded into the consuming add or multiply,
+// so every float operation of the serial recurrence happens once per lane,
+// in the reference's order, with the reference's operands.
+constexpr int kDppSwap = 0xB1;  // quad_perm [1,0,3,2]: the partner lane
+constexpr int kDppEven = 0xA0;  // quad_perm [0,0,2,2]: the pair's even lane
+constexpr int kDppOdd = 0xF5;   // quad_perm [1,1,3,3]: the pair's odd lane
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+
+// 1.0f / x rounded to nearest-even, for 2^-149 <= x < 2^-20.  The operand is
+// scaled by 2^64 (exact), inverted by v_rcp_f32 plus one Newton step with
+// fused multiply-adds, and scaled back (exact, or the same overflow to inf
+// as the division).  glp_selftest_recip checks it against the IEEE division
+// for every float of that range.  For x = 0 it gives NaN where the division
+// gives inf; a zero site sum turns the probabilities into NaN (0 * inf) in
+// the reference as well, so the returned likelihood is NaN either way.
+__device__ __forceinline__ float recip_small(float x) {
+  const float xs = x * 0x1p64f;
+  const float r = __builtin_amdgcn_rcpf(xs);
+  const float e = __builtin_fmaf(-xs, r, 1.0f);
+  return __builtin_fmaf(e, r, r) * 0x1p64f;
+}
 
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
-// Forward state of one lane.  A rescale (impute.cpp:367-372) scales the
-// probabilities at once but only queues the scale factor; the logs of queued
-// factors are taken in their original order at chunk ends (or when the
-// 4-entry queue is full), so the score is the same sum of the same terms as
-// the serial code while the wavefront's lanes, which rescale at different
-// sites, share one pass of log evaluations.
-struct FwdState {
-  float l00, l01, l10, l11;
+// State of one lane: its two probabilities, and the record of rescale
+// factors (impute.cpp:367-372).  The factor of a site that does not rescale
+// is 1.0f, so a rescale is a select rather than a branch and the probability
+// update (x * 1.0f == x) is exact either way.
+//   F32 (the reference's arithmetic): the factors go to a per-lane LDS ring
+//   and their correctly rounded logs are subtracted, oldest first, at the end
+//   of each 32-site chunk -- the same float sum in the same order.
+//   MIXED: the factors are multiplied into an FP64 product whose binary
+//   exponent is split off every 8 sites, and the score is
+//   log(last sum) - (log(mantissa) + exponent * ln 2) in FP64.
+struct PairState {
+  float la, lb;
+  double prod;
+  int pexp;
   float fscore;
-  double dscore;
-  float q0, q1, q2, q3;  // queued factors, q0 newest
   unsigned nq;
 };
 
+constexpr double kLn2 = 0.6931471805599453;
+
+// One site (impute.cpp:354-372) for the lane's two states; e = this lane's
+// two emissions.  ring: this lane's first ring slot (stride 64 floats).
 template <bool MIXED>
-__device__ __forceinline__ void add_log(FwdState &st, float inv) {
-  if (MIXED)
-    st.dscore -= log((double)inv);
-  else
-    st.fscore -= cr_logf(inv);
+__device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t0, float t1,
+                                          float t2, float nrm, float *ring) {
+  const float la = st.la, lb = st.lb;
+  const float pown = la + lb;                 // l00 + l11 (even) / l01 + l10 (odd)
+  const float pt = dpp<kDppSwap>(pown) * t1;  // the partner's pair sum * t1
+  const float ba = la * t0 + pt + lb * t2;    // b00 / b01
+  const float bb = lb * t0 + pt + la * t2;    // b11 / b10
+  const float na = ba * e.x, nb = bb * e.y;
+  float s = dpp<kDppEven>(na) + dpp<kDppOdd>(na);  // l00 + l01
+  s = dpp<kDppOdd>(nb) + s;                        // + l10
+  s = dpp<kDppEven>(nb) + s;                       // + l11
+  const bool r = s < nrm;
+  const float sc = r ? recip_small(s) : 1.0f;
+  st.la = na * sc;
+  st.lb = nb * sc;
+  if (MIXED) {
+    st.prod *= (double)sc;
+  } else {
+    ring[64u * st.nq] = sc;
+    st.nq += r ? 1u : 0u;
+  }
 }
 
-// Takes the logs of the queued factors, oldest first.
-template <bool MIXED>
-__device__ __forceinline__ void flush_logs(FwdState &st) {
-  if (__any(st.nq >= 4u) && st.nq >= 4u) add_log<MIXED>(st, st.q3);
-  if (__any(st.nq >= 3u) && st.nq >= 3u) add_log<MIXED>(st, st.q2);
-  if (__any(st.nq >= 2u) && st.nq >= 2u) add_log<MIXED>(st, st.q1);
-  if (__any(st.nq >= 1u) && st.nq >= 1u) add_log<MIXED>(st, st.q0);
+// The pair's l00 + l01 + l10 + l11 in the reference's order.
+__device__ __forceinline__ float pair_sum(const PairState &st) {
+  float s = dpp<kDppEven>(st.la) + dpp<kDppOdd>(st.la);
+  s = dpp<kDppOdd>(st.lb) + s;
+  return dpp<kDppEven>(st.lb) + s;
+}
+
+// F32: subtract the logs of the ring's factors, oldest first.
+__device__ __forceinline__ void flush_ring(PairState &st, const float *ring) {
+  for (unsigned k = 0; __any(k < st.nq); ++k)
+    if (k < st.nq) st.fscore -= cr_logf(ring[64u * k]);
   st.nq = 0;
 }
 
-// One site of the recurrence (impute.cpp:354-372) with emissions e.
-template <bool MIXED>
-__device__ __forceinline__ void site_step(FwdState &st, const float4 e, float t0, float t1,
-                                          float t2, float norm) {
-  const float l00 = st.l00, l01 = st.l01, l10 = st.l10, l11 = st.l11;
-  const float b00 = l00 * t0 + (l01 + l10) * t1 + l11 * t2;
-  const float b01 = l01 * t0 + (l00 + l11) * t1 + l10 * t2;
-  const float b10 = l10 * t0 + (l00 + l11) * t1 + l01 * t2;
-  const float b11 = l11 * t0 + (l01 + l10) * t1 + l00 * t2;
-  float n00 = b00 * e.x, n01 = b01 * e.y, n10 = b10 * e.z, n11 = b11 * e.w;
-  float sum = n00 + n01 + n10 + n11;
-  if (sum < norm) {
-    sum = 1.0f / sum;
-    if (st.nq == 4u) {  // queue full: drain it in order (rare)
-      add_log<MIXED>(st, st.q3);
-      add_log<MIXED>(st, st.q2);
-      add_log<MIXED>(st, st.q1);
-      add_log<MIXED>(st, st.q0);
-      st.nq = 0;
-    }
-    st.q3 = st.q2;
-    st.q2 = st.q1;
-    st.q1 = st.q0;
-    st.q0 = sum;
-    ++st.nq;
-    n00 *= sum;
-    n01 *= sum;
-    n10 *= sum;
-    n11 *= sum;
-  }
-  st.l00 = n00;
-  st.l01 = n01;
-  st.l10 = n10;
-  st.l11 = n11;
-}
-
-// Byte address of site s's emission entry: byte 0 = (nib << 4) from the
-// spread word, byte 1 = the site's 8-bit code.
+// Byte address of site s's emission half-entry: byte 0 = (nib << 4) | 8 *
+// (odd lane) from the spread word, byte 1 = the site's code.
 template <int BITS>
 __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned s) {
   if (BITS == 8) {
@@ -176,19 +200,20 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
     return __builtin_amdgcn_perm(cw, nbw, sel);
   } else {
     const uint32_t code = (cw >> (28u - 4u * (s & 7u))) & 15u;
-    return (code << 8) | ((nbw >> (8u * (s & 3u))) & 0xF0u);
+    return (code << 8) | ((nbw >> (8u * (s & 3u))) & 0xF8u);
   }
 }
 
-// 32 sites starting at chunk site 0 (FIRST: the window's site 0, which only
-// initialises, impute.cpp:344-347).  h = the 4 parents' 32-bit allele words,
-// cw = the chunk's code words, tr = tran of the chunk's first site
-// (wave-uniform), n = sites in this chunk.
-template <int BITS, bool MIXED, bool FIRST, bool TAIL>
-__device__ __forceinline__ void fwd_chunk(FwdState &st, const uint32_t (&h)[4],
+// 32 sites of a chunk.  h = the 4 parents' 32-bit allele words, cw = the
+// chunk's code words, tr = the wave's LDS copy of the chunk's tran, n =
+// sites in the chunk (TAIL: n < 32), nrm0 = the threshold of the chunk's
+// first site (0 at the window's site 0, which only initialises,
+// impute.cpp:339-347).
+template <int BITS, bool MIXED, bool TAIL>
+__device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
                                           const unsigned char *smem, const uint2 *spread,
-                                          float norm) {
+                                          uint32_t halfbits, float norm, float nrm0, float *ring) {
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     if (TAIL && (unsigned)(8 * g) >= n) break;
@@ -196,11 +221,11 @@ __device__ __forceinline__ void fwd_chunk(FwdState &st, const uint32_t (&h)[4],
     const uint2 s1 = spread[1 * 256 + ((h[1] >> (8 * g)) & 255u)];
     const uint2 s2 = spread[2 * 256 + ((h[2] >> (8 * g)) & 255u)];
     const uint2 s3 = spread[3 * 256 + ((h[3] >> (8 * g)) & 255u)];
-    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x, s0.y | s1.y | s2.y | s3.y};
+    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x | halfbits,
+                            s0.y | s1.y | s2.y | s3.y | halfbits};
     constexpr int CPW = 32 / BITS;
-    // emission entries and transitions of the group, loaded ahead of the
-    // recurrence (tr is the wave's LDS copy of the chunk's tran)
-    float4 e[8];
+    // emissions and transitions of the group, loaded ahead of the recurrence
+    float2 e[8];
     float4 t4[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
@@ -210,44 +235,47 @@ __device__ __forceinline__ void fwd_chunk(FwdState &st, const uint32_t (&h)[4],
       const int s = 8 * g + k;
       if (TAIL && (unsigned)s >= n) break;
       const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
-      e[k] = *reinterpret_cast<const float4 *>(smem + a);
+      e[k] = *reinterpret_cast<const float2 *>(smem + a);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int s = 8 * g + k;
       if (TAIL && (unsigned)s >= n) break;
-      if (FIRST && s == 0) {
-        st.l00 = 0.25f * e[k].x;
-        st.l01 = 0.25f * e[k].y;
-        st.l10 = 0.25f * e[k].z;
-        st.l11 = 0.25f * e[k].w;
-        continue;
-      }
-      site_step<MIXED>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], norm);
+      pair_step<MIXED>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
+                       ring);
+    }
+    if (MIXED) {
+      // at most 8 factors below 2^127 since the last split: no overflow
+      const int ex = __builtin_amdgcn_frexp_exp(st.prod);
+      st.prod = __builtin_amdgcn_frexp_mant(st.prod);
+      st.pexp += ex;
     }
   }
 }
 
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
-// pair m0, m1; rows as 32-bit words).  Every lane of the wavefront must call
-// it together (M and tran wave-uniform): the lanes copy each 32-site chunk of
-// tran into the wave's LDS buffer wtr (96 floats), one chunk ahead.
+// pair m0, m1; rows as 32-bit words), computed by a lane pair; both lanes
+// return it.  Every lane of the wavefront must call it together (M and tran
+// wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
+// LDS buffer wtr, one chunk ahead.
 template <int BITS, bool MIXED>
 __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
                                const uint32_t *__restrict__ m0, const uint32_t *__restrict__ m1,
                                const uint32_t *__restrict__ codes, size_t stride,
                                const float *__restrict__ tran, unsigned M,
                                const unsigned char *smem, unsigned lut_entries, float *wtr,
-                               float norm) {
+                               float *ring, float norm) {
   constexpr int NW = BITS;  // code words per 32 sites
   constexpr int CPW = 32 / BITS;
   const uint2 *spread = reinterpret_cast<const uint2 *>(smem + 16u * lut_entries);
   const unsigned lane = threadIdx.x & 63u;
+  const uint32_t halfbits = (lane & 1u) ? 0x08080808u : 0u;
   const unsigned ntr = 3u * M;
-  FwdState st;
+  PairState st;
+  st.la = st.lb = 0.25f;
+  st.prod = 1.0;
+  st.pexp = 0;
   st.fscore = 0.0f;
-  st.dscore = 0.0;
-  st.q0 = st.q1 = st.q2 = st.q3 = 1.0f;
   st.nq = 0;
   const unsigned nch = (M + 31u) >> 5;
   const unsigned nwords = (M + CPW - 1) / CPW;
@@ -255,15 +283,19 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, 
   uint32_t cw[NW];
 #pragma unroll
   for (int q = 0; q < NW; ++q) cw[q] = (unsigned)q < nwords ? codes[q * stride] : 0u;
-  // lane l < 48 carries floats 2l, 2l+1 of a chunk's 96 tran values
+  // lane l < 48 carries floats 2l, 2l+1 of a chunk's 96 tran values; site
+  // 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step gives
+  // l_xy = 1/4 * e_xy exactly
   float2 tp = make_float2(0.0f, 0.0f);
   if (lane < 48u && 2u * lane + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + 2u * lane);
+  if (lane == 0u) tp = make_float2(1.0f, 0.0f);
+  if (lane == 1u) tp.x = 0.0f;
   for (unsigned ch = 0; ch < nch; ++ch) {
     if (lane < 48u) reinterpret_cast<float2 *>(wtr)[lane] = tp;
     // prefetch the next chunk's tran, allele and code words
     const unsigned nx = ch + 1;
     {
-      const unsigned o = 96u * nx + 2u * lane;
+      const unsigned o = kTranFloats * nx + 2u * lane;
       tp = make_float2(0.0f, 0.0f);
       if (lane < 48u && o + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + o);
     }
@@ -281,25 +313,19 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, 
     }
     const unsigned base = ch << 5;
     const unsigned n = M - base < 32u ? M - base : 32u;
-    if (n == 32) {
-      if (ch == 0)
-        fwd_chunk<BITS, MIXED, true, false>(st, h, cw, wtr, 32, smem, spread, norm);
-      else
-        fwd_chunk<BITS, MIXED, false, false>(st, h, cw, wtr, 32, smem, spread, norm);
-    } else {
-      if (ch == 0)
-        fwd_chunk<BITS, MIXED, true, true>(st, h, cw, wtr, n, smem, spread, norm);
-      else
-        fwd_chunk<BITS, MIXED, false, true>(st, h, cw, wtr, n, smem, spread, norm);
-    }
-    flush_logs<MIXED>(st);
+    const float nrm0 = ch == 0 ? 0.0f : norm;
+    if (n == 32)
+      fwd_chunk<BITS, MIXED, false>(st, h, cw, wtr, 32, smem, spread, halfbits, norm, nrm0, ring);
+    else
+      fwd_chunk<BITS, MIXED, true>(st, h, cw, wtr, n, smem, spread, halfbits, norm, nrm0, ring);
+    if (!MIXED) flush_ring(st, ring);
 #pragma unroll
     for (int q = 0; q < 4; ++q) h[q] = hn[q];
 #pragma unroll
     for (int q = 0; q < NW; ++q) cw[q] = cn[q];
   }
-  const float last = st.l00 + st.l01 + st.l10 + st.l11;
-  if (MIXED) return st.dscore + log((double)last);
+  const float last = pair_sum(st);
+  if (MIXED) return log((double)last) - (log(st.prod) + (double)st.pexp * kLn2);
   return (double)(st.fscore + cr_logf(last));
 }
 
@@ -307,16 +333,21 @@ __device__ __forceinline__ const uint32_t *hap_row(const glp_window &W, unsigned
   return reinterpret_cast<const uint32_t *>(W.panel + (size_t)h * W.words_per_hap);
 }
 
+// LDS layout after the two tables: per wave, kTranFloats floats of tran,
+// then (F32 only) per wave a ring of kRingSlots x 64 floats.
 template <int BITS, bool MIXED>
 __device__ __forceinline__ double score_set(const glp_window &W, const unsigned (&P)[4],
                                             unsigned sample, unsigned N,
                                             const unsigned char *smem, unsigned lut_entries,
                                             float norm) {
-  float *wtr = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) + 16u * lut_entries +
-                                         kSpreadBytes) + 96u * (threadIdx.x >> 6);
+  const unsigned wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  float *tables_end = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) +
+                                                16u * lut_entries + kSpreadBytes);
+  float *wtr = tables_end + kTranFloats * wave;
+  float *ring = tables_end + kTranFloats * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
   return forward_pass<BITS, MIXED>(hap_row(W, P[0]), hap_row(W, P[1]), hap_row(W, P[2]),
                                    hap_row(W, P[3]), W.packed + sample, N, W.tran, W.num_sites,
-                                   smem, lut_entries, wtr, norm);
+                                   smem, lut_entries, wtr, ring, norm);
 }
 
 // MHSampler::SampleHap (MHSampler.hpp:82-96): accept a better or equal
@@ -363,7 +394,7 @@ __device__ __forceinline__ void proposal_draws(const KArgs &a, unsigned w,
 }
 
 // ---------------------------------------------------------------------------
-// plain path: one lane per (window, sample), one launch per MH iteration
+// plain path: one lane pair per (window, sample), one launch per MH iteration
 // ---------------------------------------------------------------------------
 template <int BITS, bool MIXED>
 __global__ void __launch_bounds__(256)
@@ -371,12 +402,12 @@ __global__ void __launch_bounds__(256)
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
   const unsigned w = blockIdx.y;
-  const unsigned ig = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = ig < a.N;
-  // the forward pass runs on whole wavefronts: lanes past N score a copy of
-  // sample N-1 and write nothing
-  if (__ballot(valid) == 0ull) return;
-  const unsigned i = valid ? ig : a.N - 1;
+  const unsigned ig = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+  // the forward pass runs on whole wavefronts: pairs past N score a copy of
+  // sample N-1 and write nothing; the even lane of a pair writes
+  const bool valid = ig < a.N && (threadIdx.x & 1u) == 0u;
+  if (__ballot(ig < a.N) == 0ull) return;
+  const unsigned i = ig < a.N ? ig : a.N - 1;
   const glp_window W = wins[w];
   const size_t row = (size_t)w * a.N + i;
   unsigned P[4];
@@ -417,12 +448,13 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Wavefronts walk the windows in `order` (longest first).  While on a window
 // a wavefront keeps its data (sites, tran, panel, codes) wave-uniform and
-// holds 64/Q chain slots of Q lanes (Q = qtab[window], a power of two).
+// holds 32/Q chain slots of Q lane pairs (Q = qtab[window], a power of two
+// <= 32); a lane pair scores one parent set per round.
 // Idle slots take the window's next samples from its task counter; when the
 // window runs dry and every slot is idle the wavefront moves on.
 //
-// A slot owns one (window, sample) task.  In a round the slot's lanes score
-// the next Q undecided proposals (the first lane of a fresh task scores its
+// A slot owns one (window, sample) task.  In a round the slot's pairs score
+// the next Q undecided proposals (the first pair of a fresh task scores its
 // initial parents), all from the slot's current parent set S.  Then every
 // lane of the slot replays the serial MH walk over those Q values in
 // proposal order.  A value is the serial chain's value exactly when every
@@ -438,12 +470,14 @@ __global__ void __launch_bounds__(256)
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
   const unsigned lane = threadIdx.x & 63u;
+  const bool even = (lane & 1u) == 0u;
   const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double pen = (double)a.pen;
 
-  // wave-uniform window state
+  // wave-uniform window state; j = this pair's index in its slot, sbase =
+  // the slot's first lane
   int wo = -1;  // position in `order`
-  unsigned bw = 0, Q = 1, j = 0, sbase = lane;
+  unsigned bw = 0, Q = 1, j = 0, sbase = lane & ~1u;
   glp_window W{};
   // slot state, replicated over the slot's lanes
   bool active = false, fresh = false;
@@ -463,7 +497,7 @@ __global__ void __launch_bounds__(256)
     }
     // 1. idle slots take the next samples of the current window
     if (wo >= 0) {
-      const bool want = (j == 0) && !active;
+      const bool want = lane == sbase && !active;
       const uint64_t wm = __ballot(want);
       if (wm) {
         unsigned base = 0;
@@ -486,8 +520,8 @@ __global__ void __launch_bounds__(256)
       if (++wo >= (int)nwin) break;
       bw = __builtin_amdgcn_readfirstlane(order[wo]);
       Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
-      j = lane & (Q - 1u);
-      sbase = lane - j;
+      j = (lane >> 1) & (Q - 1u);
+      sbase = (lane & ~1u) - 2u * j;
       W = wins[bw];
       continue;
     }
@@ -513,7 +547,7 @@ __global__ void __launch_bounds__(256)
     // the whole wavefront runs the pass; lanes without work score their
     // slot's current set and the value is ignored
     double v = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
-    if (work) ++n_eval;
+    if (work && even) ++n_eval;
 
     // 4. serial MH walk over the slot's values (every lane of the slot
     //    replays it, so the slot state stays replicated)
@@ -521,12 +555,12 @@ __global__ void __launch_bounds__(256)
     const int jstart = fresh ? 1 : 0;
     if (active && fresh) {
       cur = v0;
-      if (j == 0 && a.trace_like) a.trace_like[((size_t)bw * a.N + i) * (a.C + 1)] = v0;
+      if (lane == sbase && a.trace_like) a.trace_like[((size_t)bw * a.N + i) * (a.C + 1)] = v0;
     }
     unsigned changed = 0, decided = 0;
     bool stop = !active;
     for (int jj = 0; jj < (int)Q; ++jj) {
-      const unsigned src = sbase + (unsigned)jj;
+      const unsigned src = sbase + 2u * (unsigned)jj;
       const double vj = __shfl(v, src, 64);
       const unsigned rpj = __shfl(rp, src, 64);
       const unsigned cj = __shfl(cand, src, 64);
@@ -546,7 +580,7 @@ __global__ void __launch_bounds__(256)
             changed |= 1u << rpj;
           }
           ++decided;
-          if (j == (unsigned)jj) {
+          if (j == (unsigned)jj && even) {
             const size_t row = (size_t)bw * a.N + i;
             if (a.trace_like) a.trace_like[row * (a.C + 1) + 1 + kk] = vj;
             if (a.trace_acc) a.trace_acc[row * a.C + kk] = acc ? 1 : 0;
@@ -557,9 +591,9 @@ __global__ void __launch_bounds__(256)
     if (active) {
       kp += decided;
       fresh = false;
-      if (j == 0) n_useful += decided + (jstart ? 1u : 0u);
+      if (lane == sbase) n_useful += decided + (jstart ? 1u : 0u);
       if (kp >= a.C) {
-        if (j == 0) {
+        if (lane == sbase) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) W.parents[4 * (size_t)i + q] = S[q];
           W.like[i] = cur;
@@ -596,10 +630,10 @@ __global__ void __launch_bounds__(256)
            const unsigned *__restrict__ parents, unsigned n, float *out) {
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
-  const unsigned eg = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = eg < n;
-  if (__ballot(valid) == 0ull) return;
-  const unsigned e = valid ? eg : n - 1;
+  const unsigned eg = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+  if (__ballot(eg < n) == 0ull) return;
+  const bool valid = eg < n && (threadIdx.x & 1u) == 0u;
+  const unsigned e = eg < n ? eg : n - 1;
   const glp_window W = wins[0];
   const unsigned P[4] = {parents[4 * (size_t)e], parents[4 * (size_t)e + 1],
                          parents[4 * (size_t)e + 2], parents[4 * (size_t)e + 3]};
@@ -607,10 +641,28 @@ __global__ void __launch_bounds__(256)
   if (valid) out[e] = v;
 }
 
+// Every float with bit pattern in [lo, hi): recip_small against the division.
+__global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *bad,
+                                 uint32_t *first) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; b < hi; b += stride) {
+    const float x = __uint_as_float((uint32_t)b);
+    if (__float_as_uint(recip_small(x)) != __float_as_uint(1.0f / x)) {
+      atomicAdd(bad, 1ull);
+      atomicMin(first, (uint32_t)b);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t lds_bytes(const KArgs &a) { return (size_t)a.lut_entries * 16 + kSpreadBytes + 4 * 96 * 4; }
+size_t lds_bytes(const KArgs &a) {
+  const size_t waves = 4;  // 256-thread blocks
+  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * kTranFloats * 4;
+  if (a.precision == kPrecF32) b += waves * kRingSlots * 64 * 4;
+  return b;
+}
 
 static int allow_lds(const void *fn, size_t bytes) {
   return hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
@@ -619,7 +671,7 @@ static int allow_lds(const void *fn, size_t bytes) {
 
 int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
                  unsigned step, hipStream_t stream) {
-  const dim3 block(256), grid((a.N + 255) / 256, nwin);
+  const dim3 block(256), grid((2 * a.N + 255) / 256, nwin);
   const size_t smem = lds_bytes(a);
   const bool mixed_ = a.precision != kPrecF32;
   const void *fn = a.bits == 8 ? (mixed_ ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>)
@@ -666,15 +718,40 @@ int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu
 
 int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samples,
                 const unsigned *d_parents, unsigned n, float *d_out, hipStream_t stream) {
-  const dim3 block(256), grid((n + 255) / 256);
-  GLP_TRY_K(allow_lds(a.bits == 8 ? (const void *)&k_eval<8> : (const void *)&k_eval<4>, lds_bytes(a)));
+  const dim3 block(256), grid((2 * n + 255) / 256);
+  KArgs af = a;  // k_eval accumulates in F32 (Impute::hmm_like returns a float)
+  af.precision = kPrecF32;
+  const size_t smem = lds_bytes(af);
+  GLP_TRY_K(allow_lds(a.bits == 8 ? (const void *)&k_eval<8> : (const void *)&k_eval<4>, smem));
   if (a.bits == 8)
-    hipLaunchKernelGGL((k_eval<8>), grid, block, lds_bytes(a), stream, d_wins, a, d_samples,
+    hipLaunchKernelGGL((k_eval<8>), grid, block, smem, stream, d_wins, af, d_samples,
                        d_parents, n, d_out);
   else
-    hipLaunchKernelGGL((k_eval<4>), grid, block, lds_bytes(a), stream, d_wins, a, d_samples,
+    hipLaunchKernelGGL((k_eval<4>), grid, block, smem, stream, d_wins, af, d_samples,
                        d_parents, n, d_out);
   return hip_check(hipGetLastError(), "k_eval launch");
 }
 
 }  // namespace glp
+
+extern "C" int glp_selftest_recip(uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,
+                                  uint32_t *first_bad) {
+  using namespace glp;
+  if (!mismatches || !first_bad) return set_error(GLP_ERR_ARG, "null output");
+  if (hi_bits < lo_bits) return set_error(GLP_ERR_ARG, "hi_bits < lo_bits");
+  unsigned long long *d = nullptr;
+  GLP_TRY_K(hip_check(hipMalloc(&d, 16), "hipMalloc"));
+  unsigned long long init[2] = {0ull, 0xFFFFFFFFull};
+  int rc = hip_check(hipMemcpy(d, init, 16, hipMemcpyHostToDevice), "self-test copy");
+  if (rc == GLP_OK) {
+    hipLaunchKernelGGL(k_selftest_recip, dim3(4096), dim3(256), 0, nullptr, lo_bits, hi_bits, d,
+                       reinterpret_cast<uint32_t *>(d + 1));
+    rc = hip_check(hipGetLastError(), "self-test launch");
+  }
+  if (rc == GLP_OK) rc = hip_check(hipMemcpy(init, d, 16, hipMemcpyDeviceToHost), "self-test copy");
+  (void)hipFree(d);
+  if (rc != GLP_OK) return rc;
+  *mismatches = init[0];
+  *first_bad = (uint32_t)init[1];
+  return GLP_OK;
+}

@@ -164,11 +164,12 @@ int glp_plain_init(const glp_mh_desc *desc, void *stream);
 int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
 
 /* Batched form: a persistent work-queue kernel runs init + all C proposals of
- * every (window, sample) chain, windows longest first; lanes_per_chain lanes
- * evaluate consecutive proposals of one chain speculatively from the same
- * state, and a sequential walk keeps exactly the serial MH result.
- * lanes_per_chain is one of 1,2,4,8,16,32,64 for every window (0 = a
- * default from the chain count; see glp_plan_autotune). */
+ * every (window, sample) chain, windows longest first; each round,
+ * lanes_per_chain forward passes (each on a pair of lanes) evaluate
+ * consecutive proposals of one chain speculatively from the same state, and
+ * a sequential walk keeps exactly the serial MH result.  lanes_per_chain is
+ * one of 1,2,4,8,16,32 for every window (0 = a default from the chain count;
+ * see glp_plan_autotune). */
 typedef struct glp_plan glp_plan;
 int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain,
                     glp_plan **out);
@@ -182,10 +183,10 @@ int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
 /* Lanes per chain of the longest window and grid size the plan launches
  * with. */
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
-/* Sets lanes per chain slot per window (powers of two <= 64). */
+/* Sets passes per chain round per window (powers of two <= 32). */
 int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
 /* Autotuner: for each window-size bucket (ceil(log2 sites)) runs one
- * generation per candidate lanes-per-chain (1..64, at most the proposals + 1
+ * generation per candidate lanes-per-chain (1..32, at most the proposals + 1
  * rounded up), times it with HIP events on `stream` and keeps the fastest.
  * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
  * trial.  lanes_out (optional) receives the choice per window.
@@ -197,6 +198,14 @@ int glp_plan_autotune(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
 int glp_plan_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                        void *stream);
 int glp_plan_destroy(glp_plan *plan);
+
+/* Self-test of the device reciprocal the rescale step uses (no counterpart
+ * in the reference): compares it with the IEEE division 1.0f / x for every
+ * float x with bit pattern in [lo_bits, hi_bits) on the current device.
+ * mismatches receives the count, first_bad the lowest mismatching pattern
+ * (0xFFFFFFFF if none).  Synchronous. */
+int glp_selftest_recip(uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,
+                       uint32_t *first_bad);
 
 /* ===== 3. host-side helpers of the drop-in ================================ */
 

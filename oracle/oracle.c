@@ -142,6 +142,8 @@ static const uint32_t orc_logf_hard[20][2] = {
     {0x65d890d3u, 0x4254d1f9u}, {0x66a8c860u, 0x42595e46u},
     {0x6f31a8ecu, 0x42845a89u}, {0x79e7ec37u, 0x42a1ffb7u}};
 
+#define ORC_LN2 0.6931471805599453
+
 float orc_logf_cr(float x) {
   uint32_t b;
   memcpy(&b, &x, 4);
@@ -190,8 +192,13 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
   return score + orc_logf_cr(l00 + l01 + l10 + l11);
 }
 
-/* The same FP32 recurrence; the log of the scale factors and the final log
- * are accumulated in FP64. */
+/* The same FP32 recurrence and rescale decisions; the scale factors (1.0f at
+ * sites that do not rescale) are multiplied into an FP64 product whose binary
+ * exponent is split off after every 8th site and after the last one, and
+ * the score is log(last sum) - (log(mantissa) + exponent * ln 2) in FP64.
+ * Mathematically this is the reference's score -= log(factor) sum
+ * (impute.cpp:367-374) carried in FP64; the split points keep the product
+ * finite (at most 8 factors below 2^127 between splits). */
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M) {
@@ -200,12 +207,17 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
   const float norm = orc_norm();
   const float *e = emit, *t = tran;
   float sum;
-  double score = 0;
+  double prod = 1.0;
+  int pexp = 0, ex;
   float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
         l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
   float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
         l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
   float b00, b01, b10, b11;
+  if (M == 1) {
+    prod = frexp(prod, &ex);
+    pexp += ex;
+  }
   e += 4;
   t += 3;
   for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
@@ -219,14 +231,18 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
     l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
     if ((sum = l00 + l01 + l10 + l11) < norm) {
       sum = 1.0f / sum;
-      score -= log((double)sum);
+      prod *= (double)sum;
       l00 *= sum;
       l01 *= sum;
       l10 *= sum;
       l11 *= sum;
     }
+    if ((m & 7u) == 7u || m == M - 1) {
+      prod = frexp(prod, &ex);
+      pexp += ex;
+    }
   }
-  return score + log((double)(l00 + l01 + l10 + l11));
+  return log((double)(l00 + l01 + l10 + l11)) - (log(prod) + (double)pexp * ORC_LN2);
 }
 
 /* FP64 forward oracle: the same model, normalised at every site. */
