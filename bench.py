@@ -63,34 +63,59 @@ def time_steps(torch, fn, steps, warmup, stream_events=True):
     return start.elapsed_time(stop) / 1e3, wall
 
 
-def cpu_baseline(w, C, pen, seed, budget_s=10.0, precision=1):
-    """The oracle (C restatement of Insti::solve) on this host, OpenMP over
-    samples on all cores and single-threaded; bounded sample of chains."""
+def cpu_baseline(w, C, pen, seed, budget_s=12.0, precision=1):
+    """The oracle (C restatement of Insti::solve, -O3) on this host's cores:
+    whole generations of the same workload under OpenMP over samples,
+    repeated until about budget_s seconds of work, plus a single-threaded run
+    over a prefix of the samples."""
     from oracle import bind as orc
     lut = orc.emit_lut(w["codebook"], w["pc"])
     mh = orc.MH(w["panel"], w["packed"], 8, lut, w["tran"], w["N"])
     se_per_chain = (C + 1) * w["M"]
-    # single core
-    n1 = 4
+    # single core, about budget_s / 4 seconds
+    n1 = 8
     t0 = time.perf_counter()
     mh.run(C, pen, seed, precision=precision, s0=0, s1=n1)
     dt1 = time.perf_counter() - t0
-    n1 = int(min(w["N"], max(4, budget_s / 3 / max(dt1 / n1, 1e-6))))
+    n1 = int(min(w["N"], max(8, n1 * budget_s / 4 / max(dt1, 1e-6))))
     t0 = time.perf_counter()
     mh.run(C, pen, seed, precision=precision, s0=0, s1=n1)
     dt1 = time.perf_counter() - t0
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
-    nall = int(min(w["N"], max(cores, n1 * cores * (budget_s * 2 / 3) / max(dt1, 1e-6) / 2)))
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = min(cores, os.cpu_count() or 1)
+    gens, chains = 0, 0
     t0 = time.perf_counter()
-    mh.run(C, pen, seed, precision=precision, s0=0, s1=nall, nthreads=cores)
+    while gens == 0 or time.perf_counter() - t0 < budget_s:
+        mh.run(C, pen, seed, gen=gens, precision=precision, s0=0, s1=w["N"], nthreads=cores)
+        gens += 1
+        chains += w["N"]
     dta = time.perf_counter() - t0
     return {
-        "value": se_per_chain * nall / dta, "unit": "site-evals/s", "cores": cores, "kind": "port",
-        "sample": f"{nall} of {w['N']} chains x {C + 1} evaluations x {w['M']} sites "
-                  f"({dta:.1f}s, OpenMP over samples)",
+        "value": se_per_chain * chains / dta, "unit": "site-evals/s", "cores": cores,
+        "kind": "port",
+        "sample": f"{gens} generation(s) of all {w['N']} chains x {C + 1} evaluations x "
+                  f"{w['M']} sites ({dta:.1f}s, OpenMP over samples)",
         "single_core": {"value": se_per_chain * n1 / dt1, "cores": 1,
-                        "sample": f"{n1} chains ({dt1:.1f}s)"},
+                        "sample": f"{n1} chains of one generation ({dt1:.1f}s)"},
     }
+
+
+def recorded_traffic(workload):
+    """HBM bytes per dispatch of the batched kernel for this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
+    copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_batched_hbm_traffic.md")
+    prefix = "traffic_bytes_per_dispatch:"
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(prefix) and "=" in line:
+                    key, val = line[len(prefix):].rsplit("=", 1)
+                    if key.strip() == workload:
+                        return float(val)
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def copy_calibration(nbytes):
@@ -178,6 +203,8 @@ def main():
     log(f"plain: {pms:.3f} ms/step, {pvalue:.3e} SE/s")
 
     achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
+    workload_key = f"{N}x{M}x{C}"
+    traffic = recorded_traffic(workload_key)
     out = {
         "metric": "HMM site-evals/s (sample x proposal x site), batched MH generation",
         "value": value,
@@ -200,8 +227,9 @@ def main():
                   "batched_over_plain": value / pvalue},
         "speculation": {"chain_passes": evals, "useful_passes": useful},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "flop_per_site_eval": FLOP_PER_SE},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "flop_per_site_eval": FLOP_PER_SE,
+                     "hbm_gbps": None if traffic is None else traffic / (ms / 1e3) / 1e9},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
