@@ -34,6 +34,32 @@ class GlpWindow(ctypes.Structure):
     ]
 
 
+class GlpWorkWindow(ctypes.Structure):
+    _fields_ = [
+        ("num_sites", ctypes.c_uint),
+        ("words_per_hap", ctypes.c_uint),
+        ("panel", ctypes.c_void_p),
+        ("parents", ctypes.c_void_p),
+        ("tran", ctypes.c_void_p),
+        ("het_aa", ctypes.c_void_p),
+        ("hnew", ctypes.c_void_p),
+    ]
+
+
+class GlpWorkDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_windows", ctypes.c_uint),
+        ("windows", ctypes.POINTER(GlpWorkWindow)),
+        ("num_samples", ctypes.c_uint),
+        ("num_haps", ctypes.c_uint),
+        ("pc16", ctypes.c_float * 16),
+        ("pen", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+        ("gen", ctypes.c_uint),
+        ("workspace", ctypes.c_void_p),
+    ]
+
+
 class GlpMhDesc(ctypes.Structure):
     _fields_ = [
         ("num_windows", ctypes.c_uint),
@@ -91,6 +117,8 @@ SIGNATURES = [
     ("glp_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("glp_work_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(GlpWorkDesc)]),
+    ("glp_hmm_work", ctypes.c_int, [ctypes.POINTER(GlpWorkDesc), ctypes.c_void_p]),
     ("glp_selftest_recip", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(ctypes.c_uint32)]),

@@ -8,7 +8,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import GlpMhDesc, GlpWindow, c_f32p, c_u32p, c_u64p, check, lib, ptr
+from ._lib import (GlpMhDesc, GlpWindow, GlpWorkDesc, GlpWorkWindow, c_f32p, c_u32p, c_u64p,
+                   check, lib, ptr)
 
 
 def _torch():
@@ -189,3 +190,61 @@ class Job:
             self.close()
         except Exception:
             pass
+
+
+class Resampler:
+    """Impute::hmm_work on the device for the windows of a Job: after an MH
+    generation (Job.run_batched / run_plain) it samples every sample's two
+    new haplotypes from the chosen parents.
+
+    het_aa: per window, the input GL pairs (Het, AA) as a host array
+    [N][M][2] (Insti's prob layout); they are stored site-major on the device.
+    pc: the 4 x 4 mutation matrix.
+    """
+
+    def __init__(self, job, het_aa, pc):
+        torch = job.torch
+        self.job = job
+        self.torch = torch
+        assert len(het_aa) == job.W
+        self.q_d, self.hnew_d = [], []
+        wins = (GlpWorkWindow * job.W)()
+        for w in range(job.W):
+            M = job.M[w]
+            wn = (M + 63) // 64
+            q = np.asarray(het_aa[w], np.float32)
+            assert q.shape == (job.N, M, 2)
+            qd = _dev(torch, np.ascontiguousarray(q.transpose(1, 0, 2)), job.device)
+            hn = torch.zeros((2 * job.N, wn), dtype=torch.int64, device=job.device)
+            self.q_d.append(qd)
+            self.hnew_d.append(hn)
+            panel, tran = job.keep[3 * w], job.keep[3 * w + 2]
+            wins[w] = GlpWorkWindow(M, wn, panel.data_ptr(), job.parents_d[w].data_ptr(),
+                                    tran.data_ptr(), qd.data_ptr(), hn.data_ptr())
+        self.wins = wins
+        d = GlpWorkDesc()
+        d.num_windows = job.W
+        d.windows = ctypes.cast(wins, ctypes.POINTER(GlpWorkWindow))
+        d.num_samples = job.N
+        d.num_haps = job.H
+        for k, v in enumerate(np.asarray(pc, np.float32).reshape(16)):
+            d.pc16[k] = float(v)
+        nbytes = lib().glp_work_workspace_bytes(ctypes.byref(d))
+        self.ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=job.device)
+        d.workspace = self.ws.data_ptr()
+        self.desc = d
+
+    def run(self, pen=1.0, seed=0, gen=0):
+        self.desc.pen, self.desc.seed, self.desc.gen = pen, seed, gen
+        check(lib().glp_hmm_work(ctypes.byref(self.desc), self.job.stream_handle()), "glp_hmm_work")
+
+    def hnew(self):
+        self.torch.cuda.synchronize(self.job.device)
+        return np.stack([h.cpu().numpy().view(np.uint64) for h in self.hnew_d])
+
+    def adopt(self):
+        """Copies each window's new sample haplotypes into rows [0, 2N) of its
+        panel (Insti's haps/hnew swap, insti.cpp:1455-1458); rows past 2N
+        (reference haplotypes) are kept."""
+        for w in range(self.job.W):
+            self.job.keep[3 * w][: 2 * self.job.N].copy_(self.hnew_d[w])

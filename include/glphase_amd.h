@@ -207,6 +207,45 @@ int glp_plan_destroy(glp_plan *plan);
 int glp_selftest_recip(uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,
                        uint32_t *first_bad);
 
+/* ----- haplotype resampling (Impute::hmm_work, impute.cpp:380-485) ------- */
+
+/* One window of the resampling step that follows the MH update of a
+ * generation (insti.cpp:1448-1452).  All pointers are device pointers. */
+typedef struct {
+  unsigned num_sites;
+  unsigned words_per_hap;    /* ceil(num_sites / 64) */
+  const uint64_t *panel;     /* [num_haps][wn] haplotypes the parents index */
+  const unsigned *parents;   /* [N][4] chosen parents (f0, f1, m0, m1) */
+  const float *tran;         /* [M][3] */
+  const float *het_aa;       /* [M][N][2] input GL pairs (Het, AA), site-major */
+  uint64_t *hnew;            /* out: [2N][wn], rows 2i, 2i+1 = sample i's */
+} glp_work_window;
+
+typedef struct {
+  unsigned num_windows;
+  const glp_work_window *windows; /* host array of num_windows */
+  unsigned num_samples;           /* samples per window */
+  unsigned num_haps;              /* rows of each panel */
+  float pc16[16];                 /* mutation matrix pc[i][j] at 4 i + j */
+  float pen;                      /* S of impute.cpp:457-469 */
+  uint64_t seed;
+  unsigned gen;
+  void *workspace;                /* device, >= glp_work_workspace_bytes */
+} glp_work_desc;
+
+/* Device workspace the resampling needs: 16 B per sample and site of every
+ * window (the scaled backward probabilities). */
+size_t glp_work_workspace_bytes(const glp_work_desc *desc);
+
+/* Impute::hmm_work for every sample of every window: scaled backward
+ * recursion, forward pass with per-site posteriors, genotype weights
+ * powf(prob_g * sum_xy p_xy pc[s_xy][g], pen) and one draw per site (word
+ * m & 3 of Philox block 0x80000000 | (m >> 2) for (sample, window, gen),
+ * replacing gsl_rng_uniform).  prob and emissions come from the GL pairs as
+ * in Insti::initialize (insti.cpp:1345-1378).  Writes every hnew row in
+ * full (bits past num_sites are 0).  Stream-ordered. */
+int glp_hmm_work(const glp_work_desc *desc, void *stream);
+
 /* ===== 3. host-side helpers of the drop-in ================================ */
 
 /* Insti::initialize parameter builders (insti.cpp:1254-1315): mu from the

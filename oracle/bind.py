@@ -64,6 +64,12 @@ def L():
         lib.orc_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.c_uint32, u32p]
         lib.orc_tran_from_morgans.argtypes = [f64p, ctypes.c_uint, ctypes.c_float, f32p]
+        lib.orc_hmm_work_weights.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
+                                             ctypes.c_float, ctypes.c_uint, f32p, f32p]
+        lib.orc_hmm_work_sample.argtypes = [f32p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint64,
+                                            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            u64p, u64p]
+        lib.orc_prob_emit.argtypes = [f32p, ctypes.c_uint, f32p, f32p, f32p]
         lib.orc_hmm_work.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
                                      ctypes.c_float, ctypes.c_uint, ctypes.c_uint64,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p]
@@ -263,3 +269,39 @@ def hmm_work(haps, parents, emit, tran, prob, pc_, S, seed, sample, window, gen)
     L().orc_hmm_work(P(h, u64p), wn, P(pr, u32p), P(e, f32p), P(t, f32p), P(p, f32p), P(c, f32p),
                      S, e.shape[0], seed, sample, window, gen, P(ha, u64p), P(hb, u64p))
     return ha, hb
+
+
+def hmm_work_weights(haps, parents, emit, tran, prob, pc_, S):
+    """Genotype weights c [M][4] and unnormalised state posteriors [M][4]."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    pr = np.ascontiguousarray(parents, np.uint32)
+    e = np.ascontiguousarray(emit, np.float32)
+    t = np.ascontiguousarray(tran, np.float32)
+    p = np.ascontiguousarray(prob, np.float32)
+    c16 = np.ascontiguousarray(pc_, np.float32).reshape(16)
+    M = e.shape[0]
+    c = np.zeros((M, 4), np.float32)
+    post = np.zeros((M, 4), np.float32)
+    L().orc_hmm_work_weights(P(h, u64p), h.shape[1], P(pr, u32p), P(e, f32p), P(t, f32p),
+                             P(p, f32p), P(c16, f32p), S, M, P(c, f32p), P(post, f32p))
+    return c, post
+
+
+def hmm_work_sample(c, wn, seed, sample, window, gen):
+    cc = np.ascontiguousarray(c, np.float32)
+    ha = np.zeros(wn, np.uint64)
+    hb = np.zeros(wn, np.uint64)
+    L().orc_hmm_work_sample(P(cc, f32p), cc.shape[0], wn, seed, sample, window, gen,
+                            P(ha, u64p), P(hb, u64p))
+    return ha, hb
+
+
+def prob_emit(het_aa, pc_):
+    """Insti::initialize's (prob [M][3], emit [M][4]) from (Het, AA) pairs [M][2]."""
+    q = np.ascontiguousarray(het_aa, np.float32)
+    c16 = np.ascontiguousarray(pc_, np.float32).reshape(16)
+    M = q.shape[0]
+    prob = np.zeros((M, 3), np.float32)
+    emit = np.zeros((M, 4), np.float32)
+    L().orc_prob_emit(P(q, f32p), M, P(c16, f32p), P(prob, f32p), P(emit, f32p))
+    return prob, emit

@@ -646,11 +646,15 @@ static inline void set_bit(uint64_t *h, unsigned s, unsigned v) {
     h[s >> 6] &= ~((uint64_t)1 << (s & 63));
 }
 
-void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
-                  const float *emit, const float *tran, const float *prob,
-                  const float *pc, float S, unsigned M, uint64_t seed,
-                  uint32_t sample, uint32_t window, uint32_t gen, uint64_t *ha,
-                  uint64_t *hb) {
+/* Impute::hmm_work's weights (impute.cpp:380-470): the scaled backward
+ * recursion, then the forward pass normalised at every site, the state
+ * posteriors p_xy = l_xy * beta_xy, and per site the 4 genotype weights
+ * c[m][g] = powf(prob_g * sum_xy p_xy * pc[s_xy][g], S), g = (ha << 1) | hb,
+ * prob_g = prob[m][0], [1], [1], [2].  post (optional) receives the
+ * unnormalised p_xy. */
+void orc_hmm_work_weights(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                          const float *emit, const float *tran, const float *prob,
+                          const float *pc, float S, unsigned M, float *c, float *post) {
   const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
                  *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
   float *beta = (float *)malloc(sizeof(float) * 4 * (size_t)M);
@@ -674,13 +678,10 @@ void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
     b[2] *= sum;
     b[3] *= sum;
   }
-  for (unsigned w = 0; w < wn; ++w)
-    ha[w] = hb[w] = 0;
   const float *p = prob;
   e = emit;
   t = tran;
   b = beta;
-  uint32_t x[4] = {0, 0, 0, 0};
   for (unsigned m = 0; m < M; m++, e += 4, t += 3, p += 3, b += 4) {
     unsigned s00 = (hbit(f0, m) << 1) | hbit(m0, m);
     unsigned s01 = (hbit(f0, m) << 1) | hbit(m1, m);
@@ -708,28 +709,49 @@ void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
     l11 *= sum;
     float p00 = l00 * b[0], p01 = l01 * b[1], p10 = l10 * b[2],
           p11 = l11 * b[3];
-    float c00 = orc_powf(p[0] * (p00 * pc[4 * s00 + 0] + p01 * pc[4 * s01 + 0] +
+    if (post) {
+      post[4 * (size_t)m + 0] = p00;
+      post[4 * (size_t)m + 1] = p01;
+      post[4 * (size_t)m + 2] = p10;
+      post[4 * (size_t)m + 3] = p11;
+    }
+    float *cm = c + 4 * (size_t)m;
+    cm[0] = orc_powf(p[0] * (p00 * pc[4 * s00 + 0] + p01 * pc[4 * s01 + 0] +
                              p10 * pc[4 * s10 + 0] + p11 * pc[4 * s11 + 0]),
                      S);
-    float c01 = orc_powf(p[1] * (p00 * pc[4 * s00 + 1] + p01 * pc[4 * s01 + 1] +
+    cm[1] = orc_powf(p[1] * (p00 * pc[4 * s00 + 1] + p01 * pc[4 * s01 + 1] +
                              p10 * pc[4 * s10 + 1] + p11 * pc[4 * s11 + 1]),
                      S);
-    float c10 = orc_powf(p[1] * (p00 * pc[4 * s00 + 2] + p01 * pc[4 * s01 + 2] +
+    cm[2] = orc_powf(p[1] * (p00 * pc[4 * s00 + 2] + p01 * pc[4 * s01 + 2] +
                              p10 * pc[4 * s10 + 2] + p11 * pc[4 * s11 + 2]),
                      S);
-    float c11 = orc_powf(p[2] * (p00 * pc[4 * s00 + 3] + p01 * pc[4 * s01 + 3] +
+    cm[3] = orc_powf(p[2] * (p00 * pc[4 * s00 + 3] + p01 * pc[4 * s01 + 3] +
                              p10 * pc[4 * s10 + 3] + p11 * pc[4 * s11 + 3]),
                      S);
-    if ((m & 3u) == 0)
-      orc_draw(seed, 0x80000000u | (m >> 2), sample, window, gen, x);
-    sum = (float)(orc_u01(x[m & 3u]) * (c00 + c01 + c10 + c11));
-    if (sum < c00) {
+  }
+  free(beta);
+}
+
+/* The sampling step of impute.cpp:472-484: per site one uniform u (word
+ * m & 3 of the Philox block kWorkTag | (m >> 2), which replaces
+ * gsl_rng_uniform), x = (float)(u * (c00 + c01 + c10 + c11)); the first g
+ * with x < c00 + ... + c_g sets (ha, hb) = (g >> 1, g & 1). */
+void orc_hmm_work_sample(const float *c, unsigned M, unsigned wn, uint64_t seed,
+                         uint32_t sample, uint32_t window, uint32_t gen, uint64_t *ha,
+                         uint64_t *hb) {
+  uint32_t x[4] = {0, 0, 0, 0};
+  for (unsigned w = 0; w < wn; ++w) ha[w] = hb[w] = 0;
+  for (unsigned m = 0; m < M; ++m) {
+    const float *cm = c + 4 * (size_t)m;
+    if ((m & 3u) == 0) orc_draw(seed, 0x80000000u | (m >> 2), sample, window, gen, x);
+    const float sum = (float)(orc_u01(x[m & 3u]) * (cm[0] + cm[1] + cm[2] + cm[3]));
+    if (sum < cm[0]) {
       set_bit(ha, m, 0);
       set_bit(hb, m, 0);
-    } else if (sum < c00 + c01) {
+    } else if (sum < cm[0] + cm[1]) {
       set_bit(ha, m, 0);
       set_bit(hb, m, 1);
-    } else if (sum < c00 + c01 + c10) {
+    } else if (sum < cm[0] + cm[1] + cm[2]) {
       set_bit(ha, m, 1);
       set_bit(hb, m, 0);
     } else {
@@ -737,5 +759,33 @@ void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
       set_bit(hb, m, 1);
     }
   }
-  free(beta);
+}
+
+void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                  const float *emit, const float *tran, const float *prob,
+                  const float *pc, float S, unsigned M, uint64_t seed,
+                  uint32_t sample, uint32_t window, uint32_t gen, uint64_t *ha,
+                  uint64_t *hb) {
+  float *c = (float *)malloc(sizeof(float) * 4 * (size_t)M);
+  orc_hmm_work_weights(haps, wn, P, emit, tran, prob, pc, S, M, c, NULL);
+  orc_hmm_work_sample(c, M, wn, seed, sample, window, gen, ha, hb);
+  free(c);
+}
+
+/* Insti::initialize's per-site genotype probabilities and emissions
+ * (insti.cpp:1345-1378, autosomal case): from the input pair (Het, AA),
+ * t = (max(1 - Het - AA, 0), max(Het, 0), max(AA, 0)) and
+ * e[j] = pc[j][0] t0 + pc[j][1] t1 + pc[j][2] t1 + pc[j][3] t2. */
+void orc_prob_emit(const float *het_aa, unsigned M, const float *pc, float *prob, float *emit) {
+  for (unsigned m = 0; m < M; ++m) {
+    const float *q = het_aa + 2 * (size_t)m;
+    float *t = prob + 3 * (size_t)m, *e = emit + 4 * (size_t)m;
+    const float t0r = 1 - q[0] - q[1];
+    t[0] = t0r > 0.0f ? t0r : 0.0f;
+    t[1] = q[0] > 0.0f ? q[0] : 0.0f;
+    t[2] = q[1] > 0.0f ? q[1] : 0.0f;
+    for (unsigned j = 0; j < 4; ++j)
+      e[j] = pc[4 * j + 0] * t[0] + pc[4 * j + 1] * t[1] + pc[4 * j + 2] * t[1] +
+             pc[4 * j + 3] * t[2];
+  }
 }

@@ -128,11 +128,19 @@ void orc_tran_from_morgans(const double *r_morgans, unsigned M,
                            float inflation, float *tran);
 
 /* ---- hmm_work: forward-backward haplotype sampling ----------------------- */
+void orc_hmm_work_weights(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                          const float *emit, const float *tran, const float *prob,
+                          const float *pc, float S, unsigned M, float *c, float *post);
+void orc_hmm_work_sample(const float *c, unsigned M, unsigned wn, uint64_t seed,
+                         uint32_t sample, uint32_t window, uint32_t gen, uint64_t *ha,
+                         uint64_t *hb);
 void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
                   const float *emit, const float *tran, const float *prob,
                   const float *pc, float S, unsigned M, uint64_t seed,
                   uint32_t sample, uint32_t window, uint32_t gen,
                   uint64_t *ha, uint64_t *hb);
+/* per-site genotype probabilities and emissions from (Het, AA) GL pairs */
+void orc_prob_emit(const float *het_aa, unsigned M, const float *pc, float *prob, float *emit);
 
 #ifdef __cplusplus
 }
