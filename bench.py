@@ -44,7 +44,7 @@ def make_workload(N, M, C, depth, seed):
     packed = host.pack_codes(codes, 8)
     log(f"workload N={N} M={M} C={C}: generated in {time.time() - t0:.1f}s (VQ {iters} iterations)")
     return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, N=N, M=M,
-                H=panel.shape[0])
+                H=panel.shape[0], het_aa=np.ascontiguousarray(d["gls"][:, :, 1:3]))
 
 
 def time_steps(torch, fn, steps, warmup, stream_events=True):
@@ -188,6 +188,31 @@ def main():
     log(f"batched: {ms:.3f} ms/step, {value:.3e} SE/s (wall {wall:.3f}s); "
         f"evaluated {evals} chain passes for {useful} useful")
 
+    # the same generation through the plan's HIP graph
+    ggen = [0]
+
+    def gstep():
+        job.run_batched_graph(gen=ggen[0], pen=pen, seed=seed)
+        ggen[0] += 1
+
+    gsecs, _ = time_steps(torch, gstep, args.steps, 1)
+    gms = gsecs / args.steps * 1e3
+    log(f"batched (HIP graph): {gms:.3f} ms/step")
+
+    # haplotype resampling (Impute::hmm_work) after the generation, and the
+    # whole generation (MH update + resampling)
+    from glphase_amd.engine import Resampler
+    rs = Resampler(job, [w["het_aa"]], w["pc"])
+    wgen = [0]
+
+    def wstep():
+        rs.run(pen=pen, seed=seed, gen=wgen[0])
+        wgen[0] += 1
+
+    wsecs, _ = time_steps(torch, wstep, args.steps, 1)
+    wms = wsecs / args.steps * 1e3
+    log(f"hmm_work: {wms:.3f} ms/step")
+
     # plain path on the same inputs: one launch per MH iteration
     job_p = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED)
     job_p.make_plan(1)
@@ -201,6 +226,16 @@ def main():
     pms = psecs / args.plain_steps * 1e3
     pvalue = se_step * args.plain_steps / psecs
     log(f"plain: {pms:.3f} ms/step, {pvalue:.3e} SE/s")
+
+    pg = [0]
+
+    def pgstep():
+        job_p.run_plain_graph(gen=pg[0], pen=pen, seed=seed)
+        pg[0] += 1
+
+    pgsecs, _ = time_steps(torch, pgstep, args.plain_steps, 1)
+    pgms = pgsecs / args.plain_steps * 1e3
+    log(f"plain (HIP graph): {pgms:.3f} ms/step")
 
     achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
     workload_key = f"{N}x{M}x{C}"
@@ -224,7 +259,10 @@ def main():
         "mh_iterations_per_s": 1e3 / ms,
         "proposals_per_s": N * C / (ms / 1e3),
         "plain": {"value": pvalue, "ms_per_step": pms, "launches_per_step": C + 1,
-                  "batched_over_plain": value / pvalue},
+                  "batched_over_plain": value / pvalue, "graph_ms_per_step": pgms},
+        "graph": {"batched_ms_per_step": gms, "plain_ms_per_step": pgms},
+        "hmm_work": {"ms_per_step": wms, "generation_ms": ms + wms,
+                     "note": "Impute::hmm_work for all samples after the MH update"},
         "speculation": {"chain_passes": evals, "useful_passes": useful},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,

@@ -117,6 +117,10 @@ SIGNATURES = [
     ("glp_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("glp_plan_graph_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float,
+                                          ctypes.c_uint64, ctypes.c_void_p]),
+    ("glp_plan_graph_run_plain", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float,
+                                                ctypes.c_uint64, ctypes.c_void_p]),
     ("glp_work_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(GlpWorkDesc)]),
     ("glp_hmm_work", ctypes.c_int, [ctypes.POINTER(GlpWorkDesc), ctypes.c_void_p]),
     ("glp_selftest_recip", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32,

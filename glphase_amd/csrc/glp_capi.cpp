@@ -105,7 +105,22 @@ struct glp_plan {
   std::vector<glp_window> wins_h;
   std::vector<unsigned> order, q;  // windows longest first; passes per chain round
   DevBuf wins, order_d, qtab, wctr, ctrs;
+  // HIP graph of one generation: kind 1 = batched (reset + k_batched), kind 2
+  // = plain (C + 1 launches); node arguments are rewritten for each run
+  int graph_kind = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> nodes;
 };
+
+static void plan_graph_reset(glp_plan *p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->nodes.clear();
+  p->graph_kind = 0;
+}
 
 // Passes per chain round when none is given (each pass is a lane pair): one
 // MH round decides ~6 proposals at the ~15% acceptance of sampling
@@ -304,6 +319,7 @@ int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
       return set_error(GLP_ERR_ARG, "lanes per chain must be a power of two <= 32");
   }
   p->q.assign(lanes_per_window, lanes_per_window + p->nwin);
+  plan_graph_reset(p);
   p->grid = plan_grid(p, p->order);
   return plan_upload_q(p);
 }
@@ -311,6 +327,7 @@ int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
 int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream,
                       unsigned *lanes_out) {
   if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  plan_graph_reset(p);
   hipStream_t s = static_cast<hipStream_t>(stream);
   KArgs a = p->a;
   a.gen = gen;
@@ -416,6 +433,142 @@ int glp_plan_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void
   return GLP_OK;
 }
 
+}  // extern "C"
+
+// Node parameters of the batched generation's kernel; `a` must outlive the
+// call that consumes the parameters.
+namespace {
+struct BatchedArgs {
+  const glp_window *wins;
+  const unsigned *order, *qtab;
+  unsigned nwin;
+  unsigned *wctr;
+  unsigned long long *ctrs;
+  KArgs a;
+  void *ptrs[7];
+  void bind() {
+    ptrs[0] = &wins;
+    ptrs[1] = &order;
+    ptrs[2] = &qtab;
+    ptrs[3] = &nwin;
+    ptrs[4] = &wctr;
+    ptrs[5] = &ctrs;
+    ptrs[6] = &a;
+  }
+};
+
+struct PlainArgs {
+  const glp_window *wins;
+  KArgs a;
+  unsigned step;
+  void *ptrs[3];
+  void bind() {
+    ptrs[0] = &wins;
+    ptrs[1] = &a;
+    ptrs[2] = &step;
+  }
+};
+
+hipKernelNodeParams node_params(const void *fn, dim3 grid, unsigned shmem, void **args) {
+  hipKernelNodeParams k{};
+  k.func = const_cast<void *>(fn);
+  k.gridDim = grid;
+  k.blockDim = dim3(256);
+  k.sharedMemBytes = shmem;
+  k.kernelParams = args;
+  k.extra = nullptr;
+  return k;
+}
+}  // namespace
+
+static int plan_graph_build(glp_plan *p, int kind) {
+  plan_graph_reset(p);
+  const KArgs &a = p->a;
+  const bool mixed = a.precision != GLP_PREC_F32;
+  const unsigned shmem = (unsigned)lds_bytes(a);
+  GLP_TRY(hip_check(hipGraphCreate(&p->graph, 0), "hipGraphCreate"));
+  int rc = GLP_OK;
+  if (kind == 1) {
+    const void *fn = batched_kernel_fn(a.bits, mixed);
+    rc = prepare_kernel_lds(fn, shmem);
+    unsigned long long *ctrs = p->ctrs.as<unsigned long long>();
+    unsigned *wctr = p->wctr.as<unsigned>();
+    unsigned nwin = p->nwin;
+    void *rargs[] = {&ctrs, &wctr, &nwin};
+    hipKernelNodeParams rk = node_params(reset_kernel_fn(), dim3(1), 0, rargs);
+    hipGraphNode_t n0 = nullptr, n1 = nullptr;
+    if (rc == GLP_OK) rc = hip_check(hipGraphAddKernelNode(&n0, p->graph, nullptr, 0, &rk), "graph node");
+    BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
+                   p->nwin, wctr, ctrs, a, {}};
+    ba.bind();
+    hipKernelNodeParams bk = node_params(fn, dim3(p->grid), shmem, ba.ptrs);
+    if (rc == GLP_OK) rc = hip_check(hipGraphAddKernelNode(&n1, p->graph, &n0, 1, &bk), "graph node");
+    if (rc == GLP_OK) p->nodes = {n0, n1};
+  } else {
+    const void *fn = plain_kernel_fn(a.bits, mixed);
+    rc = prepare_kernel_lds(fn, shmem);
+    hipGraphNode_t prev = nullptr;
+    for (unsigned step = 0; step <= a.C && rc == GLP_OK; ++step) {
+      PlainArgs pa{p->wins.as<glp_window>(), a, step, {}};
+      pa.bind();
+      hipKernelNodeParams k = node_params(fn, dim3((2 * a.N + 255) / 256, p->nwin), shmem, pa.ptrs);
+      hipGraphNode_t n = nullptr;
+      rc = hip_check(hipGraphAddKernelNode(&n, p->graph, prev ? &prev : nullptr, prev ? 1 : 0, &k),
+                     "graph node");
+      p->nodes.push_back(n);
+      prev = n;
+    }
+  }
+  if (rc == GLP_OK) rc = hip_check(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0),
+                                   "hipGraphInstantiate");
+  if (rc != GLP_OK) {
+    plan_graph_reset(p);
+    return rc;
+  }
+  p->graph_kind = kind;
+  return GLP_OK;
+}
+
+// Rewrites the graph's kernel arguments for (gen, pen, seed) and launches it.
+static int plan_graph_run(glp_plan *p, int kind, unsigned gen, float pen, uint64_t seed,
+                          hipStream_t s) {
+  if (p->graph_kind != kind) GLP_TRY(plan_graph_build(p, kind));
+  KArgs a = p->a;
+  a.gen = gen;
+  a.pen = pen;
+  a.seed = seed;
+  const bool mixed = a.precision != GLP_PREC_F32;
+  const unsigned shmem = (unsigned)lds_bytes(a);
+  if (kind == 1) {
+    BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
+                   p->nwin, p->wctr.as<unsigned>(), p->ctrs.as<unsigned long long>(), a, {}};
+    ba.bind();
+    hipKernelNodeParams bk = node_params(batched_kernel_fn(a.bits, mixed), dim3(p->grid), shmem, ba.ptrs);
+    GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[1], &bk), "graph update"));
+  } else {
+    const void *fn = plain_kernel_fn(a.bits, mixed);
+    for (unsigned step = 0; step <= a.C; ++step) {
+      PlainArgs pa{p->wins.as<glp_window>(), a, step, {}};
+      pa.bind();
+      hipKernelNodeParams k = node_params(fn, dim3((2 * a.N + 255) / 256, p->nwin), shmem, pa.ptrs);
+      GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[step], &k), "graph update"));
+    }
+  }
+  return hip_check(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
+}
+
+extern "C" {
+
+int glp_plan_graph_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  return plan_graph_run(p, 1, gen, pen, seed, static_cast<hipStream_t>(stream));
+}
+
+int glp_plan_graph_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  return plan_graph_run(p, 2, gen, pen, seed, static_cast<hipStream_t>(stream));
+}
+
 int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {
   if (!p) return set_error(GLP_ERR_ARG, "null plan");
   if (lanes) *lanes = p->q.empty() ? 0 : p->q[p->order[0]];
@@ -425,6 +578,7 @@ int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {
 
 int glp_plan_destroy(glp_plan *p) {
   if (!p) return GLP_OK;
+  plan_graph_reset(p);
   for (DevBuf *b : {&p->wins, &p->order_d, &p->qtab, &p->wctr, &p->ctrs}) b->release();
   delete p;
   return GLP_OK;

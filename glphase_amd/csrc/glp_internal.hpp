@@ -41,6 +41,12 @@ size_t lds_bytes(const KArgs &a);
 int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
                    const unsigned *d_qtab, unsigned nwin, unsigned *d_wctr,
                    unsigned long long *d_ctrs, unsigned grid, hipStream_t stream);
+// Kernel entry points for HIP graph nodes, and the dynamic-LDS attribute they
+// need before any launch.
+const void *plain_kernel_fn(unsigned bits, bool mixed);
+const void *batched_kernel_fn(unsigned bits, bool mixed);
+const void *reset_kernel_fn();
+int prepare_kernel_lds(const void *fn, size_t bytes);
 int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu);
 int launch_eval(const KArgs &a, const glp_window *d_wins,
                 const unsigned *d_samples, const unsigned *d_parents,

@@ -692,6 +692,17 @@ int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
   return hip_check(hipGetLastError(), "k_plain launch");
 }
 
+const void *plain_kernel_fn(unsigned bits, bool mixed) {
+  if (bits == 8) return mixed ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>;
+  return mixed ? (const void *)&k_plain<4, true> : (const void *)&k_plain<4, false>;
+}
+
+const void *reset_kernel_fn() { return (const void *)&k_reset_ctrs; }
+
+int prepare_kernel_lds(const void *fn, size_t bytes) { return allow_lds(fn, bytes); }
+
+const void *batched_kernel_fn(unsigned bits, bool mixed);
+
 static const void *batched_kernel(unsigned bits, bool mixed) {
   if (bits == 8) return mixed ? (const void *)&k_batched<8, true> : (const void *)&k_batched<8, false>;
   return mixed ? (const void *)&k_batched<4, true> : (const void *)&k_batched<4, false>;
@@ -707,6 +718,8 @@ int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_o
   hipError_t err = hipLaunchKernel((void *)fn, dim3(grid), dim3(256), args, lds_bytes(a), stream);
   return hip_check(err, "k_batched launch");
 }
+
+const void *batched_kernel_fn(unsigned bits, bool mixed) { return batched_kernel(bits, mixed); }
 
 int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu) {
   const void *fn = batched_kernel(bits, mixed);

@@ -39,16 +39,28 @@ __device__ __forceinline__ float work_emit(const float4 row, const float3 t) {
   return row.x * t.x + row.y * t.y + row.z * t.y + row.w * t.z;
 }
 
-__device__ __forceinline__ float work_powf(float x, float S) {
-  return S == 1.0f ? x : (float)pow((double)x, (double)S);
-}
-
 __device__ __forceinline__ unsigned hbit64(uint64_t w, unsigned m) {
   return (unsigned)(w >> (m & 63u)) & 1u;
 }
 
+constexpr int kWorkBlock = 8;  // sites per prefetch block
+
+// Genotype choice of impute.cpp:472-484 from the weights and the site's
+// uniform word.
+__device__ __forceinline__ unsigned work_choose(float c00, float c01, float c10, float c11,
+                                                uint32_t xm) {
+  const float u = (float)(u01(xm) * (double)(c00 + c01 + c10 + c11));
+  if (u < c00) return 0u;
+  if (u < c00 + c01) return 1u;
+  if (u < c00 + c01 + c10) return 2u;
+  return 3u;
+}
+
 // One lane per (window, sample).  beta: this window's workspace slice,
-// [M][N] float4 (site-major, so a wavefront's stores coalesce).
+// [M][N] float4 (site-major, so a wavefront's stores coalesce).  GL pairs
+// and beta are loaded a block of sites ahead of their use.  POW: S != 1
+// (powf is the identity at S = 1).
+template <bool POW>
 __global__ void __launch_bounds__(256)
     k_hmm_work(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
                float4 *__restrict__ ws, WorkArgs a) {
@@ -73,108 +85,160 @@ __global__ void __launch_bounds__(256)
     h = h < a.H ? h : a.H - 1;
     rows[q] = W.panel + (size_t)h * wn;
   }
+  constexpr int B = kWorkBlock;
 
-  // backward recursion (impute.cpp:398-416)
+  // backward recursion (impute.cpp:398-416), sites M-1 .. 1 in blocks
   float b0 = 1, b1 = 1, b2 = 1, b3 = 1;
   beta[(size_t)(M - 1) * N + i] = make_float4(1, 1, 1, 1);
-  unsigned wcur = 0xFFFFFFFFu;
-  uint64_t hw[4] = {0, 0, 0, 0};
-  for (unsigned m = M - 1; m > 0; --m) {
-    if ((m >> 6) != wcur) {
-      wcur = m >> 6;
+  // allele words of the current 64-site word and (prefetched) the next one
+  // in the walk's direction
+  unsigned wcur = (M - 1) >> 6;
+  uint64_t hw[4], hwn[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) hw[q] = rows[q][wcur];
+  for (int q = 0; q < 4; ++q) {
+    hw[q] = rows[q][wcur];
+    hwn[q] = wcur > 0 ? rows[q][wcur - 1] : 0ull;
+  }
+  float2 qn[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int m = (int)M - 1 - k;
+    qn[k] = m >= 1 ? q2[(size_t)m * N + i] : make_float2(0, 0);
+  }
+  for (int top = (int)M - 1; top >= 1; top -= B) {
+    float2 qc[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      qc[k] = qn[k];
+      const int m = top - B - k;
+      qn[k] = m >= 1 ? q2[(size_t)m * N + i] : make_float2(0, 0);
     }
-    const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
-                   m1 = hbit64(hw[3], m);
-    const float3 t = work_prob(q2[(size_t)m * N + i]);
-    const float e00 = work_emit(s_pc[(f0 << 1) | m0], t), e01 = work_emit(s_pc[(f0 << 1) | m1], t);
-    const float e10 = work_emit(s_pc[(f1 << 1) | m0], t), e11 = work_emit(s_pc[(f1 << 1) | m1], t);
-    const float bb00 = b0 * e00, bb01 = b1 * e01, bb10 = b2 * e10, bb11 = b3 * e11;
-    const float *tr = W.tran + 3 * (size_t)m;
-    const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
-    const float n0 = bb00 * t0 + (bb01 + bb10) * t1 + bb11 * t2;
-    const float n1 = bb01 * t0 + (bb00 + bb11) * t1 + bb10 * t2;
-    const float n2 = bb10 * t0 + (bb00 + bb11) * t1 + bb01 * t2;
-    const float n3 = bb11 * t0 + (bb01 + bb10) * t1 + bb00 * t2;
-    const float sum = 1.0f / (n0 + n1 + n2 + n3);
-    b0 = n0 * sum;
-    b1 = n1 * sum;
-    b2 = n2 * sum;
-    b3 = n3 * sum;
-    beta[(size_t)(m - 1) * N + i] = make_float4(b0, b1, b2, b3);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int mi = top - k;
+      if (mi < 1) break;
+      const unsigned m = (unsigned)mi;
+      if ((m >> 6) != wcur) {
+        wcur = m >> 6;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          hw[q] = hwn[q];
+          hwn[q] = wcur > 0 ? rows[q][wcur - 1] : 0ull;
+        }
+      }
+      const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
+                     m1 = hbit64(hw[3], m);
+      const float3 t = work_prob(qc[k]);
+      const float e00 = work_emit(s_pc[(f0 << 1) | m0], t), e01 = work_emit(s_pc[(f0 << 1) | m1], t);
+      const float e10 = work_emit(s_pc[(f1 << 1) | m0], t), e11 = work_emit(s_pc[(f1 << 1) | m1], t);
+      const float bb00 = b0 * e00, bb01 = b1 * e01, bb10 = b2 * e10, bb11 = b3 * e11;
+      const float *tr = W.tran + 3 * (size_t)m;
+      const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
+      const float n0 = bb00 * t0 + (bb01 + bb10) * t1 + bb11 * t2;
+      const float n1 = bb01 * t0 + (bb00 + bb11) * t1 + bb10 * t2;
+      const float n2 = bb10 * t0 + (bb00 + bb11) * t1 + bb01 * t2;
+      const float n3 = bb11 * t0 + (bb01 + bb10) * t1 + bb00 * t2;
+      const float sum = 1.0f / (n0 + n1 + n2 + n3);
+      b0 = n0 * sum;
+      b1 = n1 * sum;
+      b2 = n2 * sum;
+      b3 = n3 * sum;
+      beta[(size_t)(m - 1) * N + i] = make_float4(b0, b1, b2, b3);
+    }
   }
 
-  // forward pass with sampling (impute.cpp:418-484)
+  // forward pass with sampling (impute.cpp:418-484), sites 0 .. M-1 in blocks
   uint64_t *ha = W.hnew + (size_t)(2 * i) * wn, *hb = ha + wn;
   float l0 = 0, l1 = 0, l2 = 0, l3 = 0;
   uint64_t wa = 0, wb = 0;
   U4 x{0, 0, 0, 0};
   const float S = a.pen;
-  for (unsigned m = 0; m < M; ++m) {
-    if ((m & 63u) == 0) {
+  float4 bn[B];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) hw[q] = rows[q][m >> 6];
+  for (int q = 0; q < 4; ++q) {
+    hw[q] = rows[q][0];
+    hwn[q] = wn > 1 ? rows[q][1] : 0ull;
+  }
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const unsigned m = (unsigned)k;
+    qn[k] = m < M ? q2[(size_t)m * N + i] : make_float2(0, 0);
+    bn[k] = m < M ? beta[(size_t)m * N + i] : make_float4(0, 0, 0, 0);
+  }
+  for (unsigned base = 0; base < M; base += B) {
+    float2 qc[B];
+    float4 bc[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      qc[k] = qn[k];
+      bc[k] = bn[k];
+      const unsigned m = base + B + (unsigned)k;
+      qn[k] = m < M ? q2[(size_t)m * N + i] : make_float2(0, 0);
+      bn[k] = m < M ? beta[(size_t)m * N + i] : make_float4(0, 0, 0, 0);
     }
-    const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
-                   m1 = hbit64(hw[3], m);
-    const unsigned s00 = (f0 << 1) | m0, s01 = (f0 << 1) | m1, s10 = (f1 << 1) | m0,
-                   s11 = (f1 << 1) | m1;
-    const float4 r00 = s_pc[s00], r01 = s_pc[s01], r10 = s_pc[s10], r11 = s_pc[s11];
-    const float3 t = work_prob(q2[(size_t)m * N + i]);
-    const float e00 = work_emit(r00, t), e01 = work_emit(r01, t), e10 = work_emit(r10, t),
-                e11 = work_emit(r11, t);
-    if (m) {
-      const float *tr = W.tran + 3 * (size_t)m;
-      const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
-      const float n0 = l0 * t0 + l1 * t1 + l2 * t1 + l3 * t2;
-      const float n1 = l0 * t1 + l1 * t0 + l2 * t2 + l3 * t1;
-      const float n2 = l0 * t1 + l1 * t2 + l2 * t0 + l3 * t1;
-      const float n3 = l0 * t2 + l1 * t1 + l2 * t1 + l3 * t0;
-      l0 = n0 * e00;
-      l1 = n1 * e01;
-      l2 = n2 * e10;
-      l3 = n3 * e11;
-    } else {
-      l0 = 0.25f * e00;
-      l1 = 0.25f * e01;
-      l2 = 0.25f * e10;
-      l3 = 0.25f * e11;
-    }
-    const float sum = 1.0f / (l0 + l1 + l2 + l3);
-    l0 *= sum;
-    l1 *= sum;
-    l2 *= sum;
-    l3 *= sum;
-    const float4 b = beta[(size_t)m * N + i];
-    const float p00 = l0 * b.x, p01 = l1 * b.y, p10 = l2 * b.z, p11 = l3 * b.w;
-    const float c00 = work_powf(t.x * (p00 * r00.x + p01 * r01.x + p10 * r10.x + p11 * r11.x), S);
-    const float c01 = work_powf(t.y * (p00 * r00.y + p01 * r01.y + p10 * r10.y + p11 * r11.y), S);
-    const float c10 = work_powf(t.y * (p00 * r00.z + p01 * r01.z + p10 * r10.z + p11 * r11.z), S);
-    const float c11 = work_powf(t.z * (p00 * r00.w + p01 * r01.w + p10 * r10.w + p11 * r11.w), S);
-    if ((m & 3u) == 0) x = draw(a.seed, kWorkTag | (m >> 2), i, w, a.gen);
-    const uint32_t xm = (m & 3u) == 0 ? x.x : (m & 3u) == 1 ? x.y : (m & 3u) == 2 ? x.z : x.w;
-    const float u = (float)(u01(xm) * (double)(c00 + c01 + c10 + c11));
-    unsigned ga, gb;
-    if (u < c00) {
-      ga = 0;
-      gb = 0;
-    } else if (u < c00 + c01) {
-      ga = 0;
-      gb = 1;
-    } else if (u < c00 + c01 + c10) {
-      ga = 1;
-      gb = 0;
-    } else {
-      ga = 1;
-      gb = 1;
-    }
-    wa |= (uint64_t)ga << (m & 63u);
-    wb |= (uint64_t)gb << (m & 63u);
-    if ((m & 63u) == 63u || m + 1 == M) {
-      ha[m >> 6] = wa;
-      hb[m >> 6] = wb;
-      wa = wb = 0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const unsigned m = base + (unsigned)k;
+      if (m >= M) break;
+      if ((m & 63u) == 0 && m) {
+        const unsigned wx = (m >> 6) + 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          hw[q] = hwn[q];
+          hwn[q] = wx < wn ? rows[q][wx] : 0ull;
+        }
+      }
+      const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
+                     m1 = hbit64(hw[3], m);
+      const float4 r00 = s_pc[(f0 << 1) | m0], r01 = s_pc[(f0 << 1) | m1],
+                   r10 = s_pc[(f1 << 1) | m0], r11 = s_pc[(f1 << 1) | m1];
+      const float3 t = work_prob(qc[k]);
+      const float e00 = work_emit(r00, t), e01 = work_emit(r01, t), e10 = work_emit(r10, t),
+                  e11 = work_emit(r11, t);
+      if (m) {
+        const float *tr = W.tran + 3 * (size_t)m;
+        const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
+        const float n0 = l0 * t0 + l1 * t1 + l2 * t1 + l3 * t2;
+        const float n1 = l0 * t1 + l1 * t0 + l2 * t2 + l3 * t1;
+        const float n2 = l0 * t1 + l1 * t2 + l2 * t0 + l3 * t1;
+        const float n3 = l0 * t2 + l1 * t1 + l2 * t1 + l3 * t0;
+        l0 = n0 * e00;
+        l1 = n1 * e01;
+        l2 = n2 * e10;
+        l3 = n3 * e11;
+      } else {
+        l0 = 0.25f * e00;
+        l1 = 0.25f * e01;
+        l2 = 0.25f * e10;
+        l3 = 0.25f * e11;
+      }
+      const float sum = 1.0f / (l0 + l1 + l2 + l3);
+      l0 *= sum;
+      l1 *= sum;
+      l2 *= sum;
+      l3 *= sum;
+      const float4 b = bc[k];
+      const float p00 = l0 * b.x, p01 = l1 * b.y, p10 = l2 * b.z, p11 = l3 * b.w;
+      float c00 = t.x * (p00 * r00.x + p01 * r01.x + p10 * r10.x + p11 * r11.x);
+      float c01 = t.y * (p00 * r00.y + p01 * r01.y + p10 * r10.y + p11 * r11.y);
+      float c10 = t.y * (p00 * r00.z + p01 * r01.z + p10 * r10.z + p11 * r11.z);
+      float c11 = t.z * (p00 * r00.w + p01 * r01.w + p10 * r10.w + p11 * r11.w);
+      if (POW) {
+        c00 = (float)pow((double)c00, (double)S);
+        c01 = (float)pow((double)c01, (double)S);
+        c10 = (float)pow((double)c10, (double)S);
+        c11 = (float)pow((double)c11, (double)S);
+      }
+      if ((m & 3u) == 0) x = draw(a.seed, kWorkTag | (m >> 2), i, w, a.gen);
+      const uint32_t xm = (m & 3u) == 0 ? x.x : (m & 3u) == 1 ? x.y : (m & 3u) == 2 ? x.z : x.w;
+      const unsigned g = work_choose(c00, c01, c10, c11, xm);
+      wa |= (uint64_t)(g >> 1) << (m & 63u);
+      wb |= (uint64_t)(g & 1u) << (m & 63u);
+      if ((m & 63u) == 63u || m + 1 == M) {
+        ha[m >> 6] = wa;
+        hb[m >> 6] = wb;
+        wa = wb = 0;
+      }
     }
   }
 }
@@ -242,10 +306,13 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
     a.gen = d->gen;
     for (int k = 0; k < 16; ++k) a.pc[k] = d->pc16[k];
     const dim3 grid((d->num_samples + 255) / 256, d->num_windows), block(256);
-    hipLaunchKernelGGL(k_hmm_work, grid, block, 0, s,
-                       reinterpret_cast<const glp_work_window *>(dev),
-                       reinterpret_cast<const size_t *>(dev + tbl),
-                       static_cast<float4 *>(d->workspace), a);
+    const auto *wt = reinterpret_cast<const glp_work_window *>(dev);
+    const auto *wo = reinterpret_cast<const size_t *>(dev + tbl);
+    float4 *ws = static_cast<float4 *>(d->workspace);
+    if (d->pen == 1.0f)
+      hipLaunchKernelGGL((k_hmm_work<false>), grid, block, 0, s, wt, wo, ws, a);
+    else
+      hipLaunchKernelGGL((k_hmm_work<true>), grid, block, 0, s, wt, wo, ws, a);
     rc = hip_check(hipGetLastError(), "k_hmm_work launch");
   }
   (void)hipFreeAsync(dev, s);

@@ -149,6 +149,20 @@ class Job:
         check(lib().glp_plan_run_plain(self.plan, gen, pen, seed, self.stream_handle()),
               "glp_plan_run_plain")
 
+    def run_batched_graph(self, gen=0, pen=1.0, seed=0):
+        """run_batched through the plan's HIP graph."""
+        if self.plan is None:
+            self.make_plan()
+        check(lib().glp_plan_graph_run(self.plan, gen, pen, seed, self.stream_handle()),
+              "glp_plan_graph_run")
+
+    def run_plain_graph(self, gen=0, pen=1.0, seed=0):
+        """run_plain through the plan's HIP graph."""
+        if self.plan is None:
+            self.make_plan()
+        check(lib().glp_plan_graph_run_plain(self.plan, gen, pen, seed, self.stream_handle()),
+              "glp_plan_graph_run_plain")
+
     def run_plain_desc(self, gen=0, pen=1.0, seed=0):
         """The plain path through the stateless glp_plain_init / glp_plain_step."""
         self.desc.gen, self.desc.pen, self.desc.seed = gen, pen, seed

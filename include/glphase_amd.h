@@ -199,6 +199,16 @@ int glp_plan_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                        void *stream);
 int glp_plan_destroy(glp_plan *plan);
 
+/* The same generations through a HIP graph: the plan builds the graph of one
+ * generation on first use (batched: counter reset + persistent kernel;
+ * plain: the C + 1 launches of glp_plan_run_plain), and each call rewrites
+ * the nodes' (gen, pen, seed) and launches it on `stream`.  Results equal the
+ * direct calls'.  Changing the plan's depths rebuilds the graph. */
+int glp_plan_graph_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
+                       void *stream);
+int glp_plan_graph_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
+                             void *stream);
+
 /* Self-test of the device reciprocal the rescale step uses (no counterpart
  * in the reference): compares it with the IEEE division 1.0f / x for every
  * float x with bit pattern in [lo_bits, hi_bits) on the current device.
