@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libglphase_amd.so")
 
 GLP_OK = 0
 PROP_UNIFORM, PROP_KLIST, PROP_HOST = 0, 1, 2
-PREC_F32, PREC_MIXED, PREC_F16LUT = 0, 1, 2
+PREC_F32, PREC_MIXED = 0, 1
 
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_u64p = ctypes.POINTER(ctypes.c_uint64)

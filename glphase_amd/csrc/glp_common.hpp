@@ -82,7 +82,6 @@ enum PropMode : int {
 enum Precision : int {
   kPrecF32 = 0,    // FP32 recurrence and FP32 log accumulation (reference)
   kPrecMixed = 1,  // FP32 recurrence, FP64 log accumulation and MH ratio
-  kPrecF16Lut = 2, // as kPrecMixed with an FP16 emission table
 };
 
 }  // namespace glp

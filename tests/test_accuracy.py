@@ -1,0 +1,95 @@
+"""Accuracy of the forward-pass arithmetic against the FP64 forward oracle
+(per-proposal |dloglik| / |loglik_FP64|) for the reference FP32 arithmetic
+(which the GPU reproduces bit for bit), the mixed form (FP32 recurrence,
+FP64 log-scale accumulation) and an FP16 emission table; and genotype
+discordance of whole GPU generations against the simulated truth.
+
+The CPU part prints its statistics (pytest -s shows them); the bounds it
+asserts are the ones the forms are held to: 1e-5 relative for FP32 and
+mixed."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import helpers
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_errors(orc, p, n_eval, seed, lut_variant=None):
+    """Relative errors of F32, mixed and (optionally) a modified emission
+    table, for n_eval random (sample, 4 parents) against FP64."""
+    rng = np.random.default_rng(seed)
+    lut = orc.emit_lut(p["codebook"], p["pc"])
+    out = {"f32": [], "mixed": [], "variant": []}
+    for _ in range(n_eval):
+        i = int(rng.integers(p["N"]))
+        P = rng.integers(0, p["H"], 4).astype(np.uint32)
+        emit = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut)
+        f64 = orc.hmm_like_f64(p["panel"], P, emit, p["tran"])
+        out["f32"].append(abs(orc.hmm_like_f32(p["panel"], P, emit, p["tran"]) - f64) / abs(f64))
+        out["mixed"].append(abs(orc.hmm_like_mixed(p["panel"], P, emit, p["tran"]) - f64) / abs(f64))
+        if lut_variant is not None:
+            ev = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut_variant)
+            v = orc.hmm_like_mixed(p["panel"], P, ev, p["tran"])
+            out["variant"].append(abs(v - f64) / abs(f64))
+    return {k: np.array(v) for k, v in out.items() if v}
+
+
+def test_forward_error_against_fp64(orc):
+    p = helpers.synthetic(400, 4096, seed=12, depth=2.0, training_size=1 << 16)
+    lut = orc.emit_lut(p["codebook"], p["pc"])
+    lut16 = lut.astype(np.float16).astype(np.float32)
+    e = rel_errors(orc, p, 300, 1, lut_variant=lut16)
+    for k, v in e.items():
+        say(f"{k:8s}: max {v.max():.3e}  p99 {np.quantile(v, 0.99):.3e}  median {np.median(v):.3e}")
+    assert e["f32"].max() < 1e-5
+    assert e["mixed"].max() < 1e-5
+    assert e["mixed"].max() <= e["f32"].max()
+    # the FP16 table is held to the same bound; record whether it meets it
+    say(f"fp16 emission table within 1e-5 of FP64: {bool(e['variant'].max() < 1e-5)}")
+
+
+@pytest.mark.gpu
+def test_generations_genotype_discordance():
+    # 30 generations (10 annealed burn-in, pen = min((g+1)/10, 1)^2 as
+    # insti.cpp:1670-1671, then sampling) of batched MH plus resampling on the
+    # GPU at 2,000 samples x 2,048 sites, depth 2.  Genotypes averaged over
+    # the 20 sampling generations must beat calling each site's GL argmax; a
+    # single generation's draw is a posterior sample and is reported only.
+    from glphase_amd import PREC_MIXED
+    from glphase_amd.engine import Job, Resampler
+    N, M, C = 2000, 2048, 64
+    p = helpers.synthetic(N, M, seed=44, depth=2.0, training_size=1 << 16)
+    win = dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])
+    job = Job([win], N, p["H"], p["codebook"], p["pc"], bits=8, num_cycles=C,
+              precision=PREC_MIXED)
+    job.make_plan(4)
+    rs = Resampler(job, [np.ascontiguousarray(p["gls"][:, :, 1:3])], p["pc"])
+    truth = p["truth"].astype(np.int8)
+    g_true = truth[0::2] + truth[1::2]
+    disc_gl = float((np.argmax(p["gls"], axis=2) != g_true).mean())
+    bn, total = 10, 30
+    dosage = np.zeros((N, M))
+    like = []
+    for g in range(total):
+        pen = min((g + 1.0) / bn, 1.0) ** 2
+        job.run_batched(gen=g, pen=pen, seed=2024)
+        rs.run(pen=pen, seed=2025, gen=g)
+        rs.adopt()
+        like.append(float(job.like()[0].mean()))
+        if g >= bn:
+            h = rs.hnew()[0]
+            bits = np.unpackbits(h.view(np.uint8).reshape(2 * N, -1), axis=1,
+                                 bitorder="little")[:, :M]
+            dosage += bits[0::2].astype(np.int8) + bits[1::2].astype(np.int8)
+            last = float(((bits[0::2] + bits[1::2]).astype(np.int8) != g_true).mean())
+    disc = float((np.rint(dosage / (total - bn)).astype(np.int8) != g_true).mean())
+    say(f"genotype discordance: averaged over sampling generations {disc:.4f}, last draw "
+        f"{last:.4f}, GL argmax {disc_gl:.4f}; mean loglik {like[0]:.1f} -> {like[-1]:.1f}")
+    assert disc < disc_gl
+    assert like[-1] > like[bn]  # the sampling chain keeps climbing
