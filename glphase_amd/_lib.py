@@ -60,6 +60,21 @@ class GlpWorkDesc(ctypes.Structure):
     ]
 
 
+class GlpKnnDesc(ctypes.Structure):
+    _fields_ = [
+        ("panel", ctypes.c_void_p),
+        ("num_haps", ctypes.c_uint),
+        ("num_sites", ctypes.c_uint),
+        ("num_samples", ctypes.c_uint),
+        ("K", ctypes.c_uint),
+        ("freq_lb", ctypes.c_double),
+        ("freq_ub", ctypes.c_double),
+        ("using_maf", ctypes.c_int),
+        ("klist", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p),
+    ]
+
+
 class GlpMhDesc(ctypes.Structure):
     _fields_ = [
         ("num_windows", ctypes.c_uint),
@@ -117,6 +132,9 @@ SIGNATURES = [
     ("glp_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("glp_knn_k_out", ctypes.c_uint, [ctypes.c_uint, ctypes.c_uint]),
+    ("glp_knn_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(GlpKnnDesc)]),
+    ("glp_knn_build", ctypes.c_int, [ctypes.POINTER(GlpKnnDesc), ctypes.c_void_p]),
     ("glp_plan_graph_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float,
                                           ctypes.c_uint64, ctypes.c_void_p]),
     ("glp_plan_graph_run_plain", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float,

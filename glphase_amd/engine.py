@@ -8,8 +8,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (GlpMhDesc, GlpWindow, GlpWorkDesc, GlpWorkWindow, c_f32p, c_u32p, c_u64p,
-                   check, lib, ptr)
+from ._lib import (GlpKnnDesc, GlpMhDesc, GlpWindow, GlpWorkDesc, GlpWorkWindow, c_f32p,
+                   c_u32p, c_u64p, check, lib, ptr)
 
 
 def _torch():
@@ -262,3 +262,26 @@ class Resampler:
         (reference haplotypes) are kept."""
         for w in range(self.job.W):
             self.job.keep[3 * w][: 2 * self.job.N].copy_(self.hnew_d[w])
+
+
+def knn_lists(panel, num_sites, num_samples, K, freq_lb=0.0, freq_ub=1.0, using_maf=False,
+              device="cuda:0"):
+    """kNN candidate lists of samples 0 .. num_samples-1 over `panel` (a host
+    [H][wn] uint64 array or a device tensor), built on the GPU; returns the
+    device tensor [num_samples][glp_knn_k_out(K, H)] (int32 view of uint32)."""
+    torch = _torch()
+    dev = torch.device(device)
+    pd = panel if hasattr(panel, "data_ptr") else _dev(torch, panel, dev)
+    H = int(pd.shape[0])
+    kout = lib().glp_knn_k_out(K, H)
+    out = torch.zeros((num_samples, kout), dtype=torch.int32, device=dev)
+    d = GlpKnnDesc(pd.data_ptr(), H, num_sites, num_samples, K, freq_lb, freq_ub,
+                   int(using_maf), out.data_ptr(), None)
+    nbytes = lib().glp_knn_workspace_bytes(ctypes.byref(d))
+    ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=dev)
+    d.workspace = ws.data_ptr()
+    s = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    check(lib().glp_knn_build(ctypes.byref(d), s), "glp_knn_build")
+    torch.cuda.synchronize(dev)
+    del ws
+    return out

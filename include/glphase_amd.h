@@ -260,6 +260,36 @@ size_t glp_work_workspace_bytes(const glp_work_desc *desc);
  * full (bits past num_sites are 0).  Stream-ordered. */
 int glp_hmm_work(const glp_work_desc *desc, void *stream);
 
+/* ----- kNN candidate lists (KNN, kNN.cpp:8-225) --------------------------- */
+
+/* Neighbour lists of samples 0 .. num_samples-1 (haplotypes 2u, 2u+1 of the
+ * panel) over the whole panel, by Hamming distance on the sites whose
+ * variant allele frequency over the panel (alt count / num_haps, a double;
+ * the minor allele frequency with using_maf) lies in [freq_lb, freq_ub]
+ * (KNN::CalculateVarAfs / CutDownHaps / ClusterHaps).  k' = ceil(K / 2), K
+ * capped at num_haps - 2: row u = (A_0, B_0, A_1, B_1, ...), A_j the j-th
+ * nearest haplotype of hap 2u, B_j the j-th nearest of hap 2u+1 not already
+ * in the row; the sample's own haplotypes never appear.  Equal distances
+ * rank by haplotype index (std::sort leaves them unspecified; below 17
+ * candidates libstdc++ sorts stably, which is the same order). */
+typedef struct {
+  const uint64_t *panel;  /* device [num_haps][ceil(num_sites / 64)] */
+  unsigned num_haps;      /* even */
+  unsigned num_sites;
+  unsigned num_samples;   /* <= num_haps / 2 */
+  unsigned K;             /* 1 .. 2048 */
+  double freq_lb, freq_ub;
+  int using_maf;
+  unsigned *klist;        /* out, device [num_samples][glp_knn_k_out(K, num_haps)] */
+  void *workspace;        /* device, >= glp_knn_workspace_bytes */
+} glp_knn_desc;
+
+/* Row length of the lists: 2 * ceil(min(K, num_haps - 2) / 2). */
+unsigned glp_knn_k_out(unsigned K, unsigned num_haps);
+size_t glp_knn_workspace_bytes(const glp_knn_desc *desc);
+/* Builds the lists (stream-ordered). */
+int glp_knn_build(const glp_knn_desc *desc, void *stream);
+
 /* ===== 3. host-side helpers of the drop-in ================================ */
 
 /* Insti::initialize parameter builders (insti.cpp:1254-1315): mu from the

@@ -70,6 +70,12 @@ def L():
                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                             u64p, u64p]
         lib.orc_prob_emit.argtypes = [f32p, ctypes.c_uint, f32p, f32p, f32p]
+        lib.orc_knn_k_out.restype = ctypes.c_uint
+        lib.orc_knn_k_out.argtypes = [ctypes.c_uint, ctypes.c_uint]
+        lib.orc_knn_mask.argtypes = [u64p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, u64p, f64p]
+        lib.orc_knn.argtypes = [u64p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int, u32p]
         lib.orc_hmm_work.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
                                      ctypes.c_float, ctypes.c_uint, ctypes.c_uint64,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p]
@@ -305,3 +311,23 @@ def prob_emit(het_aa, pc_):
     emit = np.zeros((M, 4), np.float32)
     L().orc_prob_emit(P(q, f32p), M, P(c16, f32p), P(prob, f32p), P(emit, f32p))
     return prob, emit
+
+
+def knn_mask(haps, M, lb=0.0, ub=1.0, using_maf=False):
+    """(site mask words, per-site variant allele frequencies)."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    mask = np.zeros(h.shape[1], np.uint64)
+    af = np.zeros(M, np.float64)
+    L().orc_knn_mask(P(h, u64p), h.shape[1], h.shape[0], M, lb, ub, int(using_maf),
+                     P(mask, u64p), P(af, f64p))
+    return mask, af
+
+
+def knn(haps, M, nsamp, K, lb=0.0, ub=1.0, using_maf=False):
+    """Candidate lists [nsamp][2 ceil(K/2)] (K capped at nhaps - 2)."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    kout = L().orc_knn_k_out(K, h.shape[0])
+    out = np.zeros((nsamp, kout), np.uint32)
+    L().orc_knn(P(h, u64p), h.shape[1], h.shape[0], M, nsamp, K, lb, ub, int(using_maf),
+                P(out, u32p))
+    return out

@@ -789,3 +789,89 @@ void orc_prob_emit(const float *het_aa, unsigned M, const float *pc, float *prob
              pc[4 * j + 3] * t[2];
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* kNN candidate lists (KNN::KNN / CalculateVarAfs / CutDownHaps /
+ * ClusterHaps, kNN.cpp:8-225; Haplotype::HammingDist, haplotype.cpp:35-60).
+ * Sites are kept when the variant allele frequency over all nhaps
+ * haplotypes (as a double, alt count / nhaps; with using_maf the minor
+ * allele frequency |(|af - 0.5|) - 0.5|) lies in [lb, ub].  For sample u
+ * (haplotypes 2u, 2u+1) every other haplotype is ranked by its Hamming
+ * distance over the kept sites; std::sort leaves the order of equal
+ * distances unspecified, this restatement breaks ties by haplotype index.
+ * k' = ceil(K / 2): out[2j] = hap A's j-th nearest, out[2j + 1] = the j-th
+ * nearest of hap B not already listed.  K is first capped at nhaps - 2. */
+typedef struct {
+  unsigned d, idx;
+} orc_dist;
+
+static int orc_dist_cmp(const void *a, const void *b) {
+  const orc_dist *x = (const orc_dist *)a, *y = (const orc_dist *)b;
+  if (x->d != y->d) return x->d < y->d ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+unsigned orc_knn_k_out(unsigned K, unsigned nhaps) {
+  if (K > nhaps - 2) K = nhaps - 2;
+  return 2 * ((K + 1) / 2);
+}
+
+void orc_knn_mask(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, double lb,
+                  double ub, int using_maf, uint64_t *mask, double *af) {
+  for (unsigned w = 0; w < wn; ++w) mask[w] = 0;
+  for (unsigned s = 0; s < M; ++s) {
+    unsigned alt = 0;
+    for (unsigned h = 0; h < nhaps; ++h) alt += hbit(haps + (size_t)h * wn, s);
+    double a = (double)alt / nhaps;
+    if (af) af[s] = a;
+    if (using_maf) a = fabs(fabs(a - 0.5) - 0.5);
+    if (a >= lb && a <= ub) mask[s >> 6] |= (uint64_t)1 << (s & 63);
+  }
+}
+
+void orc_knn(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned nsamp,
+             unsigned K, double lb, double ub, int using_maf, unsigned *out) {
+  uint64_t *mask = (uint64_t *)malloc(sizeof(uint64_t) * wn);
+  orc_knn_mask(haps, wn, nhaps, M, lb, ub, using_maf, mask, NULL);
+  const unsigned kout = orc_knn_k_out(K, nhaps), kh = kout / 2;
+  orc_dist *da = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
+  orc_dist *db = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
+  for (unsigned u = 0; u < nsamp; ++u) {
+    const uint64_t *a = haps + (size_t)(2 * u) * wn, *b = a + wn;
+    unsigned n = 0;
+    for (unsigned h = 0; h < nhaps; ++h) {
+      if ((h >> 1) == u) continue;
+      const uint64_t *c = haps + (size_t)h * wn;
+      unsigned x = 0, y = 0;
+      for (unsigned w = 0; w < wn; ++w) {
+        x += (unsigned)__builtin_popcountll((a[w] ^ c[w]) & mask[w]);
+        y += (unsigned)__builtin_popcountll((b[w] ^ c[w]) & mask[w]);
+      }
+      da[n].d = x;
+      da[n].idx = h;
+      db[n].d = y;
+      db[n].idx = h;
+      ++n;
+    }
+    qsort(da, n, sizeof(orc_dist), orc_dist_cmp);
+    qsort(db, n, sizeof(orc_dist), orc_dist_cmp);
+    unsigned *o = out + (size_t)u * kout;
+    for (unsigned j = 0; j < kh; ++j) o[2 * j] = da[j].idx;
+    unsigned hb = 0;
+    for (unsigned j = 0; j < kh; ++j) {
+      for (;;) {
+        const unsigned cand = db[hb++].idx;
+        int seen = 0;
+        for (unsigned t = 0; t < kh && !seen; ++t) seen = o[2 * t] == cand;
+        for (unsigned t = 0; t < j && !seen; ++t) seen = o[2 * t + 1] == cand;
+        if (!seen) {
+          o[2 * j + 1] = cand;
+          break;
+        }
+      }
+    }
+  }
+  free(da);
+  free(db);
+  free(mask);
+}

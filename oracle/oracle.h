@@ -142,6 +142,13 @@ void orc_hmm_work(const uint64_t *haps, unsigned wn, const unsigned P[4],
 /* per-site genotype probabilities and emissions from (Het, AA) GL pairs */
 void orc_prob_emit(const float *het_aa, unsigned M, const float *pc, float *prob, float *emit);
 
+/* ---- kNN candidate lists -------------------------------------------------- */
+unsigned orc_knn_k_out(unsigned K, unsigned nhaps);
+void orc_knn_mask(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, double lb,
+                  double ub, int using_maf, uint64_t *mask, double *af);
+void orc_knn(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned nsamp,
+             unsigned K, double lb, double ub, int using_maf, unsigned *out);
+
 #ifdef __cplusplus
 }
 #endif
