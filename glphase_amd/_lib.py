@@ -149,6 +149,13 @@ SIGNATURES = [
     ("glp_vq_train", ctypes.c_int,
      [c_f32p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint64, ctypes.c_size_t, c_f32p, c_u32p]),
     ("glp_vq_codes", ctypes.c_int, [c_f32p, ctypes.c_size_t, c_f32p, ctypes.c_uint, c_u8p]),
+    ("glp_vq_train_device", ctypes.c_int,
+     [c_f32p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint64, ctypes.c_size_t, c_f32p, c_u32p]),
+    ("glp_vq_codes_device", ctypes.c_int,
+     [c_f32p, ctypes.c_size_t, c_f32p, ctypes.c_uint, c_u8p]),
+    ("glp_vq_assign_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, c_f32p, ctypes.c_uint, ctypes.c_void_p,
+      ctypes.c_void_p]),
     ("glp_pack_codes", ctypes.c_int,
      [c_u8p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_u32p]),
     ("glp_pack_nonvq", ctypes.c_int,

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglphase_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["glp_kernels.hip", "glp_work.hip", "glp_knn.hip", "glp_capi.cpp", "glp_host.cpp"]
+SOURCES = ["glp_kernels.hip", "glp_work.hip", "glp_knn.hip", "glp_vq.hip", "glp_capi.cpp",
+           "glp_host.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

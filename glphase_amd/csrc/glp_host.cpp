@@ -115,16 +115,16 @@ GLVQ::GLVQ(const std::vector<float> &inGLs, Rng &rng, unsigned codeBookSize,
            size_t training_size) {
   if (inGLs.size() % 3 != 0 || inGLs.empty()) throw std::runtime_error("GLVQ: GLs must be trios");
   const uint64_t key = (uint64_t)rng.get() | ((uint64_t)rng.get() << 32);
-  build(inGLs.data(), inGLs.size() / 3, key, codeBookSize, training_size);
+  build(inGLs.data(), inGLs.size() / 3, key, codeBookSize, training_size, nullptr);
 }
 
 GLVQ::GLVQ(const float *gls, size_t numGLs, uint64_t key, unsigned codeBookSize,
-           size_t training_size) {
-  build(gls, numGLs, key, codeBookSize, training_size);
+           size_t training_size, AssignFn assign) {
+  build(gls, numGLs, key, codeBookSize, training_size, assign);
 }
 
 void GLVQ::build(const float *inGLs, size_t numGLs, uint64_t key, unsigned codeBookSize,
-                 size_t training_size) {
+                 size_t training_size, const AssignFn &device_assign) {
   if (numGLs == 0) throw std::runtime_error("GLVQ: no GLs");
   if (codeBookSize < 4) throw std::runtime_error("GLVQ: codebook size < 4");
   if (codeBookSize - 1 > std::numeric_limits<unsigned char>::max())
@@ -158,9 +158,10 @@ void GLVQ::build(const float *inGLs, size_t numGLs, uint64_t key, unsigned codeB
   std::vector<double> d2(n);
   // AssignPoints (glVQ.cpp:149-163); the variance is summed in point order
   auto assign = [&]() {
+    if (device_assign) device_assign(m_codeBook, pts.data(), n, lab.data());
     parallel_for(n, [&](size_t b, size_t e) {
       for (size_t p = b; p < e; ++p) {
-        const unsigned c = nearest(m_codeBook, pts[2 * p], pts[2 * p + 1]);
+        const unsigned c = device_assign ? lab[p] : nearest(m_codeBook, pts[2 * p], pts[2 * p + 1]);
         lab[p] = (uint8_t)c;
         const double dr = (double)(pts[2 * p] - m_codeBook[c].first);
         const double dh = (double)(pts[2 * p + 1] - m_codeBook[c].second);

@@ -8,6 +8,7 @@
 
 #include <stdint.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -52,12 +53,19 @@ void transition_matrix(const double *r_morgans, unsigned M, float inflation, flo
 // ---- GL vector quantisation: glVQ.hpp / glVQ.cpp ----------------------------
 class GLVQ {
  public:
+  // Nearest-codeword step of AssignPoints for the n training points of the
+  // current codebook: lab[p] = FindGLCode(points[p]).  The default runs on
+  // the host's threads; the device entry points pass one that runs on the
+  // GPU (the other steps stay sequential on the host, which keeps the
+  // reference's summation order).
+  using AssignFn = std::function<void(const std::vector<std::pair<float, float>> &cb,
+                                      const float *points, size_t n, uint8_t *lab)>;
   // BuildCodeBook (glVQ.cpp:26-146); training_size 0 = every GL (reference).
   GLVQ(const std::vector<float> &inGLs, Rng &rng, unsigned codeBookSize,
        size_t training_size = 0);
   // the same with the Philox key given directly
   GLVQ(const float *gls, size_t numGLs, uint64_t key, unsigned codeBookSize,
-       size_t training_size = 0);
+       size_t training_size = 0, AssignFn assign = nullptr);
   // FindGLCode (glVQ.cpp:156-165): first index of the smallest Euclidean
   // distance (compared in FP64 after sqrt).
   unsigned char FindGLCode(float RR, float Het) const;
@@ -68,7 +76,7 @@ class GLVQ {
   std::vector<std::pair<float, float>> m_codeBook;
   unsigned m_iters = 0;
   void build(const float *gls, size_t numGLs, uint64_t key, unsigned codeBookSize,
-             size_t training_size);
+             size_t training_size, const AssignFn &assign);
 };
 
 unsigned nearest(const std::vector<std::pair<float, float>> &cb, float RR, float Het);

@@ -1,0 +1,188 @@
+// glp_vq.hip -- the GL codebook's nearest-codeword search on gfx950
+// (GLVQ::AssignPoints / FindGLCode, glVQ.cpp:149-165 and glVQ.hpp:23-34) and
+// the device entry points of codebook training and code assignment
+// (include/glphase_amd.h).
+//
+// The search is the O(points x codewords) part of training; the centroid and
+// variance sums stay sequential on the host in point order (float and double,
+// glVQ.cpp:104-117,149-163), so device-assisted training returns exactly the
+// codebook of the host path.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <exception>
+#include <vector>
+
+#include "../../include/glphase_amd.h"
+#include "glp_host.hpp"
+#include "glp_internal.hpp"
+
+namespace glp {
+
+// FindGLCode of (RR, Het): the first codeword of least
+// sqrt((c.RR - RR)^2 + (c.Het - Het)^2), the differences taken in float and
+// the rest in FP64.  The square root is taken only when the squared sum
+// improves, which gives the same choice: sqrt is monotone, so a codeword can
+// only win with a smaller sum, and it wins when its root is smaller.
+__device__ __forceinline__ unsigned vq_nearest(const float2 *cb, unsigned n, float RR, float Het) {
+  double dr = (double)(cb[0].x - RR), dh = (double)(cb[0].y - Het);
+  double bs = dr * dr + dh * dh, bd = sqrt(bs);
+  unsigned best = 0;
+  for (unsigned c = 1; c < n; ++c) {
+    dr = (double)(cb[c].x - RR);
+    dh = (double)(cb[c].y - Het);
+    const double s = dr * dr + dh * dh;
+    if (s < bs) {
+      const double d = sqrt(s);
+      if (d < bd) {
+        bd = d;
+        bs = s;
+        best = c;
+      }
+    }
+  }
+  return best;
+}
+
+// RAW: points are GL trios, normalised as glp_vq_codes does
+// (RR, Het) / ((RR + Het) + AA); else (RR, Het) pairs.
+template <bool RAW>
+__global__ void __launch_bounds__(256)
+    k_vq_assign(const float *__restrict__ pts, size_t n, const float2 *__restrict__ g_cb,
+                unsigned ncode, uint8_t *__restrict__ out) {
+  __shared__ float2 cb[256];
+  for (unsigned c = threadIdx.x; c < ncode; c += blockDim.x) cb[c] = g_cb[c];
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    float RR, Het;
+    if (RAW) {
+      RR = pts[3 * p];
+      Het = pts[3 * p + 1];
+      const float sum = RR + Het + pts[3 * p + 2];
+      RR /= sum;
+      Het /= sum;
+    } else {
+      RR = pts[2 * p];
+      Het = pts[2 * p + 1];
+    }
+    out[p] = (uint8_t)vq_nearest(cb, ncode, RR, Het);
+  }
+}
+
+int launch_vq_assign(const float *pts_dev, size_t n, bool raw, const float *cb_dev,
+                     unsigned ncode, uint8_t *out_dev, hipStream_t s) {
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  if (raw)
+    hipLaunchKernelGGL((k_vq_assign<true>), dim3(blocks), dim3(256), 0, s, pts_dev, n,
+                       reinterpret_cast<const float2 *>(cb_dev), ncode, out_dev);
+  else
+    hipLaunchKernelGGL((k_vq_assign<false>), dim3(blocks), dim3(256), 0, s, pts_dev, n,
+                       reinterpret_cast<const float2 *>(cb_dev), ncode, out_dev);
+  return hip_check(hipGetLastError(), "k_vq_assign launch");
+}
+
+}  // namespace glp
+
+using namespace glp;
+
+namespace {
+
+// Device buffers of one training or coding call, released on every path.
+struct VqBuffers {
+  float *pts = nullptr, *cb = nullptr;
+  uint8_t *lab = nullptr;
+  ~VqBuffers() {
+    if (pts) (void)hipFree(pts);
+    if (cb) (void)hipFree(cb);
+    if (lab) (void)hipFree(lab);
+  }
+};
+
+struct HipFailure : std::exception {
+  const char *what() const noexcept override { return "HIP call failed"; }
+};
+
+void must(int rc) {
+  if (rc != GLP_OK) throw HipFailure();
+}
+
+}  // namespace
+
+extern "C" int glp_vq_assign_device(const float *points_dev, size_t n, int raw,
+                                    const float *codebook, unsigned n_codes, uint8_t *codes_dev,
+                                    void *stream) {
+  if (!points_dev || !codebook || !codes_dev || n_codes == 0 || n_codes > 256)
+    return set_error(GLP_ERR_ARG, "bad argument");
+  if (n == 0) return GLP_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float *cb = nullptr;
+  int rc = hip_check(hipMallocAsync((void **)&cb, 8 * (size_t)n_codes, s), "hipMallocAsync");
+  if (rc != GLP_OK) return rc;
+  rc = hip_check(hipMemcpyAsync(cb, codebook, 8 * (size_t)n_codes, hipMemcpyHostToDevice, s),
+                 "codebook copy");
+  if (rc == GLP_OK) rc = launch_vq_assign(points_dev, n, raw != 0, cb, n_codes, codes_dev, s);
+  (void)hipFreeAsync(cb, s);
+  return rc;
+}
+
+extern "C" int glp_vq_train_device(const float *gls, size_t n_gl, unsigned n_codes, uint64_t key,
+                                   size_t training_size, float *codebook_out, unsigned *iters) {
+  if (!gls || !codebook_out || n_gl == 0) return set_error(GLP_ERR_ARG, "null or empty input");
+  try {
+    VqBuffers b;
+    size_t cap = 0;
+    GLVQ::AssignFn fn = [&](const std::vector<std::pair<float, float>> &cb, const float *points,
+                            size_t n, uint8_t *lab) {
+      if (n > cap) {  // the training points are fixed for the whole run
+        must(hip_check(hipMalloc((void **)&b.pts, 8 * n), "hipMalloc"));
+        must(hip_check(hipMalloc((void **)&b.lab, n), "hipMalloc"));
+        must(hip_check(hipMalloc((void **)&b.cb, 8 * 256), "hipMalloc"));
+        must(hip_check(hipMemcpy(b.pts, points, 8 * n, hipMemcpyHostToDevice), "points copy"));
+        cap = n;
+      }
+      std::vector<float> flat(2 * cb.size());
+      for (size_t c = 0; c < cb.size(); ++c) {
+        flat[2 * c] = cb[c].first;
+        flat[2 * c + 1] = cb[c].second;
+      }
+      must(hip_check(hipMemcpy(b.cb, flat.data(), 4 * flat.size(), hipMemcpyHostToDevice),
+                     "codebook copy"));
+      must(launch_vq_assign(b.pts, n, false, b.cb, (unsigned)cb.size(), b.lab, nullptr));
+      must(hip_check(hipMemcpy(lab, b.lab, n, hipMemcpyDeviceToHost), "labels copy"));
+    };
+    GLVQ vq(gls, n_gl, key, n_codes, training_size, fn);
+    const auto cb = vq.GetCodeBook();
+    for (size_t c = 0; c < cb.size(); ++c) {
+      codebook_out[2 * c] = cb[c].first;
+      codebook_out[2 * c + 1] = cb[c].second;
+    }
+    if (iters) *iters = vq.iterations();
+    return GLP_OK;
+  } catch (const HipFailure &) {
+    return GLP_ERR_HIP;  // the message is already set
+  } catch (const std::exception &e) {
+    return set_error(GLP_ERR_ARG, e.what());
+  }
+}
+
+extern "C" int glp_vq_codes_device(const float *gls, size_t n_gl, const float *codebook,
+                                   unsigned n_codes, uint8_t *codes_out) {
+  if (!gls || !codebook || !codes_out || n_codes == 0 || n_codes > 256)
+    return set_error(GLP_ERR_ARG, "bad argument");
+  if (n_gl == 0) return GLP_OK;
+  try {
+    VqBuffers b;
+    must(hip_check(hipMalloc((void **)&b.pts, 12 * n_gl), "hipMalloc"));
+    must(hip_check(hipMalloc((void **)&b.lab, n_gl), "hipMalloc"));
+    must(hip_check(hipMalloc((void **)&b.cb, 8 * (size_t)n_codes), "hipMalloc"));
+    must(hip_check(hipMemcpy(b.pts, gls, 12 * n_gl, hipMemcpyHostToDevice), "GL copy"));
+    must(hip_check(hipMemcpy(b.cb, codebook, 8 * (size_t)n_codes, hipMemcpyHostToDevice),
+                   "codebook copy"));
+    must(launch_vq_assign(b.pts, n_gl, true, b.cb, n_codes, b.lab, nullptr));
+    must(hip_check(hipMemcpy(codes_out, b.lab, n_gl, hipMemcpyDeviceToHost), "codes copy"));
+    return GLP_OK;
+  } catch (const HipFailure &) {
+    return GLP_ERR_HIP;  // the message is already set
+  }
+}

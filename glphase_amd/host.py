@@ -20,23 +20,28 @@ def model_params(n_haps, r_morgans, inflation=1.0):
     return tran, pc, mu.value
 
 
-def vq_train(gls, n_codes=256, key=0, training_size=0):
+def vq_train(gls, n_codes=256, key=0, training_size=0, device=False):
+    """(codebook [n_codes][2] f32, Lloyd iterations).  device=True runs the
+    nearest-codeword search of each iteration on the current GPU; the result
+    is the same."""
     g = np.ascontiguousarray(gls, dtype=np.float32).reshape(-1, 3)
     cb = np.zeros((n_codes, 2), np.float32)
     it = ctypes.c_uint32()
-    check(lib().glp_vq_train(ptr(g, c_f32p), g.shape[0], n_codes, key, training_size,
-                             ptr(cb, c_f32p), ctypes.byref(it)), "glp_vq_train")
+    name = "glp_vq_train_device" if device else "glp_vq_train"
+    check(getattr(lib(), name)(ptr(g, c_f32p), g.shape[0], n_codes, key, training_size,
+                               ptr(cb, c_f32p), ctypes.byref(it)), name)
     return cb, it.value
 
 
-def vq_codes(gls, codebook):
+def vq_codes(gls, codebook, device=False):
     g = np.ascontiguousarray(gls, dtype=np.float32)
     shape = g.shape[:-1]
     g = g.reshape(-1, 3)
     cb = np.ascontiguousarray(codebook, dtype=np.float32)
     out = np.zeros(g.shape[0], np.uint8)
-    check(lib().glp_vq_codes(ptr(g, c_f32p), g.shape[0], ptr(cb, c_f32p), cb.shape[0],
-                             ptr(out, c_u8p)), "glp_vq_codes")
+    name = "glp_vq_codes_device" if device else "glp_vq_codes"
+    check(getattr(lib(), name)(ptr(g, c_f32p), g.shape[0], ptr(cb, c_f32p), cb.shape[0],
+                               ptr(out, c_u8p)), name)
     return out.reshape(shape)
 
 

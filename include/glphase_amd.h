@@ -313,6 +313,25 @@ int glp_vq_train(const float *gls, size_t n_gl, unsigned n_codes, uint64_t key,
 int glp_vq_codes(const float *gls, size_t n_gl, const float *codebook,
                  unsigned n_codes, uint8_t *codes_out);
 
+/* glp_vq_train with the nearest-codeword search of every Lloyd iteration
+ * (GLVQ::AssignPoints, glVQ.cpp:149-163) on the current device; the
+ * centroid and variance sums stay sequential on the host in point order, so
+ * codebook and iteration count equal glp_vq_train's.  Synchronous. */
+int glp_vq_train_device(const float *gls, size_t n_gl, unsigned n_codes, uint64_t key,
+                        size_t training_size, float *codebook_out, unsigned *iters);
+
+/* glp_vq_codes on the current device (host buffers in and out). */
+int glp_vq_codes_device(const float *gls, size_t n_gl, const float *codebook,
+                        unsigned n_codes, uint8_t *codes_out);
+
+/* The search kernel itself on device buffers: codes_dev[p] = nearest
+ * codeword of point p, where points_dev holds GL trios (raw != 0,
+ * normalised by (RR + Het) + AA as glp_vq_codes) or normalised (RR, Het)
+ * pairs (raw == 0); codebook is a host array of n_codes (RR, Het) pairs.
+ * Enqueued on `stream`. */
+int glp_vq_assign_device(const float *points_dev, size_t n, int raw, const float *codebook,
+                         unsigned n_codes, uint8_t *codes_dev, void *stream);
+
 /* GLPack::GetPackedGLs (glPack.cpp:32-192) for samples [s0, s1) of codes
  * [n_samples][num_sites]; out = [ceil(num_sites*bits/32)][s1-s0]. */
 int glp_pack_codes(const uint8_t *codes, unsigned n_samples,

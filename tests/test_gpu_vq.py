@@ -1,0 +1,97 @@
+"""GL codebook training and VQ coding with the nearest-codeword search on the
+GPU (glp_vq_train_device, glp_vq_codes_device, glp_vq_assign_device) against
+the oracle's GLVQ restatement (glVQ.cpp:26-165, glPack.cpp:195-219): the same
+codebook bit for bit, the same iteration count, the same codes -- including
+codebooks with repeated codewords, where the first of the tied codewords must
+win, and the 5,000-sample x 4,096-site GL set of the MH update benchmark."""
+import sys
+
+import numpy as np
+import pytest
+
+from glphase_amd import host, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gls(N, M, seed, depth=4.0):
+    return synth.make_dataset(N, M, depth=depth, seed=seed)["gls"]
+
+
+@pytest.mark.parametrize("N,M,codes,training", [(40, 300, 256, 0), (200, 500, 16, 0),
+                                                (300, 1000, 256, 1 << 15), (64, 256, 4, 0)])
+def test_vq_train_device_matches_oracle(orc, N, M, codes, training):
+    say(f"vq train N={N} M={M} codes={codes} training={training}")
+    gls = _gls(N, M, seed=N + M)
+    cb, it = host.vq_train(gls, codes, key=5, training_size=training, device=True)
+    rc, cbo, ito = orc.vq_train(gls, codes, 5, training)
+    assert rc == 0
+    assert it == ito
+    assert cb.tobytes() == cbo.tobytes()
+    got = host.vq_codes(gls, cb, device=True)
+    assert (got == orc.vq_codes(gls, cb)).all()
+
+
+def test_vq_codes_device_ties_take_first(orc):
+    say("vq codes with repeated codewords")
+    rng = np.random.default_rng(4)
+    base = rng.random((40, 2)).astype(np.float32) * np.float32(0.5)
+    cb = np.concatenate([base, base[::-1], base[:8]]).astype(np.float32)  # every codeword twice+
+    # GL trios whose normalised (RR, Het) land exactly on codewords, and
+    # points half-way between two codewords
+    aa = np.float32(1) - cb[:, 0] - cb[:, 1]
+    on = np.stack([cb[:, 0], cb[:, 1], aa], 1)
+    mid = (cb[:40] + cb[40:80]) / np.float32(2)
+    mid = np.stack([mid[:, 0], mid[:, 1], np.float32(1) - mid[:, 0] - mid[:, 1]], 1)
+    rnd = rng.random((5000, 3)).astype(np.float32)
+    gls = np.concatenate([on, mid, rnd]).astype(np.float32)
+    got = host.vq_codes(gls, cb, device=True)
+    want = orc.vq_codes(gls, cb)
+    assert (got == want).all()
+    assert (got == host.vq_codes(gls, cb)).all()
+
+
+def test_vq_assign_device_on_pairs(orc):
+    say("vq assign on normalised pairs")
+    import torch
+    from glphase_amd._lib import c_f32p, check, lib, ptr
+    gls = _gls(100, 400, seed=2).reshape(-1, 3)
+    cb, _ = host.vq_train(gls, 64, key=1)
+    s = gls.sum(1, dtype=np.float32)
+    pts = np.stack([gls[:, 0] / s, gls[:, 1] / s], 1).astype(np.float32)
+    dev = torch.from_numpy(pts).cuda()
+    out = torch.zeros(pts.shape[0], dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().glp_vq_assign_device(dev.data_ptr(), pts.shape[0], 0, ptr(cb, c_f32p), 64,
+                                     out.data_ptr(), stream), "glp_vq_assign_device")
+    torch.cuda.synchronize()
+    want = np.array([orc.vq_find(cb, float(r), float(h)) for r, h in pts[:4000]], np.uint8)
+    assert (out.cpu().numpy()[:4000] == want).all()
+
+
+def test_vq_full_benchmark_gls(orc):
+    # the GL set of the MH update benchmark: 5,000 samples x 4,096 sites
+    say("vq 5000 x 4096 GLs")
+    gls = _gls(5000, 4096, seed=0)
+    cb, it = host.vq_train(gls, 256, key=0, training_size=1 << 16, device=True)
+    cbh, ith = host.vq_train(gls, 256, key=0, training_size=1 << 16)
+    assert it == ith and cb.tobytes() == cbh.tobytes()
+    codes = host.vq_codes(gls, cb, device=True)
+    assert (codes == host.vq_codes(gls, cb)).all()
+    pick = np.random.default_rng(0).integers(0, 5000 * 4096, 200000)
+    flat = gls.reshape(-1, 3)
+    assert (codes.reshape(-1)[pick] == orc.vq_codes(flat[pick], cb)).all()
+
+
+def test_vq_device_rejects_bad_arguments():
+    from glphase_amd._lib import GlpError
+    gls = np.ones((10, 3), np.float32)
+    with pytest.raises(GlpError):
+        host.vq_codes(gls, np.zeros((300, 2), np.float32), device=True)
+    with pytest.raises(GlpError):
+        host.vq_train(gls, 2, device=True)
