@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_SE = 29      # 2 pair sums + 4 x (3 mul + 2 add) + 4 emission mul + 3 adds
 PEAK_FP32_TFLOPS = 157.3
+VQ_TRAINING = 1 << 18  # GL trios the codebook is trained on
 
 
 def log(*a):
@@ -39,12 +40,15 @@ def make_workload(N, M, C, depth, seed):
     hb = synth.initial_haps(d["gls"], seed=seed + 1)
     panel = synth.pack_haps(hb)
     tran, pc, mu = host.model_params(2 * N, d["r_morgans"])
-    cb, iters = host.vq_train(d["gls"], 256, key=seed, training_size=1 << 18)
+    t1 = time.perf_counter()
+    cb, iters = host.vq_train(d["gls"], 256, key=seed, training_size=VQ_TRAINING)
+    vq_host_s = time.perf_counter() - t1
     codes = host.vq_codes(d["gls"], cb)
     packed = host.pack_codes(codes, 8)
     log(f"workload N={N} M={M} C={C}: generated in {time.time() - t0:.1f}s (VQ {iters} iterations)")
     return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, N=N, M=M,
-                H=panel.shape[0], het_aa=np.ascontiguousarray(d["gls"][:, :, 1:3]))
+                H=panel.shape[0], het_aa=np.ascontiguousarray(d["gls"][:, :, 1:3]),
+                gls=d["gls"], codes=codes, vq_iters=iters, vq_host_s=vq_host_s, seed=seed)
 
 
 def time_steps(torch, fn, steps, warmup, stream_events=True):
@@ -100,6 +104,79 @@ def cpu_baseline(w, C, pen, seed, budget_s=12.0, precision=1):
     }
 
 
+def aux_knn(torch, steps, seed=11):
+    """kNN candidate lists (KNN::ClusterHaps) of the reference-panel workload:
+    10,000 samples plus 20,000 panel haplotypes x 2,048 sites, K = 200, panel
+    resident on the device.  Cost unit: one 64-bit word compared between a
+    query haplotype and a panel haplotype (20,000 x 40,000 x 32 per build).
+    The oracle's kNN is timed on the host on a prefix of the samples."""
+    from glphase_amd import synth
+    from glphase_amd.engine import knn_lists
+    from oracle import bind as orc
+    N, M, R, K = 10000, 2048, 20000, 200
+    t0 = time.time()
+    d = synth.make_dataset(N, M, depth=4.0, seed=seed, n_ref_haps=R)
+    hb = np.concatenate([synth.initial_haps(d["gls"], seed=seed + 1), d["ref_haps"]], axis=0)
+    panel = synth.pack_haps(hb)
+    H, wn = panel.shape
+    log(f"kNN workload: {N} samples + {R} panel haplotypes x {M} sites in {time.time() - t0:.1f}s")
+    pd = torch.from_numpy(panel.view(np.int64)).to("cuda:0")
+    out = knn_lists(pd, M, N, K)
+    secs, _ = time_steps(torch, lambda: knn_lists(pd, M, N, K), steps, 1)
+    ms = secs / steps * 1e3
+    words = 2 * N * H * wn
+    ns = 200
+    t0 = time.perf_counter()
+    want = orc.knn(panel, M, ns, K)
+    cpu_s = time.perf_counter() - t0
+    same = bool((out[:ns].cpu().numpy().view(np.uint32) == want).all())
+    log(f"kNN: {ms:.3f} ms/build, {words / (ms / 1e3):.3e} word-compares/s; "
+        f"oracle {ns} samples in {cpu_s:.2f}s; first {ns} rows identical: {same}")
+    return {"ms_per_build": ms, "unit": "word-compares/s", "value": words / (ms / 1e3),
+            "config": {"samples": N, "panel_haplotypes": R, "sites": M, "K": K},
+            "identical_to_oracle_rows": ns if same else 0,
+            "cpu_baseline": {"value": 2 * ns * H * wn / cpu_s, "unit": "word-compares/s",
+                             "cores": 1, "kind": "port",
+                             "sample": f"oracle kNN lists of {ns} samples ({cpu_s:.2f}s)"}}
+
+
+def aux_vq(torch, w, steps):
+    """GL codebook training (GLVQ) and coding of the MH workload's GLs with
+    the nearest-codeword search on the device.  Coding cost unit: one
+    (GL, codeword) distance; the GLs are resident on the device."""
+    from glphase_amd import host
+    from glphase_amd._lib import c_f32p, check, lib, ptr
+    gls, cb = w["gls"], w["codebook"]
+    t0 = time.perf_counter()
+    cbd, itd = host.vq_train(gls, 256, key=w["seed"], training_size=VQ_TRAINING, device=True)
+    dev_s = time.perf_counter() - t0
+    same_cb = bool(cbd.tobytes() == cb.tobytes() and itd == w["vq_iters"])
+    flat = np.ascontiguousarray(gls.reshape(-1, 3))
+    n = flat.shape[0]
+    gd = torch.from_numpy(flat).to("cuda:0")
+    out = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def code():
+        check(lib().glp_vq_assign_device(gd.data_ptr(), n, 1, ptr(cb, c_f32p), cb.shape[0],
+                                         out.data_ptr(), stream), "glp_vq_assign_device")
+
+    secs, _ = time_steps(torch, code, steps, 1)
+    ms = secs / steps * 1e3
+    same_codes = bool((out.cpu().numpy() == w["codes"].reshape(-1)).all())
+    t0 = time.perf_counter()
+    host.vq_codes(gls, cb)
+    host_codes_s = time.perf_counter() - t0
+    log(f"VQ: training on {VQ_TRAINING} GLs {w['vq_host_s']:.2f}s host / {dev_s:.2f}s device "
+        f"(identical: {same_cb}); codes of {n} GLs {ms:.3f} ms device / "
+        f"{host_codes_s * 1e3:.1f} ms host (identical: {same_codes})")
+    return {"codes_ms": ms, "unit": "distances/s", "value": n * cb.shape[0] / (ms / 1e3),
+            "gls": n, "codewords": int(cb.shape[0]), "codes_identical": same_codes,
+            "host_codes_ms": host_codes_s * 1e3,
+            "train": {"training_gls": VQ_TRAINING, "iterations": itd,
+                      "device_s": dev_s, "host_s": w["vq_host_s"], "identical": same_cb}}
+
+
 def recorded_traffic(workload):
     """HBM bytes per dispatch of the batched kernel for this workload, from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
@@ -144,6 +221,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="lanes per chain (0 = auto)")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-aux", action="store_true", help="skip the kNN and VQ measurements")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES bytes (counter scaling)")
@@ -269,6 +347,9 @@ def main():
                      "flop_per_site_eval": FLOP_PER_SE,
                      "hbm_gbps": None if traffic is None else traffic / (ms / 1e3) / 1e9},
     }
+    if not args.no_aux:
+        out["vq"] = aux_vq(torch, w, args.steps)
+        out["knn"] = aux_knn(torch, args.steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
     print(json.dumps(out), flush=True)

@@ -5,10 +5,10 @@
 // of kept sites (CalculateVarAfs / CutDownHaps); a masked, word-major copy of
 // the panel feeds an all-pairs Hamming kernel that writes 16-bit distances
 // of a batch of query haplotypes (the samples' own) against the panel; one
-// workgroup per sample then selects its neighbours by a two-pass 8-bit radix
-// select, an index-ordered compaction of the ties at the threshold and a
-// rank sort by (distance, index), and interleaves A's and B's lists as
-// ClusterHaps does (kNN.cpp:176-199).
+// wavefront per query row then selects its neighbours from a histogram of
+// the row's distances, an index-ordered compaction of the ties at the
+// threshold and a rank sort by (distance, index), and one workgroup per
+// sample interleaves A's and B's lists as ClusterHaps does (kNN.cpp:176-199).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,7 +24,8 @@ namespace glp {
 constexpr unsigned kKnnBatch = 4096;     // query haplotypes per distance batch
 constexpr unsigned kKnnCountChunk = 1024;  // haplotypes per allele-count block
 constexpr unsigned kKnnMaxHalf = 1024;   // ceil(K / 2) limit
-constexpr unsigned kKnnQpb = 8;          // queries per distance block
+constexpr unsigned kKnnQpb = 32;         // queries per distance block
+constexpr unsigned kKnnBins = 2560;      // distance histogram bins per row
 constexpr unsigned kKnnMaxWords = 1024;  // 65,536 sites
 
 // Alt-allele counts of the 64 sites of word w over haplotypes [h0, h0 + 1024).
@@ -59,141 +60,190 @@ __global__ void __launch_bounds__(256)
   if (h < H) pt[(size_t)w * H + h] = panel[(size_t)h * wn + w] & mask[w];
 }
 
-// Distances of queries q0 + blockIdx.y * 8 + q (q < 8, < q0 + nq) to every
-// haplotype h: d[(query - q0) * H + h].
+// Distances of queries q0 + blockIdx.y * kKnnQpb + q (q < kKnnQpb, below
+// q0 + nq) to every haplotype h: d[(query - q0) * H + h].  A query's words are
+// the same for every lane, so they come through the scalar unit in one run of
+// kKnnQpb words; each lane streams its haplotype's words once per kKnnQpb
+// queries.  The last block's run may pass its last query, and the last word
+// row's may pass the panel's end: pt is followed by kKnnQpb words of padding.
 __global__ void __launch_bounds__(256)
     k_knn_dist(const uint64_t *__restrict__ pt, unsigned H, unsigned wn, unsigned q0,
                unsigned nq, uint16_t *__restrict__ d) {
-  extern __shared__ uint64_t sq[];  // [wn][kKnnQpb]
-  const unsigned qb = blockIdx.y * kKnnQpb;
-  for (unsigned t = threadIdx.x; t < wn * kKnnQpb; t += blockDim.x) {
-    const unsigned w = t / kKnnQpb, q = t % kKnnQpb;
-    sq[t] = qb + q < nq ? pt[(size_t)w * H + q0 + qb + q] : 0ull;
-  }
-  __syncthreads();
+  const unsigned qb = blockIdx.y * kKnnQpb, last = nq - 1 - qb;  // grid: qb < nq
   const unsigned h = blockIdx.x * 256u + threadIdx.x;
   if (h >= H) return;
   unsigned acc[kKnnQpb];
 #pragma unroll
   for (unsigned q = 0; q < kKnnQpb; ++q) acc[q] = 0;
   for (unsigned w = 0; w < wn; ++w) {
-    const uint64_t c = pt[(size_t)w * H + h];
+    const uint64_t *row = pt + (size_t)w * H;
+    const uint64_t c = row[h];
+    const uint64_t *qw = row + q0 + qb;
 #pragma unroll
-    for (unsigned q = 0; q < kKnnQpb; ++q) acc[q] += (unsigned)__popcll(c ^ sq[w * kKnnQpb + q]);
+    for (unsigned q = 0; q < kKnnQpb; ++q)
+      acc[q] += (unsigned)__popcll(c ^ qw[q]);
   }
 #pragma unroll
   for (unsigned q = 0; q < kKnnQpb; ++q)
-    if (qb + q < nq) d[(size_t)(qb + q) * H + h] = (uint16_t)acc[q];
+    if (q <= last) d[(size_t)(qb + q) * H + h] = (uint16_t)acc[q];
 }
 
-// Exclusive prefix sums of a 0/1 flag over the 256 threads of a block;
-// returns the thread's prefix and sets *total.
-__device__ __forceinline__ unsigned block_scan(bool f, unsigned *s_wave, unsigned *total) {
-  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint64_t b = __ballot(f);
-  const unsigned in_wave = (unsigned)__popcll(b & (lane ? (~0ull >> (64 - lane)) : 0ull));
-  if (lane == 0) s_wave[wave] = (unsigned)__popcll(b);
-  __syncthreads();
-  unsigned before = 0, all = 0;
-  for (unsigned v = 0; v < 4; ++v) {
-    if (v < wave) before += s_wave[v];
-    all += s_wave[v];
-  }
-  __syncthreads();
-  *total = all;
-  return before + in_wave;
+// Orders a wavefront's LDS accesses before and after this point (the
+// wavefront's own lanes only; no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The k nearest haplotypes of row `dr` (length H; haplotypes 2u, 2u+1
-// excluded) by (distance, index), as keys (d << 32 | idx) in out[0, k).
-__device__ void knn_select(const uint16_t *__restrict__ dr, unsigned H, unsigned u, unsigned k,
-                           uint64_t *out, uint64_t *tmp, unsigned *hist, unsigned *s_wave,
-                           unsigned *s_misc) {
-  const unsigned tid = threadIdx.x;
-  // pass 1: high bytes
-  for (unsigned b = tid; b < 256; b += 256) hist[b] = 0;
-  __syncthreads();
-  for (unsigned h = tid; h < H; h += 256)
-    if ((h >> 1) != u) atomicAdd(&hist[dr[h] >> 8], 1u);
-  __syncthreads();
-  if (tid == 0) {
-    unsigned cum = 0, b = 0;
-    while (cum + hist[b] < k) cum += hist[b++];
-    s_misc[0] = b;
-    s_misc[1] = cum;
+__device__ __forceinline__ uint64_t lanes_below() {
+  const unsigned lane = threadIdx.x & 63u;
+  return lane ? ~0ull >> (64 - lane) : 0ull;
+}
+
+// The bin b of hist[0, nb) (nb a multiple of 64) holding the k-th smallest
+// counted value (1 <= k <= total count): cum(< b) < k <= cum(<= b); *below
+// = cum(< b).  Each lane sums nb / 64 consecutive bins, a lane scan finds the
+// lane holding the k-th value, and that lane walks its bins.
+__device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k, unsigned *below) {
+  const unsigned lane = threadIdx.x & 63u, per = nb / 64;
+  const unsigned *mine = hist + lane * per;
+  unsigned s = 0;
+  for (unsigned i = 0; i < per; ++i) s += mine[i];
+  unsigned incl = s;
+  for (unsigned o = 1; o < 64; o <<= 1) {
+    const unsigned t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
   }
-  __syncthreads();
-  const unsigned hb = s_misc[0], below = s_misc[1];
-  __syncthreads();
-  // pass 2: low bytes under the chosen high byte
-  hist[tid] = 0;
-  __syncthreads();
-  for (unsigned h = tid; h < H; h += 256)
-    if ((h >> 1) != u && (dr[h] >> 8) == hb) atomicAdd(&hist[dr[h] & 255u], 1u);
-  __syncthreads();
-  if (tid == 0) {
-    unsigned cum = below, b = 0;
-    while (cum + hist[b] < k) cum += hist[b++];
-    s_misc[2] = (hb << 8) | b;  // threshold distance
-    s_misc[3] = k - cum;        // ties at the threshold to take
+  const unsigned excl = incl - s;
+  const bool here = excl < k && k <= incl;
+  const unsigned src = (unsigned)__builtin_ctzll(__ballot(here));
+  unsigned bin = 0, cum = excl;
+  if (here) {
+    unsigned i = 0;
+    while (cum + mine[i] < k) cum += mine[i++];
+    bin = lane * per + i;
   }
-  __syncthreads();
-  const unsigned t = s_misc[2], need = s_misc[3];
-  // index-ordered compaction: every d < t, and the first `need` with d == t
-  unsigned taken = 0, ties = 0;
+  *below = (unsigned)__shfl((int)cum, (int)src, 64);
+  return (unsigned)__shfl((int)bin, (int)src, 64);
+}
+
+// The k nearest haplotypes of row dr (length H; haplotypes 2u and 2u + 1
+// excluded) by (distance, index), as keys (d << 32 | h) sorted into out[0, k).
+// One wavefront per row, no workgroup barrier:
+//  1. a histogram of d >> shift (exact distances when shift = 0) in the
+//     wavefront's own kKnnBins counters gives the threshold bin; with
+//     shift > 0, a second histogram of the low bits inside that bin refines
+//     it to the threshold distance t and the count of distances below t;
+//  2. one pass in index order keeps every d < t and the first `need` with
+//     d == t, placed by ballot prefix counts (so ties go by index);
+//  3. a rank sort of the k keys.
+__device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
+                                unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
+                                uint64_t *out) {
+  const unsigned lane = threadIdx.x & 63u;
+  for (unsigned b = lane; b < kKnnBins; b += 64) hist[b] = 0;
+  wave_lds_sync();
   for (unsigned base = 0; base < H; base += 256) {
-    const unsigned h = base + tid;
-    const bool ok = h < H && (h >> 1) != u;
-    const unsigned dv = ok ? dr[h] : 0xFFFFu;
-    const bool eq = ok && dv == t;
-    unsigned eq_total = 0;
-    const unsigned eq_rank = block_scan(eq, s_wave, &eq_total);
-    const bool take = (ok && dv < t) || (eq && ties + eq_rank < need);
-    unsigned take_total = 0;
-    const unsigned pos = block_scan(take, s_wave, &take_total);
-    if (take) tmp[taken + pos] = ((uint64_t)dv << 32) | h;
-    taken += take_total;
-    ties += eq_total;
+    unsigned v[4];
+#pragma unroll
+    for (unsigned j = 0; j < 4; ++j) {
+      const unsigned h = base + 64 * j + lane;
+      v[j] = h < H && (h >> 1) != u ? dr[h] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (unsigned j = 0; j < 4; ++j)
+      if (v[j] != 0xFFFFFFFFu) atomicAdd(&hist[v[j] >> shift], 1u);
   }
-  __syncthreads();
-  // rank sort of the k distinct keys
-  for (unsigned i = tid; i < k; i += 256) {
+  wave_lds_sync();
+  unsigned below;
+  unsigned t = wave_find_bin(hist, kKnnBins, k, &below);
+  if (shift) {
+    wave_lds_sync();
+    hist[lane] = 0;
+    wave_lds_sync();
+    const unsigned lo_mask = (1u << shift) - 1;
+    for (unsigned base = 0; base < H; base += 256) {
+      unsigned v[4];
+#pragma unroll
+      for (unsigned j = 0; j < 4; ++j) {
+        const unsigned h = base + 64 * j + lane;
+        v[j] = h < H && (h >> 1) != u ? dr[h] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (unsigned j = 0; j < 4; ++j)
+        if (v[j] != 0xFFFFFFFFu && (v[j] >> shift) == t) atomicAdd(&hist[v[j] & lo_mask], 1u);
+    }
+    wave_lds_sync();
+    unsigned below_lo;
+    const unsigned lo = wave_find_bin(hist, 64, k - below, &below_lo);
+    t = (t << shift) | lo;
+    below += below_lo;
+  }
+  const unsigned need = k - below;  // ties at t to take, lowest indices first
+  const uint64_t lt = lanes_below();
+  unsigned nless = 0, nties = 0;
+  for (unsigned base = 0; base < H && (nless < below || nties < need); base += 64) {
+    const unsigned h = base + lane;
+    const bool ok = h < H && (h >> 1) != u;
+    const unsigned dv = ok ? dr[h] : 0xFFFFFFFFu;
+    const bool less = dv < t, eq = dv == t;
+    const uint64_t bl = __ballot(less), be = __ballot(eq);
+    const uint64_t key = ((uint64_t)dv << 32) | h;
+    if (less) tmp[nless + (unsigned)__popcll(bl & lt)] = key;
+    if (eq) {
+      const unsigned r = nties + (unsigned)__popcll(be & lt);
+      if (r < need) tmp[below + r] = key;
+    }
+    nless += (unsigned)__popcll(bl);
+    nties += (unsigned)__popcll(be);
+  }
+  wave_lds_sync();
+  for (unsigned i = lane; i < k; i += 64) {
     const uint64_t key = tmp[i];
     unsigned r = 0;
     for (unsigned j = 0; j < k; ++j) r += tmp[j] < key;
     out[r] = key;
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
-// One workgroup per sample u in [u0, u0 + nu): rows of d are the batch's
-// queries from haplotype q0 = 2 * u0.
-__global__ void __launch_bounds__(256)
+// One workgroup of two wavefronts per sample u in [u0, u0 + nu): wavefront 0
+// selects from row 2(u - u0) of d (haplotype 2u, list A, kh nearest),
+// wavefront 1 from the next row (haplotype 2u + 1, list B, kb nearest); then
+// wavefront 0 interleaves A with B's nearest not already in A (kNN.cpp:176-199).
+// Dynamic LDS: keys A (2 kh), B (2 kb), then 2 x kKnnBins counters.
+__global__ void __launch_bounds__(128)
     k_knn_select(const uint16_t *__restrict__ d, unsigned H, unsigned u0, unsigned kh,
-                 unsigned kb, unsigned kout, unsigned *__restrict__ klist) {
-  __shared__ uint64_t s_a[kKnnMaxHalf], s_b[2 * kKnnMaxHalf], s_tmp[2 * kKnnMaxHalf];
-  __shared__ unsigned s_hist[256], s_wave[4], s_misc[4];
-  const unsigned u = u0 + blockIdx.x;
-  const uint16_t *da = d + (size_t)(2 * blockIdx.x) * H, *db = da + H;
-  knn_select(da, H, u, kh, s_a, s_tmp, s_hist, s_wave, s_misc);
-  knn_select(db, H, u, kb, s_b, s_tmp, s_hist, s_wave, s_misc);
-  unsigned *row = klist + (size_t)u * kout;
-  for (unsigned j = threadIdx.x; j < kh; j += 256) row[2 * j] = (unsigned)s_a[j];
-  // B's nearest not already among A's, in B's order
+                 unsigned kb, unsigned kout, unsigned shift, unsigned *__restrict__ klist) {
+  extern __shared__ uint64_t s_keys[];
+  uint64_t *a_tmp = s_keys, *a_out = a_tmp + kh, *b_tmp = a_out + kh, *b_out = b_tmp + kb;
+  unsigned *hist = reinterpret_cast<unsigned *>(b_out + kb);
+  const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, u = u0 + blockIdx.x;
+  const uint16_t *row = d + (size_t)(2 * blockIdx.x + wave) * H;
+  if (wave == 0)
+    knn_select_wave(row, H, u, kh, shift, hist, a_tmp, a_out);
+  else
+    knn_select_wave(row, H, u, kb, shift, hist + kKnnBins, b_tmp, b_out);
+  __syncthreads();
+  if (wave) return;
+  unsigned *out = klist + (size_t)u * kout;
+  for (unsigned j = lane; j < kh; j += 64) out[2 * j] = (unsigned)a_out[j];
+  const uint64_t lt = lanes_below();
   unsigned placed = 0;
-  for (unsigned base = 0; base < kb && placed < kh; base += 256) {
-    const unsigned i = base + threadIdx.x;
+  for (unsigned base = 0; base < kb && placed < kh; base += 64) {
+    const unsigned i = base + lane;
     bool fresh = false;
     unsigned idx = 0;
     if (i < kb) {
-      idx = (unsigned)s_b[i];
+      idx = (unsigned)b_out[i];
       fresh = true;
-      for (unsigned j = 0; j < kh && fresh; ++j) fresh = (unsigned)s_a[j] != idx;
+      for (unsigned j = 0; j < kh && fresh; ++j) fresh = (unsigned)a_out[j] != idx;
     }
-    unsigned total = 0;
-    const unsigned pos = block_scan(fresh, s_wave, &total);
-    if (fresh && placed + pos < kh) row[2 * (placed + pos) + 1] = idx;
-    placed += total;
+    const uint64_t bal = __ballot(fresh);
+    const unsigned pos = placed + (unsigned)__popcll(bal & lt);
+    if (fresh && pos < kh) out[2 * pos + 1] = idx;
+    placed += (unsigned)__popcll(bal);
   }
 }
 
@@ -214,7 +264,7 @@ KnnLayout knn_layout(const glp_knn_desc *d) {
   L.mask = 0;
   L.counts = (wn * 8 + 255) & ~(size_t)255;
   L.pt = L.counts + ((64 * wn * 4 + 255) & ~(size_t)255);
-  L.dist = L.pt + wn * H * 8;
+  L.dist = L.pt + (((wn * H + kKnnQpb) * 8 + 255) & ~(size_t)255);  // padding: k_knn_dist
   L.total = L.dist + batch * H * 2;
   return L;
 }
@@ -270,18 +320,21 @@ extern "C" int glp_knn_build(const glp_knn_desc *d, void *stream) {
   hipLaunchKernelGGL(k_knn_transpose, dim3((H + 255) / 256, wn), dim3(256), 0, s, d->panel, H, wn,
                      mask, pt);
   rc = hip_check(hipGetLastError(), "kNN setup launch");
-  const size_t lds = (size_t)wn * kKnnQpb * 8;
+  // distances are at most M: bins of 2^shift distances keep them below kKnnBins
+  unsigned shift = 0;
+  while ((M >> shift) >= kKnnBins) ++shift;
+  const size_t lds = (size_t)(2 * kh + 2 * kb) * 8 + 2 * kKnnBins * 4;
   if (rc == GLP_OK)
-    rc = hip_check(hipFuncSetAttribute((const void *)&k_knn_dist,
+    rc = hip_check(hipFuncSetAttribute((const void *)&k_knn_select,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                    "dynamic LDS attribute");
   const unsigned per = kKnnBatch / 2;  // samples per batch
   for (unsigned u0 = 0; u0 < d->num_samples && rc == GLP_OK; u0 += per) {
     const unsigned nu = std::min(per, d->num_samples - u0), nq = 2 * nu;
     hipLaunchKernelGGL(k_knn_dist, dim3((H + 255) / 256, (nq + kKnnQpb - 1) / kKnnQpb), dim3(256),
-                       lds, s, pt, H, wn, 2 * u0, nq, dist);
-    hipLaunchKernelGGL(k_knn_select, dim3(nu), dim3(256), 0, s, dist, H, u0, kh, kb, kout,
-                       d->klist);
+                       0, s, pt, H, wn, 2 * u0, nq, dist);
+    hipLaunchKernelGGL(k_knn_select, dim3(nu), dim3(128), lds, s, dist, H, u0, kh, kb, kout,
+                       shift, d->klist);
     rc = hip_check(hipGetLastError(), "kNN batch launch");
   }
   return rc;

@@ -21,24 +21,34 @@ namespace glp {
 
 // FindGLCode of (RR, Het): the first codeword of least
 // sqrt((c.RR - RR)^2 + (c.Het - Het)^2), the differences taken in float and
-// the rest in FP64.  The square root is taken only when the squared sum
-// improves, which gives the same choice: sqrt is monotone, so a codeword can
-// only win with a smaller sum, and it wins when its root is smaller.
+// the rest in FP64, decided without most of the square roots:
+//  - sqrt is monotone, so codeword c can only win when its squared sum s is
+//    below the best one, bs;
+//  - when s < round(bs * (1 - 2^-50)), the exact roots differ by a factor
+//    below 1 - 1.7 * 2^-52 and the correctly rounded roots (relative error
+//    <= 2^-53 each) are strictly ordered: c wins;
+//  - only in the band between (a near tie) are the two rounded roots compared
+//    as FindGLCode compares them, so equal roots keep the earlier codeword.
 __device__ __forceinline__ unsigned vq_nearest(const float2 *cb, unsigned n, float RR, float Het) {
+  constexpr double kShrink = 1.0 - 0x1p-50;
   double dr = (double)(cb[0].x - RR), dh = (double)(cb[0].y - Het);
-  double bs = dr * dr + dh * dh, bd = sqrt(bs);
+  double bs = dr * dr + dh * dh, thr = bs * kShrink;
   unsigned best = 0;
   for (unsigned c = 1; c < n; ++c) {
     dr = (double)(cb[c].x - RR);
     dh = (double)(cb[c].y - Het);
     const double s = dr * dr + dh * dh;
-    if (s < bs) {
-      const double d = sqrt(s);
-      if (d < bd) {
-        bd = d;
-        bs = s;
-        best = c;
-      }
+    bool win = s < thr;
+    if (!win && s < bs) {
+      // near tie, rare: the empty asm keeps the compiler from computing both
+      // square roots for every codeword and selecting afterwards
+      asm volatile("");
+      win = sqrt(s) < sqrt(bs);
+    }
+    if (win) {
+      bs = s;
+      thr = s * kShrink;
+      best = c;
     }
   }
   return best;

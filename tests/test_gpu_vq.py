@@ -2,8 +2,10 @@
 GPU (glp_vq_train_device, glp_vq_codes_device, glp_vq_assign_device) against
 the oracle's GLVQ restatement (glVQ.cpp:26-165, glPack.cpp:195-219): the same
 codebook bit for bit, the same iteration count, the same codes -- including
-codebooks with repeated codewords, where the first of the tied codewords must
-win, and the 5,000-sample x 4,096-site GL set of the MH update benchmark."""
+codebooks with repeated codewords and squared distances one or two units in
+the last place apart, where the first of two codewords at equal rounded
+distance must win, and the 5,000-sample x 4,096-site GL set of the MH update
+benchmark."""
 import sys
 
 import numpy as np
@@ -72,6 +74,45 @@ def test_vq_assign_device_on_pairs(orc):
     torch.cuda.synchronize()
     want = np.array([orc.vq_find(cb, float(r), float(h)) for r, h in pts[:4000]], np.uint8)
     assert (out.cpu().numpy()[:4000] == want).all()
+
+
+def _near_ties(count, seed=1):
+    """Codeword pairs (a, 0), (b1, b2) whose squared distances from (0, 0)
+    differ in the last bits: s1 * (1 - 2^-50) <= s2 < s1.  Returns the pairs
+    whose double roots are equal (the first codeword must win) and those
+    whose roots differ (the nearer one must win)."""
+    rng = np.random.default_rng(seed)
+    eq, lt = [], []
+    while len(eq) < count or len(lt) < count:
+        a = np.float32(rng.uniform(0.3, 0.9))
+        b1 = np.nextafter(a, np.float32(0))
+        s1 = np.float64(a) ** 2
+        b2c = np.float32(np.sqrt(s1 - np.float64(b1) ** 2))
+        for b2 in (b2c, np.nextafter(b2c, np.float32(0)), np.nextafter(b2c, np.float32(1))):
+            s2 = np.float64(b1) ** 2 + np.float64(b2) ** 2
+            if s1 * (1 - 2.0**-50) <= s2 < s1:
+                (eq if np.sqrt(s2) == np.sqrt(s1) else lt).append((a, b1, b2))
+    return eq[:count], lt[:count]
+
+
+def test_vq_assign_device_near_ties(orc):
+    say("vq assign on near-tie distances")
+    import torch
+    from glphase_amd._lib import c_f32p, check, lib, ptr
+    eq, lt = _near_ties(24)
+    pt = torch.zeros((1, 2), dtype=torch.float32, device="cuda")
+    out = torch.zeros(1, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    for kind, cases in (("equal roots", eq), ("distinct roots", lt)):
+        for a, b1, b2 in cases:
+            for cb in (np.array([[a, 0], [b1, b2]], np.float32),
+                       np.array([[b1, b2], [a, 0]], np.float32)):
+                check(lib().glp_vq_assign_device(pt.data_ptr(), 1, 0, ptr(cb, c_f32p), 2,
+                                                 out.data_ptr(), stream), "glp_vq_assign_device")
+                want = orc.vq_find(cb, 0.0, 0.0)
+                assert int(out.cpu()[0]) == want, (kind, cb.tolist())
+            if kind == "equal roots":
+                assert want == 0  # the later codeword lost the tie
 
 
 def test_vq_full_benchmark_gls(orc):
