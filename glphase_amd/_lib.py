@@ -144,6 +144,27 @@ SIGNATURES = [
     ("glp_selftest_recip", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(ctypes.c_uint32)]),
+    ("glp_init_haps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint64,
+                                     ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]),
+    ("glp_replace", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+                                   ctypes.c_void_p]),
+    ("glp_stbin_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                                      ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("glp_stbin_dims", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p,
+                                      ctypes.POINTER(ctypes.c_size_t)]),
+    ("glp_stbin_name", ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_uint]),
+    ("glp_stbin_site", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint,
+                                      ctypes.POINTER(ctypes.c_char_p), c_u32p,
+                                      ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(ctypes.c_char_p)]),
+    ("glp_stbin_gls", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f32p]),
+    ("glp_stbin_free", None, [ctypes.c_void_p]),
+    ("glp_tran_from_positions", ctypes.c_int, [c_u32p, ctypes.c_uint, ctypes.c_uint,
+                                               ctypes.c_double, ctypes.c_double, c_f32p]),
+    ("glp_write_vcf", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint,
+      ctypes.POINTER(ctypes.c_char_p), c_u32p, ctypes.POINTER(ctypes.c_char_p),
+      ctypes.POINTER(ctypes.c_char_p), c_u32p, ctypes.c_uint, ctypes.c_char_p]),
     ("glp_model_params", ctypes.c_int,
      [ctypes.c_uint, ctypes.c_uint, c_f64p, ctypes.c_float, c_f32p, c_f32p, c_f32p]),
     ("glp_vq_train", ctypes.c_int,

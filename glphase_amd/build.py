@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglphase_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["glp_kernels.hip", "glp_work.hip", "glp_knn.hip", "glp_vq.hip", "glp_capi.cpp",
-           "glp_host.cpp"]
+SOURCES = ["glp_kernels.hip", "glp_work.hip", "glp_knn.hip", "glp_vq.hip", "glp_gen.hip",
+           "glp_capi.cpp", "glp_host.cpp", "glp_io.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -36,7 +36,7 @@ def build(verbose=False):
         _run(cmd, verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"], verbose)
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

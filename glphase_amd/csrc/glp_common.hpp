@@ -49,9 +49,12 @@ GLP_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 
 // Counter layout: {k, sample, window, generation}; key = 64-bit seed.
 // k < 2^31 is MH proposal k; kInitTag draws the 4 fresh parents of a
-// generation; kWorkTag | (site >> 2) draws hmm_work's per-site uniforms.
+// generation; kWorkTag | (site >> 2) draws hmm_work's per-site uniforms;
+// kHapInitTag, with the site in place of the window and the window in place
+// of the generation, draws a sample's initial alleles at one site.
 constexpr uint32_t kInitTag = 0xFFFFFFFFu;
 constexpr uint32_t kWorkTag = 0x80000000u;
+constexpr uint32_t kHapInitTag = 0xFFFFFFFEu;
 
 GLP_HD U4 draw(uint64_t seed, uint32_t k, uint32_t sample, uint32_t window,
                uint32_t gen) {

@@ -107,6 +107,7 @@ class Job:
             d.trace_acc = self.trace_acc.data_ptr()
         self.desc = d
         self.plan = None
+        self.extra_plans = []
 
     # -- execution ----------------------------------------------------------
     def stream_handle(self):
@@ -138,10 +139,29 @@ class Job:
         check(lib().glp_plan_lanes(self.plan, ctypes.byref(q), ctypes.byref(g)), "glp_plan_lanes")
         return q.value, g.value
 
-    def run_batched(self, gen=0, pen=1.0, seed=0):
+    def plan_with(self, prop_mode, lanes=0):
+        """A second plan over the same device buffers with another proposal
+        mode (e.g. uniform candidates before K-lists exist); released by close()."""
+        d = GlpMhDesc.from_buffer_copy(self.desc)
+        d.prop_mode = prop_mode
+        p = ctypes.c_void_p()
+        check(lib().glp_plan_create(ctypes.byref(d), lanes, ctypes.byref(p)), "glp_plan_create")
+        self.extra_plans.append(p)
+        return p
+
+    def set_klist(self, klist):
+        """Overwrite the K-lists in place ([N][K] for one window, or
+        [W][N][K]; a device tensor or a host array of the shape given at
+        construction)."""
+        src = klist if hasattr(klist, "data_ptr") else _dev(self.torch, np.asarray(klist, np.uint32),
+                                                           self.device)
+        self.klist_d.copy_(src.reshape(self.klist_d.shape))
+
+    def run_batched(self, gen=0, pen=1.0, seed=0, plan=None):
         if self.plan is None:
             self.make_plan()
-        check(lib().glp_plan_run(self.plan, gen, pen, seed, self.stream_handle()), "glp_plan_run")
+        check(lib().glp_plan_run(plan or self.plan, gen, pen, seed, self.stream_handle()),
+              "glp_plan_run")
 
     def run_plain(self, gen=0, pen=1.0, seed=0):
         if self.plan is None:
@@ -198,6 +218,9 @@ class Job:
         if self.plan is not None:
             lib().glp_plan_destroy(self.plan)
             self.plan = None
+        for p in getattr(self, "extra_plans", []):
+            lib().glp_plan_destroy(p)
+        self.extra_plans = []
 
     def __del__(self):
         try:

@@ -63,3 +63,58 @@ def pack_nonvq(gls, s0=0, s1=None):
     out = np.zeros(((M + 3) // 4, s1 - s0), np.uint32)
     check(lib().glp_pack_nonvq(ptr(g, c_f32p), N, M, s0, s1, ptr(out, c_u32p)), "glp_pack_nonvq")
     return out
+
+
+def read_stbin(path, region=None, size_limit=0, keep_names=None, drop_first=True):
+    """A SNPTools .bin file through glp_stbin_read: dict(names, chrom, pos,
+    ref, alt, gls, not_read); gls is [M][N][2] (Het, AA) / sum with
+    drop_first, else [M][N][3] (RR, Het, AA)."""
+    L = lib()
+    h = ctypes.c_void_p()
+    enc = (lambda s: None if s is None else str(s).encode())
+    check(L.glp_stbin_read(enc(path), enc(region), size_limit, enc(keep_names), ctypes.byref(h)),
+          "glp_stbin_read")
+    try:
+        n, m, nr = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_size_t()
+        check(L.glp_stbin_dims(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nr)),
+              "glp_stbin_dims")
+        N, M = n.value, m.value
+        names = [L.glp_stbin_name(h, i).decode() for i in range(N)]
+        chrom, pos, ref, alt = [], np.zeros(M, np.uint32), [], []
+        c, r, a, p = ctypes.c_char_p(), ctypes.c_char_p(), ctypes.c_char_p(), ctypes.c_uint32()
+        for j in range(M):
+            check(L.glp_stbin_site(h, j, ctypes.byref(c), ctypes.byref(p), ctypes.byref(r),
+                                   ctypes.byref(a)), "glp_stbin_site")
+            chrom.append(c.value.decode())
+            pos[j] = p.value
+            ref.append(r.value.decode())
+            alt.append(a.value.decode())
+        k = 2 if drop_first else 3
+        gls = np.zeros((M, N, k), np.float32)
+        if M and N:
+            check(L.glp_stbin_gls(h, int(drop_first), ptr(gls, c_f32p)), "glp_stbin_gls")
+        return dict(names=names, chrom=chrom, pos=pos, ref=ref, alt=alt, gls=gls,
+                    not_read=nr.value)
+    finally:
+        L.glp_stbin_free(h)
+
+
+def tran_from_positions(pos, n_haps, density=1.0, inflation=1.0):
+    """tran [M][3] from site positions (no genetic map), as Insti::initialize."""
+    p = np.ascontiguousarray(pos, dtype=np.uint32)
+    out = np.zeros((p.shape[0], 3), np.float32)
+    check(lib().glp_tran_from_positions(ptr(p, c_u32p), p.shape[0], n_haps, density, inflation,
+                                        ptr(out, c_f32p)), "glp_tran_from_positions")
+    return out
+
+
+def write_vcf(path, names, chrom, pos, ref, alt, hsum, sampling_gens, command_line=""):
+    """Impute::result + Insti::save_vcf from allele tallies hsum [2N][M] (uint32)."""
+    h = np.ascontiguousarray(hsum, dtype=np.uint32)
+    N, M = len(names), len(pos)
+    assert h.shape == (2 * N, M)
+    strs = lambda xs: (ctypes.c_char_p * len(xs))(*[str(x).encode() for x in xs])
+    p = np.ascontiguousarray(pos, dtype=np.uint32)
+    check(lib().glp_write_vcf(str(path).encode(), N, strs(names), M, strs(chrom), ptr(p, c_u32p),
+                              strs(ref), strs(alt), ptr(h, c_u32p), sampling_gens,
+                              command_line.encode()), "glp_write_vcf")

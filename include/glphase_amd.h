@@ -260,6 +260,71 @@ size_t glp_work_workspace_bytes(const glp_work_desc *desc);
  * full (bits past num_sites are 0).  Stream-ordered. */
 int glp_hmm_work(const glp_work_desc *desc, void *stream);
 
+/* ----- generation bookkeeping (Insti::initialize, Impute::replace) -------- */
+
+/* Insti::initialize's random initial haplotypes (insti.cpp:1316-1365) of one
+ * window: for sample i and site m, t = (max(1 - Het - AA, 0), max(Het, 0),
+ * max(AA, 0)) from het_aa [M][N][2] (site-major GL pairs); the three
+ * gsl_rng_uniform draws become words 0, 1, 2 of Philox block
+ * {0xFFFFFFFE, i, m, window} under `seed` (doubles of [0, 1), 32 bits):
+ * u0 < t0 gives 0|0; else u1 < t1 / (t1 + t2) a heterozygote, the
+ * alternative allele on hap 2i when u2 < 0.5, else on hap 2i + 1; else 1|1.
+ * Writes rows [0, 2N) of haps [.][ceil(M / 64)] in full (bits past M are 0).
+ * The male chromosome-X branch is not supported.  Device pointers,
+ * stream-ordered. */
+int glp_init_haps(const float *het_aa, unsigned num_samples, unsigned num_sites, uint64_t seed,
+                  unsigned window, uint64_t *haps, void *stream);
+
+/* Impute::replace (impute.cpp:564-597) for every sample of one window: the
+ * running tallies hsum [2N][M] (u32; rows 2i, 2i + 1 are sample i's ha, hb)
+ * count haps rows 2i, 2i + 1 in the orientation that agrees more with them.
+ * Call once per sampling generation after the haplotypes are updated; the
+ * allele probabilities of the output are hsum / sampling generations
+ * (Impute::result, impute.cpp:599-613).  Device pointers, stream-ordered. */
+int glp_replace(const uint64_t *haps, unsigned num_samples, unsigned num_sites, uint32_t *hsum,
+                void *stream);
+
+/* ----- data formats (GLReader, Insti::save_vcf) -------------------------- */
+
+typedef struct glp_stbin glp_stbin;
+
+/* GLReader's SNPTools .bin path (glReader.cpp:25-90, 201-285): a gzip or
+ * plain tab-separated file, header "chr pos allele name...", one line per
+ * site with "Het AA" per sample.  Keeps the sites inside `region`
+ * ("chrom:start-end", either bound may be empty; NULL or "" = all), at most
+ * size_limit of them (0 = no limit; the rest are counted as not read), and
+ * the samples named in the file keep_names_path (one name a line; NULL = all).
+ * Errors as the reference: wrong column count, unsorted positions, a second
+ * chromosome, a repeated site, an unparsable region. */
+int glp_stbin_read(const char *path, const char *region, size_t size_limit,
+                   const char *keep_names_path, glp_stbin **out);
+int glp_stbin_dims(const glp_stbin *b, unsigned *num_samples, unsigned *num_sites,
+                   size_t *num_not_read);
+const char *glp_stbin_name(const glp_stbin *b, unsigned i);
+int glp_stbin_site(const glp_stbin *b, unsigned m, const char **chrom, unsigned *pos,
+                   const char **ref, const char **alt);
+/* GLs site-major: drop_first = 0 gives [M][N][3] (max(0, 1 - Het - AA), Het,
+ * AA) (gl_ret_t::STANDARD); drop_first = 1 gives [M][N][2] (Het, AA) / sum
+ * (gl_ret_t::ST_DROP_FIRST, the layout Insti and glp_hmm_work use). */
+int glp_stbin_gls(const glp_stbin *b, int drop_first, float *out);
+void glp_stbin_free(glp_stbin *b);
+
+/* Insti::initialize's transitions without a genetic map (insti.cpp:1253-1300):
+ * mu from the harmonic number of hn haplotypes, rho = 0.5 mu (M - 1) /
+ * (pos[M-1] - pos[0]) / density, r = d rho / (d rho + hn) for the distance d
+ * to the previous site, times inflation, capped at 1.  tran_out [M][3]. */
+int glp_tran_from_positions(const unsigned *pos, unsigned num_sites, unsigned hn,
+                            double density, double inflation, float *tran_out);
+
+/* Impute::result + Insti::save_vcf (impute.cpp:599-613, insti.cpp:2098-2167):
+ * VCFv4.2 with GT:GP:APP per sample from the allele tallies hsum (host,
+ * [2N][M], glp_replace) over sampling_gens generations; gzip when path ends
+ * in ".gz".  The ##source line names this library. */
+int glp_write_vcf(const char *path, unsigned num_samples, const char *const *names,
+                  unsigned num_sites, const char *const *chrom, const unsigned *pos,
+                  const char *const *ref, const char *const *alt, const uint32_t *hsum,
+                  unsigned sampling_gens, const char *command_line);
+
 /* ----- kNN candidate lists (KNN, kNN.cpp:8-225) --------------------------- */
 
 /* Neighbour lists of samples 0 .. num_samples-1 (haplotypes 2u, 2u+1 of the

@@ -79,6 +79,11 @@ def L():
         lib.orc_hmm_work.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
                                      ctypes.c_float, ctypes.c_uint, ctypes.c_uint64,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p]
+        lib.orc_tran_from_positions.argtypes = [u32p, ctypes.c_uint, ctypes.c_uint,
+                                                ctypes.c_double, ctypes.c_double, f32p]
+        lib.orc_init_haps.argtypes = [f32p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint64,
+                                      ctypes.c_uint32, u64p]
+        lib.orc_replace.argtypes = [u64p, ctypes.c_uint, ctypes.c_uint, u32p]
         _L = lib
     return _L
 
@@ -260,6 +265,30 @@ def tran_from_morgans(r, inflation=1.0):
     out = np.zeros((rr.shape[0], 3), np.float32)
     L().orc_tran_from_morgans(P(rr, f64p), rr.shape[0], inflation, P(out, f32p))
     return out
+
+
+def tran_from_positions(pos, hn, density=1.0, inflation=1.0):
+    p = np.ascontiguousarray(pos, np.uint32)
+    out = np.zeros((p.shape[0], 3), np.float32)
+    L().orc_tran_from_positions(P(p, u32p), p.shape[0], hn, density, inflation, P(out, f32p))
+    return out
+
+
+def init_haps(het_aa, seed, window=0):
+    """Initial haplotypes [2N][wn] of het_aa [M][N][2]."""
+    g = np.ascontiguousarray(het_aa, np.float32)
+    M, N, _ = g.shape
+    out = np.zeros((2 * N, (M + 63) // 64), np.uint64)
+    L().orc_init_haps(P(g, f32p), N, M, seed, window, P(out, u64p))
+    return out
+
+
+def replace(haps, N, M, hsum):
+    """Impute::replace for samples [0, N) on hsum [2N][M] (updated in place)."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    assert hsum.dtype == np.uint32 and hsum.flags["C_CONTIGUOUS"] and hsum.shape == (2 * N, M)
+    L().orc_replace(P(h, u64p), N, M, P(hsum, u32p))
+    return hsum
 
 
 def hmm_work(haps, parents, emit, tran, prob, pc_, S, seed, sample, window, gen):

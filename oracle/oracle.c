@@ -629,6 +629,98 @@ void orc_tran_from_morgans(const double *r_morgans, unsigned M,
   }
 }
 
+/* insti.cpp:1253-1300 without a genetic map: rho from the SNP density of the
+ * window's positions, r = (pos_m - pos_{m-1}) rho / (that + hn), times the
+ * inflation factor (a double, insti.hpp:86), capped at 1; tran[0..2] unused */
+void orc_tran_from_positions(const unsigned *pos, unsigned M, unsigned hn, double density,
+                             double inflation, float *tran) {
+  float mu = 0;
+  for (unsigned i = 1; i < hn; i++)
+    mu += 1.0 / i;
+  mu = 1 / mu;
+  const size_t firstPos = pos[0], lastPos = pos[M - 1];
+  const float rho = lastPos == firstPos ? 0.0f
+                                        : 0.5 * mu * (M - 1) / (lastPos - firstPos) / density;
+  tran[0] = tran[1] = tran[2] = 0;
+  for (unsigned m = M - 1; m; m--) {
+    const float tmp = (pos[m] - pos[m - 1]) * rho;
+    float r = tmp / (tmp + hn);
+    r *= inflation;
+    r = r > 1 ? 1 : r;
+    tran[m * 3] = (1 - r) * (1 - r);
+    tran[m * 3 + 1] = r * (1 - r);
+    tran[m * 3 + 2] = r * r;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initial haplotypes: Insti::initialize (insti.cpp:1316-1365) for one window */
+/* of het_aa [M][N][2] GL pairs; the three gsl_rng_uniform calls of (i, m)    */
+/* become words 0, 1, 2 of Philox block {0xFFFFFFFE, i, m, window}.  haps is */
+/* [2N][wn], zeroed here.                                                     */
+/* ------------------------------------------------------------------------ */
+static inline void set1(uint64_t *h, unsigned s) { h[s >> 6] |= 1ull << (s & 63); }
+
+void orc_init_haps(const float *het_aa, unsigned N, unsigned M, uint64_t seed,
+                   uint32_t window, uint64_t *haps) {
+  const unsigned wn = (M + 63) / 64, hn = 2 * N;
+  memset(haps, 0, sizeof(uint64_t) * (size_t)hn * wn);
+  for (unsigned i = 0; i < N; i++) {
+    uint64_t *ha = &haps[(size_t)i * 2 * wn], *hb = ha + wn;
+    const float *p = &het_aa[i * 2];
+    for (unsigned m = 0; m < M; m++, p += hn) {
+      float t[3];
+      const float r = 1 - p[0] - p[1];
+      t[0] = r < 0.0f ? 0.0f : r; /* max(1 - p[0] - p[1], 0.0f) */
+      t[1] = p[0] < 0.0f ? 0.0f : p[0];
+      t[2] = p[1] < 0.0f ? 0.0f : p[1];
+      uint32_t x[4];
+      orc_draw(seed, 0xFFFFFFFEu, i, m, window, x);
+      if (orc_u01(x[0]) < t[0])
+        ;
+      else if (orc_u01(x[1]) < t[1] / (t[1] + t[2])) {
+        if (orc_u01(x[2]) < 0.5)
+          set1(ha, m);
+        else
+          set1(hb, m);
+      } else {
+        set1(ha, m);
+        set1(hb, m);
+      }
+    }
+  }
+}
+
+/* Impute::replace (impute.cpp:564-597) for samples [0, N): hsum [2N][M]. */
+void orc_replace(const uint64_t *haps, unsigned N, unsigned M, uint32_t *hsum) {
+  const unsigned wn = (M + 63) / 64;
+  for (unsigned I = 0; I < N; I++) {
+    const uint64_t *oa = &haps[(size_t)I * 2 * wn], *ob = oa + wn;
+    uint32_t *ha = &hsum[(size_t)I * 2 * M], *hb = ha + M;
+    unsigned sis = 0, tra = 0;
+    for (unsigned m = 0; m < M; m++) {
+      if (hbit(oa, m)) {
+        sis += ha[m];
+        tra += hb[m];
+      }
+      if (hbit(ob, m)) {
+        sis += hb[m];
+        tra += ha[m];
+      }
+    }
+    if (sis > tra)
+      for (unsigned m = 0; m < M; m++) {
+        ha[m] += hbit(oa, m);
+        hb[m] += hbit(ob, m);
+      }
+    else
+      for (unsigned m = 0; m < M; m++) {
+        ha[m] += hbit(ob, m);
+        hb[m] += hbit(oa, m);
+      }
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* hmm_work: Impute::hmm_work (impute.cpp:380-485).  The per-site uniform of   */
 /* gsl_rng_uniform becomes Philox counter {0x80000000 | m>>2, sample, window, */

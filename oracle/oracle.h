@@ -126,6 +126,13 @@ float orc_mu(unsigned hn);
 void orc_pc(float mu, float *pc);
 void orc_tran_from_morgans(const double *r_morgans, unsigned M,
                            float inflation, float *tran);
+void orc_tran_from_positions(const unsigned *pos, unsigned M, unsigned hn, double density,
+                             double inflation, float *tran);
+
+/* ---- generation bookkeeping: initial haplotypes, allele tallies ---------- */
+void orc_init_haps(const float *het_aa, unsigned N, unsigned M, uint64_t seed,
+                   uint32_t window, uint64_t *haps);
+void orc_replace(const uint64_t *haps, unsigned N, unsigned M, uint32_t *hsum);
 
 /* ---- hmm_work: forward-backward haplotype sampling ----------------------- */
 void orc_hmm_work_weights(const uint64_t *haps, unsigned wn, const unsigned P[4],
