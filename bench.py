@@ -291,6 +291,24 @@ def main():
     wms = wsecs / args.steps * 1e3
     log(f"hmm_work: {wms:.3f} ms/step")
 
+    # a whole sampling generation as Insti::estimate runs it: MH update,
+    # hmm_work, the new haplotypes into the panel, Impute::replace's tallies
+    from glphase_amd._lib import check as lib_check, lib as glp_lib
+    hsum = torch.zeros((2 * N, M), dtype=torch.int32, device="cuda:0")
+    fgen = [0]
+
+    def fstep():
+        job.run_batched(gen=fgen[0], pen=pen, seed=seed)
+        rs.run(pen=pen, seed=seed, gen=fgen[0])
+        rs.adopt()
+        lib_check(glp_lib().glp_replace(job.keep[0].data_ptr(), N, M, hsum.data_ptr(),
+                                        job.stream_handle()), "glp_replace")
+        fgen[0] += 1
+
+    fsecs, _ = time_steps(torch, fstep, args.steps, 1)
+    fms = fsecs / args.steps * 1e3
+    log(f"full generation (MH + hmm_work + adopt + replace): {fms:.3f} ms/step")
+
     # plain path on the same inputs: one launch per MH iteration
     job_p = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED)
     job_p.make_plan(1)
@@ -339,7 +357,7 @@ def main():
         "plain": {"value": pvalue, "ms_per_step": pms, "launches_per_step": C + 1,
                   "batched_over_plain": value / pvalue, "graph_ms_per_step": pgms},
         "graph": {"batched_ms_per_step": gms, "plain_ms_per_step": pgms},
-        "hmm_work": {"ms_per_step": wms, "generation_ms": ms + wms,
+        "hmm_work": {"ms_per_step": wms, "generation_ms": ms + wms, "full_generation_ms": fms,
                      "note": "Impute::hmm_work for all samples after the MH update"},
         "speculation": {"chain_passes": evals, "useful_passes": useful},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
