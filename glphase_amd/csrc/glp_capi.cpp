@@ -122,25 +122,30 @@ static void plan_graph_reset(glp_plan *p) {
   p->graph_kind = 0;
 }
 
-// Passes per chain round when none is given (each pass is a lane pair): one
-// MH round decides ~6 proposals at the ~15% acceptance of sampling
-// generations, so depth 8 wastes little; deeper only when there are too few
-// chains to fill the device.
-// a wavefront holds 32 lane pairs
+// Passes per chain round (lane pairs per chain) when none is given: the
+// deepest speculation whose wavefronts still fit one per SIMD.  A chain's
+// time is its rounds times one pass, and rounds fall with depth while the
+// SIMDs are not shared; past one wavefront per SIMD two wavefronts split the
+// issue slots and every pass slows down.
+// a wavefront holds 32 lane pairs, floor(32 / Q) chain slots
 constexpr unsigned kMaxPasses = 32;
 
+static double waves_for(unsigned nchains, unsigned Q) {
+  return std::ceil((double)nchains / (double)(32u / Q));
+}
+
 static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu) {
-  const double target = (double)num_cu * 4.0 * 32.0;  // one wave per SIMD
-  unsigned Q = 8;
-  while (Q < kMaxPasses && (double)nchains * Q < target) Q *= 2;
-  while (Q > 1 && Q / 2 >= C + 1) Q /= 2;
-  return Q;
+  const double target = (double)num_cu * 4.0;  // one wave per SIMD
+  unsigned best = 1;
+  for (unsigned Q = 1; Q <= kMaxPasses && Q <= C + 1; ++Q)
+    if (waves_for(nchains, Q) <= target) best = Q;
+  return best;
 }
 
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
-  double lanes = 0;
-  for (unsigned w : wins) lanes += 2.0 * p->a.N * p->q[w];
-  const unsigned blocks = (unsigned)std::ceil(lanes / 256.0);
+  double waves = 0;
+  for (unsigned w : wins) waves += waves_for(p->a.N, p->q[w]);
+  const unsigned blocks = (unsigned)std::ceil(waves / 4.0);
   return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
 }
 
@@ -255,9 +260,8 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   *out = nullptr;
   KArgs a;
   GLP_TRY(fill_args(desc, a));
-  if (lanes_per_chain != 0 &&
-      (lanes_per_chain > kMaxPasses || (lanes_per_chain & (lanes_per_chain - 1)) != 0))
-    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 or a power of two <= 32");
+  if (lanes_per_chain > kMaxPasses)
+    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 (automatic) or 1..32");
   glp_plan *p = new glp_plan();
   p->a = a;
   int rc = hip_check(hipGetDevice(&p->dev), "hipGetDevice");
@@ -315,8 +319,7 @@ int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
   if (!p || !lanes_per_window) return set_error(GLP_ERR_ARG, "null argument");
   for (unsigned w = 0; w < p->nwin; ++w) {
     const unsigned Q = lanes_per_window[w];
-    if (Q == 0 || Q > kMaxPasses || (Q & (Q - 1)) != 0)
-      return set_error(GLP_ERR_ARG, "lanes per chain must be a power of two <= 32");
+    if (Q == 0 || Q > kMaxPasses) return set_error(GLP_ERR_ARG, "lanes per chain must be 1..32");
   }
   p->q.assign(lanes_per_window, lanes_per_window + p->nwin);
   plan_graph_reset(p);
@@ -374,8 +377,9 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
     if (rc != GLP_OK) break;
     unsigned best = p->q[b[0]];
     float best_ms = 1e30f;
-    for (unsigned Q = 1; Q <= kMaxPasses && rc == GLP_OK; Q *= 2) {
-      if (Q > 1 && Q / 2 >= a.C + 1) break;
+    static const unsigned kTrials[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 32};
+    for (unsigned Q : kTrials) {
+      if (rc != GLP_OK || Q > a.C + 1) break;
       for (unsigned w : b) p->q[w] = Q;
       rc = plan_upload_q(p);
       const unsigned grid = plan_grid(p, b);

@@ -448,8 +448,9 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Wavefronts walk the windows in `order` (longest first).  While on a window
 // a wavefront keeps its data (sites, tran, panel, codes) wave-uniform and
-// holds 32/Q chain slots of Q lane pairs (Q = qtab[window], a power of two
-// <= 32); a lane pair scores one parent set per round.
+// holds floor(32/Q) chain slots of Q lane pairs (Q = qtab[window], 1..32;
+// the 32 mod Q pairs past the last slot stay idle); a lane pair scores one
+// parent set per round.
 // Idle slots take the window's next samples from its task counter; when the
 // window runs dry and every slot is idle the wavefront moves on.
 //
@@ -478,6 +479,7 @@ __global__ void __launch_bounds__(256)
   // the slot's first lane
   int wo = -1;  // position in `order`
   unsigned bw = 0, Q = 1, j = 0, sbase = lane & ~1u;
+  bool inslot = true;  // false for the pairs past the last whole slot (32 % Q of them)
   glp_window W{};
   // slot state, replicated over the slot's lanes
   bool active = false, fresh = false;
@@ -497,7 +499,7 @@ __global__ void __launch_bounds__(256)
     }
     // 1. idle slots take the next samples of the current window
     if (wo >= 0) {
-      const bool want = lane == sbase && !active;
+      const bool want = inslot && lane == sbase && !active;
       const uint64_t wm = __ballot(want);
       if (wm) {
         unsigned base = 0;
@@ -520,8 +522,12 @@ __global__ void __launch_bounds__(256)
       if (++wo >= (int)nwin) break;
       bw = __builtin_amdgcn_readfirstlane(order[wo]);
       Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
-      j = (lane >> 1) & (Q - 1u);
-      sbase = (lane & ~1u) - 2u * j;
+      {
+        const unsigned pair = lane >> 1, slot = pair / Q;
+        j = pair - slot * Q;
+        sbase = 2u * Q * slot;
+        inslot = slot < 32u / Q;
+      }
       W = wins[bw];
       continue;
     }
@@ -560,7 +566,8 @@ __global__ void __launch_bounds__(256)
     unsigned changed = 0, decided = 0;
     bool stop = !active;
     for (int jj = 0; jj < (int)Q; ++jj) {
-      const unsigned src = sbase + 2u * (unsigned)jj;
+      // pairs outside a whole slot are never active; keep their reads in the wave
+      const unsigned src = (sbase + 2u * (unsigned)jj) & 63u;
       const double vj = __shfl(v, src, 64);
       const unsigned rpj = __shfl(rp, src, 64);
       const unsigned cj = __shfl(cand, src, 64);

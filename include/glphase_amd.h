@@ -172,8 +172,8 @@ int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
  * lanes_per_chain forward passes (each on a pair of lanes) evaluate
  * consecutive proposals of one chain speculatively from the same state, and
  * a sequential walk keeps exactly the serial MH result.  lanes_per_chain is
- * one of 1,2,4,8,16,32 for every window (0 = a default from the chain count;
- * see glp_plan_autotune). */
+ * 1..32 for every window (a wavefront holds floor(32 / lanes) chains; 0 = the
+ * deepest value whose wavefronts fit one per SIMD; see glp_plan_autotune). */
 typedef struct glp_plan glp_plan;
 int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain,
                     glp_plan **out);
@@ -187,7 +187,7 @@ int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
 /* Lanes per chain of the longest window and grid size the plan launches
  * with. */
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
-/* Sets passes per chain round per window (powers of two <= 32). */
+/* Sets passes per chain round per window (1..32). */
 int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
 /* Autotuner: for each window-size bucket (ceil(log2 sites)) runs one
  * generation per candidate lanes-per-chain (1..32, at most the proposals + 1

@@ -95,7 +95,7 @@ def test_plain_path_matches_oracle(orc, prec):
     assert (job2.parents() == job.parents()).all()
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32])
+@pytest.mark.parametrize("lanes", [1, 2, 3, 4, 5, 6, 7, 8, 11, 16, 32])
 def test_batched_matches_oracle(orc, lanes):
     say(f"batched lanes={lanes}")
     p = helpers.synthetic(200, 500, seed=lanes)
@@ -109,7 +109,7 @@ def test_batched_matches_oracle(orc, lanes):
     assert ev >= useful
 
 
-@pytest.mark.parametrize("lanes", [4, 16])
+@pytest.mark.parametrize("lanes", [4, 6, 16])
 def test_batched_mixed_precision(orc, lanes):
     say(f"batched mixed lanes={lanes}")
     p = helpers.synthetic(150, 900, seed=7, depth=2.0)
@@ -140,7 +140,7 @@ def test_autotune_keeps_parity_and_inputs(orc):
               init_parents=init, extra=extra, trace=True)
     job.make_plan(0)
     lanes = job.autotune(gen=1, pen=0.5, seed=3)
-    assert all(q in (1, 2, 4, 8, 16, 32) for q in lanes)
+    assert all(1 <= q <= 32 for q in lanes)
     assert (job.parents() == init).all()  # host-mode inputs restored
     job.run_batched(gen=1, pen=0.5, seed=3)
     par = job.parents()
