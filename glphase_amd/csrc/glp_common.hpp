@@ -13,6 +13,45 @@
 
 #define GLP_HD __host__ __device__ __forceinline__
 
+#if defined(__HIPCC__)
+namespace glp {
+// A device-memory pointer in the global address space.  Loads through it are
+// global loads, counted by vmcnt alone; through a generic pointer (as read
+// from a descriptor struct) they are flat loads, which also count against
+// lgkmcnt, so every wait for an LDS read would wait for them as well.
+template <class T>
+using gptr_t = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr_t<T> gmem(const T *p) {
+  return (gptr_t<T>)p;
+}
+template <class T>
+using gmut_t = __attribute__((address_space(1))) T *;
+
+// 2 and 4 floats at one aligned global address, as one load or store (the
+// HIP vector types cannot bind global-address-space references; the
+// compiler's vector types can)
+typedef float glp_f2v __attribute__((ext_vector_type(2)));
+typedef float glp_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 load_f2(gptr_t<float> p) {
+  const glp_f2v v = *reinterpret_cast<gptr_t<glp_f2v>>(p);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 load_f4(gptr_t<float> p) {
+  const glp_f4v v = *reinterpret_cast<gptr_t<glp_f4v>>(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_f4(float *p, float a, float b, float c, float d) {
+  glp_f4v v;
+  v.x = a;
+  v.y = b;
+  v.z = c;
+  v.w = d;
+  *(gmut_t<glp_f4v>)p = v;  // a C cast: the one that may change address space
+}
+}  // namespace glp
+#endif
+
 namespace glp {
 
 // Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).

@@ -259,12 +259,15 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
 // wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
 // LDS buffer wtr, one chunk ahead.
 template <int BITS, bool MIXED>
-__device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
-                               const uint32_t *__restrict__ m0, const uint32_t *__restrict__ m1,
-                               const uint32_t *__restrict__ codes, size_t stride,
-                               const float *__restrict__ tran, unsigned M,
+__device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p, const uint32_t *__restrict__ f1p,
+                               const uint32_t *__restrict__ m0p, const uint32_t *__restrict__ m1p,
+                               const uint32_t *__restrict__ codesp, size_t stride,
+                               const float *__restrict__ tranp, unsigned M,
                                const unsigned char *smem, unsigned lut_entries, float *wtr,
                                float *ring, float norm) {
+  const auto f0 = gmem(f0p), f1 = gmem(f1p), m0 = gmem(m0p), m1 = gmem(m1p);
+  const auto codes = gmem(codesp);
+  const auto tran = gmem(tranp);
   constexpr int NW = BITS;  // code words per 32 sites
   constexpr int CPW = 32 / BITS;
   const uint2 *spread = reinterpret_cast<const uint2 *>(smem + 16u * lut_entries);
@@ -287,7 +290,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, 
   // 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step gives
   // l_xy = 1/4 * e_xy exactly
   float2 tp = make_float2(0.0f, 0.0f);
-  if (lane < 48u && 2u * lane + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + 2u * lane);
+  if (lane < 48u && 2u * lane + 1u < ntr) tp = load_f2(tran + 2u * lane);
   if (lane == 0u) tp = make_float2(1.0f, 0.0f);
   if (lane == 1u) tp.x = 0.0f;
   for (unsigned ch = 0; ch < nch; ++ch) {
@@ -297,7 +300,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0, 
     {
       const unsigned o = kTranFloats * nx + 2u * lane;
       tp = make_float2(0.0f, 0.0f);
-      if (lane < 48u && o + 1u < ntr) tp = *reinterpret_cast<const float2 *>(tran + o);
+      if (lane < 48u && o + 1u < ntr) tp = load_f2(tran + o);
     }
     uint32_t hn[4] = {0, 0, 0, 0}, cn[NW];
     if (nx < nch) {

@@ -12,6 +12,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 
 #include "../../include/glphase_amd.h"
@@ -56,84 +57,100 @@ __device__ __forceinline__ unsigned work_choose(float c00, float c01, float c10,
   return 3u;
 }
 
-// One lane per (window, sample).  beta: this window's workspace slice,
-// [M][N] float4 (site-major, so a wavefront's stores coalesce).  GL pairs
-// and beta are loaded a block of sites ahead of their use.  POW: S != 1
-// (powf is the identity at S = 1).
-template <bool POW>
-__global__ void __launch_bounds__(256)
-    k_hmm_work(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
-               float4 *__restrict__ ws, WorkArgs a) {
-  __shared__ float4 s_pc[4];  // rows pc[s][0..3]
+__device__ __forceinline__ void load_pc_rows(float4 *s_pc, const WorkArgs &a) {
   if (threadIdx.x < 4)
     s_pc[threadIdx.x] = make_float4(a.pc[4 * threadIdx.x], a.pc[4 * threadIdx.x + 1],
                                     a.pc[4 * threadIdx.x + 2], a.pc[4 * threadIdx.x + 3]);
   __syncthreads();
+}
+
+// One block of the recursions: sites 8 blk .. 8 blk + 7 (aligned, so a block
+// lies inside one 64-site allele word), their GL pairs and transitions.
+struct WorkBlock {
+  float2 q[kWorkBlock];
+  float t[3 * kWorkBlock];
+};
+
+// Loads block blk's inputs for sites in [lo, hi] (others zero).
+__device__ __forceinline__ void load_block(WorkBlock &b, int blk, int lo, int hi,
+                                           gptr_t<float> q2, gptr_t<float> tran, unsigned N,
+                                           unsigned i) {
+#pragma unroll
+  for (int k = 0; k < kWorkBlock; ++k) {
+    const int m = kWorkBlock * blk + k;
+    const bool in = blk >= 0 && m >= lo && m <= hi;
+    b.q[k] = in ? load_f2(q2 + 2 * ((size_t)m * N + i)) : make_float2(0, 0);
+    b.t[3 * k] = in ? tran[3 * m] : 0.0f;
+    b.t[3 * k + 1] = in ? tran[3 * m + 1] : 0.0f;
+    b.t[3 * k + 2] = in ? tran[3 * m + 2] : 0.0f;
+  }
+}
+
+// Emissions e00, e01, e10, e11 of site m (alleles from the block's word).
+__device__ __forceinline__ float4 site_emit(const float4 *s_pc, const uint64_t (&hw)[4],
+                                            unsigned m, float2 q) {
+  const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
+                 m1 = hbit64(hw[3], m);
+  const float3 t = work_prob(q);
+  return make_float4(work_emit(s_pc[(f0 << 1) | m0], t), work_emit(s_pc[(f0 << 1) | m1], t),
+                     work_emit(s_pc[(f1 << 1) | m0], t), work_emit(s_pc[(f1 << 1) | m1], t));
+}
+
+// The two recursions of impute.cpp:398-470, one lane per (window, sample):
+// the scaled backward pass writes beta to the workspace ([M][N] float4,
+// site-major, so a wavefront's stores coalesce), then the scaled forward
+// pass replaces each site's beta with the state posteriors
+// (l00 b00, l01 b01, l10 b10, l11 b11) that the sampling step weighs.  Sites
+// go in aligned blocks of 8 whose GL pairs, transitions (and, forward, beta)
+// are loaded a block ahead; a block's emissions are computed before its
+// recursion steps, off their dependency chain.  Parent indices past the
+// panel are clamped (the MH update never produces them).
+__global__ void __launch_bounds__(256)
+    k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
+                    float4 *__restrict__ ws, WorkArgs a) {
+  __shared__ float4 s_pc[4];  // rows pc[s][0..3]
+  load_pc_rows(s_pc, a);
   const unsigned w = blockIdx.y;
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const glp_work_window W = wins[w];
   const unsigned M = W.num_sites, wn = W.words_per_hap, N = a.N;
-  float4 *beta = ws + beta_off[w];
-  const float2 *q2 = reinterpret_cast<const float2 *>(W.het_aa);
-  // parent rows; indices past the panel are clamped (the MH update never
-  // produces them)
-  const uint64_t *rows[4];
+  float *beta = reinterpret_cast<float *>(ws + beta_off[w]);  // [M][N] x 4
+  const auto gbeta = gmem(beta);
+  const auto q2 = gmem(W.het_aa), tran = gmem(W.tran);
+  gptr_t<uint64_t> rows[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    unsigned h = W.parents[4 * (size_t)i + q];
+    unsigned h = gmem(W.parents)[4 * (size_t)i + q];
     h = h < a.H ? h : a.H - 1;
-    rows[q] = W.panel + (size_t)h * wn;
+    rows[q] = gmem(W.panel) + (size_t)h * wn;
   }
   constexpr int B = kWorkBlock;
+  const int top_blk = (int)(M - 1) / B;
 
-  // backward recursion (impute.cpp:398-416), sites M-1 .. 1 in blocks
+  // backward recursion (impute.cpp:398-416): site m = M-1 .. 1 turns beta_m
+  // into beta_{m-1}
   float b0 = 1, b1 = 1, b2 = 1, b3 = 1;
-  beta[(size_t)(M - 1) * N + i] = make_float4(1, 1, 1, 1);
-  // allele words of the current 64-site word and (prefetched) the next one
-  // in the walk's direction
-  unsigned wcur = (M - 1) >> 6;
-  uint64_t hw[4], hwn[4];
+  store_f4(beta + 4 * ((size_t)(M - 1) * N + i), 1, 1, 1, 1);
+  uint64_t hw[4];
+  WorkBlock cur, nxt;
+  load_block(nxt, top_blk, 1, (int)M - 1, q2, tran, N, i);
+  for (int blk = top_blk; blk >= 0; --blk) {
+    cur = nxt;
+    load_block(nxt, blk - 1, 1, (int)M - 1, q2, tran, N, i);
+    const unsigned word = (unsigned)(B * blk) >> 6;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    hw[q] = rows[q][wcur];
-    hwn[q] = wcur > 0 ? rows[q][wcur - 1] : 0ull;
-  }
-  float2 qn[B];
+    for (int q = 0; q < 4; ++q) hw[q] = rows[q][word];
+    float4 e[B];
 #pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const int m = (int)M - 1 - k;
-    qn[k] = m >= 1 ? q2[(size_t)m * N + i] : make_float2(0, 0);
-  }
-  for (int top = (int)M - 1; top >= 1; top -= B) {
-    float2 qc[B];
+    for (int k = 0; k < B; ++k) e[k] = site_emit(s_pc, hw, (unsigned)(B * blk + k), cur.q[k]);
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      qc[k] = qn[k];
-      const int m = top - B - k;
-      qn[k] = m >= 1 ? q2[(size_t)m * N + i] : make_float2(0, 0);
-    }
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const int mi = top - k;
-      if (mi < 1) break;
+    for (int k = B - 1; k >= 0; --k) {
+      const int mi = B * blk + k;
+      if (mi > (int)M - 1 || mi < 1) continue;
       const unsigned m = (unsigned)mi;
-      if ((m >> 6) != wcur) {
-        wcur = m >> 6;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          hw[q] = hwn[q];
-          hwn[q] = wcur > 0 ? rows[q][wcur - 1] : 0ull;
-        }
-      }
-      const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
-                     m1 = hbit64(hw[3], m);
-      const float3 t = work_prob(qc[k]);
-      const float e00 = work_emit(s_pc[(f0 << 1) | m0], t), e01 = work_emit(s_pc[(f0 << 1) | m1], t);
-      const float e10 = work_emit(s_pc[(f1 << 1) | m0], t), e11 = work_emit(s_pc[(f1 << 1) | m1], t);
-      const float bb00 = b0 * e00, bb01 = b1 * e01, bb10 = b2 * e10, bb11 = b3 * e11;
-      const float *tr = W.tran + 3 * (size_t)m;
-      const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
+      const float bb00 = b0 * e[k].x, bb01 = b1 * e[k].y, bb10 = b2 * e[k].z, bb11 = b3 * e[k].w;
+      const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
       const float n0 = bb00 * t0 + (bb01 + bb10) * t1 + bb11 * t2;
       const float n1 = bb01 * t0 + (bb00 + bb11) * t1 + bb10 * t2;
       const float n2 = bb10 * t0 + (bb00 + bb11) * t1 + bb01 * t2;
@@ -143,74 +160,53 @@ __global__ void __launch_bounds__(256)
       b1 = n1 * sum;
       b2 = n2 * sum;
       b3 = n3 * sum;
-      beta[(size_t)(m - 1) * N + i] = make_float4(b0, b1, b2, b3);
+      store_f4(beta + 4 * ((size_t)(m - 1) * N + i), b0, b1, b2, b3);
     }
   }
 
-  // forward pass with sampling (impute.cpp:418-484), sites 0 .. M-1 in blocks
-  uint64_t *ha = W.hnew + (size_t)(2 * i) * wn, *hb = ha + wn;
+  // forward recursion (impute.cpp:418-455), sites 0 .. M-1; each site's beta
+  // is read (a block ahead) before its slot takes the posterior
   float l0 = 0, l1 = 0, l2 = 0, l3 = 0;
-  uint64_t wa = 0, wb = 0;
-  U4 x{0, 0, 0, 0};
-  const float S = a.pen;
   float4 bn[B];
+  load_block(nxt, 0, 0, (int)M - 1, q2, tran, N, i);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    hw[q] = rows[q][0];
-    hwn[q] = wn > 1 ? rows[q][1] : 0ull;
-  }
-#pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const unsigned m = (unsigned)k;
-    qn[k] = m < M ? q2[(size_t)m * N + i] : make_float2(0, 0);
-    bn[k] = m < M ? beta[(size_t)m * N + i] : make_float4(0, 0, 0, 0);
-  }
-  for (unsigned base = 0; base < M; base += B) {
-    float2 qc[B];
+  for (int k = 0; k < B; ++k)
+    bn[k] = (unsigned)k < M ? load_f4(gbeta + 4 * ((size_t)k * N + i)) : make_float4(0, 0, 0, 0);
+  for (int blk = 0; blk <= top_blk; ++blk) {
+    cur = nxt;
     float4 bc[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      qc[k] = qn[k];
       bc[k] = bn[k];
-      const unsigned m = base + B + (unsigned)k;
-      qn[k] = m < M ? q2[(size_t)m * N + i] : make_float2(0, 0);
-      bn[k] = m < M ? beta[(size_t)m * N + i] : make_float4(0, 0, 0, 0);
+      const unsigned m = (unsigned)(B * (blk + 1) + k);
+      bn[k] = m < M ? load_f4(gbeta + 4 * ((size_t)m * N + i)) : make_float4(0, 0, 0, 0);
     }
+    load_block(nxt, blk + 1, 0, (int)M - 1, q2, tran, N, i);
+    const unsigned word = (unsigned)(B * blk) >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) hw[q] = rows[q][word];
+    float4 e[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) e[k] = site_emit(s_pc, hw, (unsigned)(B * blk + k), cur.q[k]);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const unsigned m = base + (unsigned)k;
+      const unsigned m = (unsigned)(B * blk + k);
       if (m >= M) break;
-      if ((m & 63u) == 0 && m) {
-        const unsigned wx = (m >> 6) + 1;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          hw[q] = hwn[q];
-          hwn[q] = wx < wn ? rows[q][wx] : 0ull;
-        }
-      }
-      const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
-                     m1 = hbit64(hw[3], m);
-      const float4 r00 = s_pc[(f0 << 1) | m0], r01 = s_pc[(f0 << 1) | m1],
-                   r10 = s_pc[(f1 << 1) | m0], r11 = s_pc[(f1 << 1) | m1];
-      const float3 t = work_prob(qc[k]);
-      const float e00 = work_emit(r00, t), e01 = work_emit(r01, t), e10 = work_emit(r10, t),
-                  e11 = work_emit(r11, t);
       if (m) {
-        const float *tr = W.tran + 3 * (size_t)m;
-        const float t0 = tr[0], t1 = tr[1], t2 = tr[2];
+        const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
         const float n0 = l0 * t0 + l1 * t1 + l2 * t1 + l3 * t2;
         const float n1 = l0 * t1 + l1 * t0 + l2 * t2 + l3 * t1;
         const float n2 = l0 * t1 + l1 * t2 + l2 * t0 + l3 * t1;
         const float n3 = l0 * t2 + l1 * t1 + l2 * t1 + l3 * t0;
-        l0 = n0 * e00;
-        l1 = n1 * e01;
-        l2 = n2 * e10;
-        l3 = n3 * e11;
+        l0 = n0 * e[k].x;
+        l1 = n1 * e[k].y;
+        l2 = n2 * e[k].z;
+        l3 = n3 * e[k].w;
       } else {
-        l0 = 0.25f * e00;
-        l1 = 0.25f * e01;
-        l2 = 0.25f * e10;
-        l3 = 0.25f * e11;
+        l0 = 0.25f * e[k].x;
+        l1 = 0.25f * e[k].y;
+        l2 = 0.25f * e[k].z;
+        l3 = 0.25f * e[k].w;
       }
       const float sum = 1.0f / (l0 + l1 + l2 + l3);
       l0 *= sum;
@@ -218,29 +214,68 @@ __global__ void __launch_bounds__(256)
       l2 *= sum;
       l3 *= sum;
       const float4 b = bc[k];
-      const float p00 = l0 * b.x, p01 = l1 * b.y, p10 = l2 * b.z, p11 = l3 * b.w;
-      float c00 = t.x * (p00 * r00.x + p01 * r01.x + p10 * r10.x + p11 * r11.x);
-      float c01 = t.y * (p00 * r00.y + p01 * r01.y + p10 * r10.y + p11 * r11.y);
-      float c10 = t.y * (p00 * r00.z + p01 * r01.z + p10 * r10.z + p11 * r11.z);
-      float c11 = t.z * (p00 * r00.w + p01 * r01.w + p10 * r10.w + p11 * r11.w);
-      if (POW) {
-        c00 = (float)pow((double)c00, (double)S);
-        c01 = (float)pow((double)c01, (double)S);
-        c10 = (float)pow((double)c10, (double)S);
-        c11 = (float)pow((double)c11, (double)S);
-      }
-      if ((m & 3u) == 0) x = draw(a.seed, kWorkTag | (m >> 2), i, w, a.gen);
-      const uint32_t xm = (m & 3u) == 0 ? x.x : (m & 3u) == 1 ? x.y : (m & 3u) == 2 ? x.z : x.w;
-      const unsigned g = work_choose(c00, c01, c10, c11, xm);
-      wa |= (uint64_t)(g >> 1) << (m & 63u);
-      wb |= (uint64_t)(g & 1u) << (m & 63u);
-      if ((m & 63u) == 63u || m + 1 == M) {
-        ha[m >> 6] = wa;
-        hb[m >> 6] = wb;
-        wa = wb = 0;
-      }
+      store_f4(beta + 4 * ((size_t)m * N + i), l0 * b.x, l1 * b.y, l2 * b.z, l3 * b.w);
     }
   }
+}
+
+// The sampling of impute.cpp:457-484, one lane per (window, sample, 64-site
+// word), sample fastest so a wavefront's reads of a site coalesce: from the
+// posteriors p (written by k_hmm_work_pass) and the GL pair, the genotype
+// weights c_g = (t_g * sum_s p_s pc[s][g])^S, one Philox word per site, the
+// genotype choice, and the two new haplotype words.  POW: S != 1 (powf is
+// the identity at S = 1).
+template <bool POW>
+__global__ void __launch_bounds__(256)
+    k_hmm_work_sample(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
+                      const float4 *__restrict__ ws, WorkArgs a) {
+  __shared__ float4 s_pc[4];
+  load_pc_rows(s_pc, a);
+  const unsigned w = blockIdx.y;
+  const glp_work_window W = wins[w];
+  const unsigned M = W.num_sites, wn = W.words_per_hap, N = a.N;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * wn) return;
+  const unsigned i = (unsigned)(t % N), wd = (unsigned)(t / N);
+  const auto post = gmem(reinterpret_cast<const float *>(ws + beta_off[w]));
+  const auto q2 = gmem(W.het_aa);
+  uint64_t hw[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    unsigned h = gmem(W.parents)[4 * (size_t)i + q];
+    h = h < a.H ? h : a.H - 1;
+    hw[q] = gmem(W.panel)[(size_t)h * wn + wd];
+  }
+  const float S = a.pen;
+  const unsigned m1 = 64u * wd + 64u < M ? 64u * wd + 64u : M;
+  uint64_t wa = 0, wb = 0;
+  U4 x{0, 0, 0, 0};
+  for (unsigned m = 64u * wd; m < m1; ++m) {
+    const float4 p = load_f4(post + 4 * ((size_t)m * N + i));
+    const float3 tq = work_prob(load_f2(q2 + 2 * ((size_t)m * N + i)));
+    const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
+                   m1b = hbit64(hw[3], m);
+    const float4 r00 = s_pc[(f0 << 1) | m0], r01 = s_pc[(f0 << 1) | m1b],
+                 r10 = s_pc[(f1 << 1) | m0], r11 = s_pc[(f1 << 1) | m1b];
+    float c00 = tq.x * (p.x * r00.x + p.y * r01.x + p.z * r10.x + p.w * r11.x);
+    float c01 = tq.y * (p.x * r00.y + p.y * r01.y + p.z * r10.y + p.w * r11.y);
+    float c10 = tq.y * (p.x * r00.z + p.y * r01.z + p.z * r10.z + p.w * r11.z);
+    float c11 = tq.z * (p.x * r00.w + p.y * r01.w + p.z * r10.w + p.w * r11.w);
+    if (POW) {
+      c00 = (float)pow((double)c00, (double)S);
+      c01 = (float)pow((double)c01, (double)S);
+      c10 = (float)pow((double)c10, (double)S);
+      c11 = (float)pow((double)c11, (double)S);
+    }
+    if ((m & 3u) == 0) x = draw(a.seed, kWorkTag | (m >> 2), i, w, a.gen);
+    const uint32_t xm = (m & 3u) == 0 ? x.x : (m & 3u) == 1 ? x.y : (m & 3u) == 2 ? x.z : x.w;
+    const unsigned g = work_choose(c00, c01, c10, c11, xm);
+    wa |= (uint64_t)(g >> 1) << (m & 63u);
+    wb |= (uint64_t)(g & 1u) << (m & 63u);
+  }
+  const auto hnew = (gmut_t<uint64_t>)W.hnew;
+  hnew[(size_t)(2 * i) * wn + wd] = wa;
+  hnew[(size_t)(2 * i + 1) * wn + wd] = wb;
 }
 
 }  // namespace glp
@@ -305,15 +340,21 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
     a.seed = d->seed;
     a.gen = d->gen;
     for (int k = 0; k < 16; ++k) a.pc[k] = d->pc16[k];
+    unsigned max_wn = 0;
+    for (unsigned w = 0; w < d->num_windows; ++w)
+      max_wn = std::max(max_wn, d->windows[w].words_per_hap);
     const dim3 grid((d->num_samples + 255) / 256, d->num_windows), block(256);
+    const size_t words = (size_t)d->num_samples * max_wn;
+    const dim3 grid2((unsigned)((words + 255) / 256), d->num_windows);
     const auto *wt = reinterpret_cast<const glp_work_window *>(dev);
     const auto *wo = reinterpret_cast<const size_t *>(dev + tbl);
     float4 *ws = static_cast<float4 *>(d->workspace);
+    hipLaunchKernelGGL(k_hmm_work_pass, grid, block, 0, s, wt, wo, ws, a);
     if (d->pen == 1.0f)
-      hipLaunchKernelGGL((k_hmm_work<false>), grid, block, 0, s, wt, wo, ws, a);
+      hipLaunchKernelGGL((k_hmm_work_sample<false>), grid2, block, 0, s, wt, wo, ws, a);
     else
-      hipLaunchKernelGGL((k_hmm_work<true>), grid, block, 0, s, wt, wo, ws, a);
-    rc = hip_check(hipGetLastError(), "k_hmm_work launch");
+      hipLaunchKernelGGL((k_hmm_work_sample<true>), grid2, block, 0, s, wt, wo, ws, a);
+    rc = hip_check(hipGetLastError(), "hmm_work launch");
   }
   (void)hipFreeAsync(dev, s);
   return rc;
