@@ -17,18 +17,19 @@
 #include <cstdint>
 
 #include "../../include/glphase_amd.h"
+#include "glp_common.hpp"
 #include "glp_internal.hpp"
 
 namespace glp {
 
 constexpr unsigned kKnnBatch = 4096;     // query haplotypes per distance batch
-constexpr unsigned kKnnCountChunk = 1024;  // haplotypes per allele-count block
+constexpr unsigned kKnnCountChunk = 128;  // haplotypes per allele-count block
 constexpr unsigned kKnnMaxHalf = 1024;   // ceil(K / 2) limit
 constexpr unsigned kKnnQpb = 32;         // queries per distance block
 constexpr unsigned kKnnBins = 2560;      // distance histogram bins per row
 constexpr unsigned kKnnMaxWords = 1024;  // 65,536 sites
 
-// Alt-allele counts of the 64 sites of word w over haplotypes [h0, h0 + 1024).
+// Alt-allele counts of the 64 sites of word w over haplotypes [h0, h0 + kKnnCountChunk).
 __global__ void __launch_bounds__(64)
     k_knn_count(const uint64_t *__restrict__ panel, unsigned H, unsigned wn,
                 unsigned *__restrict__ counts) {
@@ -36,7 +37,9 @@ __global__ void __launch_bounds__(64)
   const unsigned h0 = blockIdx.y * kKnnCountChunk;
   const unsigned h1 = h0 + kKnnCountChunk < H ? h0 + kKnnCountChunk : H;
   unsigned alt = 0;
-  for (unsigned h = h0; h < h1; ++h) alt += (unsigned)(panel[(size_t)h * wn + w] >> lane) & 1u;
+  const auto p = gmem(panel);
+#pragma unroll 8
+  for (unsigned h = h0; h < h1; ++h) alt += (unsigned)(p[(size_t)h * wn + w] >> lane) & 1u;
   atomicAdd(&counts[64 * w + lane], alt);
 }
 
