@@ -145,18 +145,27 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
                                 unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
                                 uint64_t *out) {
   const unsigned lane = threadIdx.x & 63u;
+  // distances 2p, 2p + 1 as one 32-bit word (H is even, so rows are 4-byte
+  // aligned); pair u is the sample's own two haplotypes, which never count
+  const auto row2 = gmem(reinterpret_cast<const uint32_t *>(dr));
+  const unsigned np = H >> 1;
   for (unsigned b = lane; b < kKnnBins; b += 64) hist[b] = 0;
   wave_lds_sync();
-  for (unsigned base = 0; base < H; base += 256) {
-    unsigned v[4];
+  for (unsigned base = 0; base < np; base += 256) {
+    uint32_t v[4];
+    bool ok[4];
 #pragma unroll
     for (unsigned j = 0; j < 4; ++j) {
-      const unsigned h = base + 64 * j + lane;
-      v[j] = h < H && (h >> 1) != u ? dr[h] : 0xFFFFFFFFu;
+      const unsigned p = base + 64 * j + lane;
+      ok[j] = p < np && p != u;
+      v[j] = ok[j] ? row2[p] : 0u;
     }
 #pragma unroll
     for (unsigned j = 0; j < 4; ++j)
-      if (v[j] != 0xFFFFFFFFu) atomicAdd(&hist[v[j] >> shift], 1u);
+      if (ok[j]) {
+        atomicAdd(&hist[(v[j] & 0xFFFFu) >> shift], 1u);
+        atomicAdd(&hist[(v[j] >> 16) >> shift], 1u);
+      }
   }
   wave_lds_sync();
   unsigned below;
@@ -166,16 +175,22 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
     hist[lane] = 0;
     wave_lds_sync();
     const unsigned lo_mask = (1u << shift) - 1;
-    for (unsigned base = 0; base < H; base += 256) {
-      unsigned v[4];
+    for (unsigned base = 0; base < np; base += 256) {
+      uint32_t v[4];
+      bool ok[4];
 #pragma unroll
       for (unsigned j = 0; j < 4; ++j) {
-        const unsigned h = base + 64 * j + lane;
-        v[j] = h < H && (h >> 1) != u ? dr[h] : 0xFFFFFFFFu;
+        const unsigned p = base + 64 * j + lane;
+        ok[j] = p < np && p != u;
+        v[j] = ok[j] ? row2[p] : 0u;
       }
 #pragma unroll
       for (unsigned j = 0; j < 4; ++j)
-        if (v[j] != 0xFFFFFFFFu && (v[j] >> shift) == t) atomicAdd(&hist[v[j] & lo_mask], 1u);
+        if (ok[j]) {
+          const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
+          if ((d0 >> shift) == t) atomicAdd(&hist[d0 & lo_mask], 1u);
+          if ((d1 >> shift) == t) atomicAdd(&hist[d1 & lo_mask], 1u);
+        }
     }
     wave_lds_sync();
     unsigned below_lo;
@@ -186,20 +201,35 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
   const unsigned need = k - below;  // ties at t to take, lowest indices first
   const uint64_t lt = lanes_below();
   unsigned nless = 0, nties = 0;
-  for (unsigned base = 0; base < H && (nless < below || nties < need); base += 64) {
-    const unsigned h = base + lane;
-    const bool ok = h < H && (h >> 1) != u;
-    const unsigned dv = ok ? dr[h] : 0xFFFFFFFFu;
-    const bool less = dv < t, eq = dv == t;
-    const uint64_t bl = __ballot(less), be = __ballot(eq);
-    const uint64_t key = ((uint64_t)dv << 32) | h;
-    if (less) tmp[nless + (unsigned)__popcll(bl & lt)] = key;
-    if (eq) {
-      const unsigned r = nties + (unsigned)__popcll(be & lt);
-      if (r < need) tmp[below + r] = key;
+  // index order: 64-pair chunks ascending, lanes ascending, element 2p
+  // before 2p + 1; four chunks' loads are in flight together
+  for (unsigned base = 0; base < np && (nless < below || nties < need); base += 256) {
+    uint32_t v[4];
+    bool ok[4];
+#pragma unroll
+    for (unsigned j = 0; j < 4; ++j) {
+      const unsigned p = base + 64 * j + lane;
+      ok[j] = p < np && p != u;
+      v[j] = ok[j] ? row2[p] : 0u;
     }
-    nless += (unsigned)__popcll(bl);
-    nties += (unsigned)__popcll(be);
+#pragma unroll
+    for (unsigned j = 0; j < 4; ++j) {
+      const unsigned p = base + 64 * j + lane;
+      const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
+      const bool l0 = ok[j] && d0 < t, l1 = ok[j] && d1 < t;
+      const bool e0 = ok[j] && d0 == t, e1 = ok[j] && d1 == t;
+      const uint64_t bl0 = __ballot(l0), bl1 = __ballot(l1);
+      const uint64_t be0 = __ballot(e0), be1 = __ballot(e1);
+      const unsigned pl = nless + (unsigned)__popcll(bl0 & lt) + (unsigned)__popcll(bl1 & lt);
+      if (l0) tmp[pl] = ((uint64_t)d0 << 32) | (2u * p);
+      if (l1) tmp[pl + (l0 ? 1u : 0u)] = ((uint64_t)d1 << 32) | (2u * p + 1u);
+      const unsigned pe = nties + (unsigned)__popcll(be0 & lt) + (unsigned)__popcll(be1 & lt);
+      if (e0 && pe < need) tmp[below + pe] = ((uint64_t)d0 << 32) | (2u * p);
+      const unsigned pe1 = pe + (e0 ? 1u : 0u);
+      if (e1 && pe1 < need) tmp[below + pe1] = ((uint64_t)d1 << 32) | (2u * p + 1u);
+      nless += (unsigned)(__popcll(bl0) + __popcll(bl1));
+      nties += (unsigned)(__popcll(be0) + __popcll(be1));
+    }
   }
   wave_lds_sync();
   for (unsigned i = lane; i < k; i += 64) {
