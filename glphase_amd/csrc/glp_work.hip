@@ -86,135 +86,145 @@ __device__ __forceinline__ void load_block(WorkBlock &b, int blk, int lo, int hi
   }
 }
 
-// Emissions e00, e01, e10, e11 of site m (alleles from the block's word).
-__device__ __forceinline__ float4 site_emit(const float4 *s_pc, const uint64_t (&hw)[4],
-                                            unsigned m, float2 q) {
-  const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
-                 m1 = hbit64(hw[3], m);
-  const float3 t = work_prob(q);
-  return make_float4(work_emit(s_pc[(f0 << 1) | m0], t), work_emit(s_pc[(f0 << 1) | m1], t),
-                     work_emit(s_pc[(f1 << 1) | m0], t), work_emit(s_pc[(f1 << 1) | m1], t));
+// quad_perm DPP moves within each group of 4 lanes (one sample's 4 states,
+// lane s = state s = 2 x + y for parents f_x, m_y)
+template <int CTRL>
+__device__ __forceinline__ float qperm(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+constexpr int kQSwap1 = 0xB1;  // [1,0,3,2]
+constexpr int kQSwap2 = 0x4E;  // [2,3,0,1]
+constexpr int kQOpp = 0x1B;    // [3,2,1,0]: state 3 - s
+constexpr int kQ0 = 0x00, kQ1 = 0x55, kQ2 = 0xAA, kQ3 = 0xFF;  // broadcasts of lane 0..3
+
+// ((x0 + x1) + x2) + x3 of the quad's four values, in every lane: the
+// reference's order of a 4-state sum.
+__device__ __forceinline__ float quad_sum(float x) {
+  float s = qperm<kQ0>(x) + qperm<kQ1>(x);
+  s = s + qperm<kQ2>(x);
+  return s + qperm<kQ3>(x);
 }
 
-// The two recursions of impute.cpp:398-470, one lane per (window, sample):
-// the scaled backward pass writes beta to the workspace ([M][N] float4,
-// site-major, so a wavefront's stores coalesce), then the scaled forward
-// pass replaces each site's beta with the state posteriors
-// (l00 b00, l01 b01, l10 b10, l11 b11) that the sampling step weighs.  Sites
-// go in aligned blocks of 8 whose GL pairs, transitions (and, forward, beta)
-// are loaded a block ahead; a block's emissions are computed before its
-// recursion steps, off their dependency chain.  Parent indices past the
-// panel are clamped (the MH update never produces them).
+// The two recursions of impute.cpp:398-470 with one lane per (window,
+// sample, state): lane s of a quad holds state s's value, the cross-state
+// terms come by quad DPP moves in the reference's operand order, and the
+// four lanes compute each site's normaliser together.  The scaled backward
+// pass writes beta to the workspace ([M][N] float4, site-major: a quad
+// stores one float4, a wavefront 16 samples' contiguous float4s); the
+// scaled forward pass then stores each site's state posteriors
+// (l00 b00, l01 b01, l10 b10, l11 b11) over its beta.  Sites go in aligned
+// blocks of 8 whose GL pairs, transitions and (forward) beta are loaded a
+// block ahead.  Parent indices past the panel are clamped (the MH update
+// never produces them).
 __global__ void __launch_bounds__(256)
     k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
                     float4 *__restrict__ ws, WorkArgs a) {
   __shared__ float4 s_pc[4];  // rows pc[s][0..3]
   load_pc_rows(s_pc, a);
   const unsigned w = blockIdx.y;
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned i = g >> 2, st = g & 3u;
+  if (i >= a.N) return;  // whole quads leave together
   const glp_work_window W = wins[w];
   const unsigned M = W.num_sites, wn = W.words_per_hap, N = a.N;
   float *beta = reinterpret_cast<float *>(ws + beta_off[w]);  // [M][N] x 4
   const auto gbeta = gmem(beta);
   const auto q2 = gmem(W.het_aa), tran = gmem(W.tran);
-  gptr_t<uint64_t> rows[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    unsigned h = gmem(W.parents)[4 * (size_t)i + q];
-    h = h < a.H ? h : a.H - 1;
-    rows[q] = gmem(W.panel) + (size_t)h * wn;
+  // this state's two parents: f_x (slot x) and m_y (slot 2 + y)
+  gptr_t<uint64_t> rx, ry;
+  {
+    unsigned hx = gmem(W.parents)[4 * (size_t)i + (st >> 1)];
+    unsigned hy = gmem(W.parents)[4 * (size_t)i + 2 + (st & 1u)];
+    hx = hx < a.H ? hx : a.H - 1;
+    hy = hy < a.H ? hy : a.H - 1;
+    rx = gmem(W.panel) + (size_t)hx * wn;
+    ry = gmem(W.panel) + (size_t)hy * wn;
   }
+  // forward coefficient of state j's value in this state's update:
+  // t[popcount(st ^ j)] (impute.cpp:435-445)
+  const bool own[4] = {st == 0, st == 1, st == 2, st == 3};
+  const bool opp[4] = {st == 3, st == 2, st == 1, st == 0};
   constexpr int B = kWorkBlock;
   const int top_blk = (int)(M - 1) / B;
 
   // backward recursion (impute.cpp:398-416): site m = M-1 .. 1 turns beta_m
   // into beta_{m-1}
-  float b0 = 1, b1 = 1, b2 = 1, b3 = 1;
-  store_f4(beta + 4 * ((size_t)(M - 1) * N + i), 1, 1, 1, 1);
-  uint64_t hw[4];
+  float b = 1;
+  beta[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
   WorkBlock cur, nxt;
   load_block(nxt, top_blk, 1, (int)M - 1, q2, tran, N, i);
   for (int blk = top_blk; blk >= 0; --blk) {
     cur = nxt;
     load_block(nxt, blk - 1, 1, (int)M - 1, q2, tran, N, i);
     const unsigned word = (unsigned)(B * blk) >> 6;
+    const uint64_t wx = rx[word], wy = ry[word];
+    float e[B];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) hw[q] = rows[q][word];
-    float4 e[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) e[k] = site_emit(s_pc, hw, (unsigned)(B * blk + k), cur.q[k]);
+    for (int k = 0; k < B; ++k) {
+      const unsigned m = (unsigned)(B * blk + k);
+      const unsigned row = (hbit64(wx, m) << 1) | hbit64(wy, m);
+      e[k] = work_emit(s_pc[row], work_prob(cur.q[k]));
+    }
 #pragma unroll
     for (int k = B - 1; k >= 0; --k) {
       const int mi = B * blk + k;
       if (mi > (int)M - 1 || mi < 1) continue;
-      const unsigned m = (unsigned)mi;
-      const float bb00 = b0 * e[k].x, bb01 = b1 * e[k].y, bb10 = b2 * e[k].z, bb11 = b3 * e[k].w;
       const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-      const float n0 = bb00 * t0 + (bb01 + bb10) * t1 + bb11 * t2;
-      const float n1 = bb01 * t0 + (bb00 + bb11) * t1 + bb10 * t2;
-      const float n2 = bb10 * t0 + (bb00 + bb11) * t1 + bb01 * t2;
-      const float n3 = bb11 * t0 + (bb01 + bb10) * t1 + bb00 * t2;
-      const float sum = 1.0f / (n0 + n1 + n2 + n3);
-      b0 = n0 * sum;
-      b1 = n1 * sum;
-      b2 = n2 * sum;
-      b3 = n3 * sum;
-      store_f4(beta + 4 * ((size_t)(m - 1) * N + i), b0, b1, b2, b3);
+      const float bb = b * e[k];
+      // bb_s t0 + (pair-cross sum) t1 + bb_{3-s} t2; the cross sum is
+      // bb01 + bb10 for s = 00, 11 and bb00 + bb11 for s = 01, 10
+      const float cross = qperm<kQSwap1>(bb) + qperm<kQSwap2>(bb);
+      const float n = bb * t0 + cross * t1 + qperm<kQOpp>(bb) * t2;
+      const float sum = 1.0f / quad_sum(n);
+      b = n * sum;
+      beta[4 * ((size_t)(mi - 1) * N + i) + st] = b;
     }
   }
 
   // forward recursion (impute.cpp:418-455), sites 0 .. M-1; each site's beta
   // is read (a block ahead) before its slot takes the posterior
-  float l0 = 0, l1 = 0, l2 = 0, l3 = 0;
-  float4 bn[B];
+  float l = 0;
+  float bn[B];
   load_block(nxt, 0, 0, (int)M - 1, q2, tran, N, i);
 #pragma unroll
-  for (int k = 0; k < B; ++k)
-    bn[k] = (unsigned)k < M ? load_f4(gbeta + 4 * ((size_t)k * N + i)) : make_float4(0, 0, 0, 0);
+  for (int k = 0; k < B; ++k) bn[k] = (unsigned)k < M ? gbeta[4 * ((size_t)k * N + i) + st] : 0.0f;
   for (int blk = 0; blk <= top_blk; ++blk) {
     cur = nxt;
-    float4 bc[B];
+    float bc[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
       bc[k] = bn[k];
       const unsigned m = (unsigned)(B * (blk + 1) + k);
-      bn[k] = m < M ? load_f4(gbeta + 4 * ((size_t)m * N + i)) : make_float4(0, 0, 0, 0);
+      bn[k] = m < M ? gbeta[4 * ((size_t)m * N + i) + st] : 0.0f;
     }
     load_block(nxt, blk + 1, 0, (int)M - 1, q2, tran, N, i);
     const unsigned word = (unsigned)(B * blk) >> 6;
+    const uint64_t wx = rx[word], wy = ry[word];
+    float e[B];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) hw[q] = rows[q][word];
-    float4 e[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) e[k] = site_emit(s_pc, hw, (unsigned)(B * blk + k), cur.q[k]);
+    for (int k = 0; k < B; ++k) {
+      const unsigned m = (unsigned)(B * blk + k);
+      const unsigned row = (hbit64(wx, m) << 1) | hbit64(wy, m);
+      e[k] = work_emit(s_pc[row], work_prob(cur.q[k]));
+    }
 #pragma unroll
     for (int k = 0; k < B; ++k) {
       const unsigned m = (unsigned)(B * blk + k);
       if (m >= M) break;
       if (m) {
         const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-        const float n0 = l0 * t0 + l1 * t1 + l2 * t1 + l3 * t2;
-        const float n1 = l0 * t1 + l1 * t0 + l2 * t2 + l3 * t1;
-        const float n2 = l0 * t1 + l1 * t2 + l2 * t0 + l3 * t1;
-        const float n3 = l0 * t2 + l1 * t1 + l2 * t1 + l3 * t0;
-        l0 = n0 * e[k].x;
-        l1 = n1 * e[k].y;
-        l2 = n2 * e[k].z;
-        l3 = n3 * e[k].w;
+        float c[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[j] = own[j] ? t0 : (opp[j] ? t2 : t1);
+        const float n = qperm<kQ0>(l) * c[0] + qperm<kQ1>(l) * c[1] + qperm<kQ2>(l) * c[2] +
+                        qperm<kQ3>(l) * c[3];
+        l = n * e[k];
       } else {
-        l0 = 0.25f * e[k].x;
-        l1 = 0.25f * e[k].y;
-        l2 = 0.25f * e[k].z;
-        l3 = 0.25f * e[k].w;
+        l = 0.25f * e[k];
       }
-      const float sum = 1.0f / (l0 + l1 + l2 + l3);
-      l0 *= sum;
-      l1 *= sum;
-      l2 *= sum;
-      l3 *= sum;
-      const float4 b = bc[k];
-      store_f4(beta + 4 * ((size_t)m * N + i), l0 * b.x, l1 * b.y, l2 * b.z, l3 * b.w);
+      const float sum = 1.0f / quad_sum(l);
+      l *= sum;
+      beta[4 * ((size_t)m * N + i) + st] = l * bc[k];
     }
   }
 }
@@ -343,7 +353,7 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
     unsigned max_wn = 0;
     for (unsigned w = 0; w < d->num_windows; ++w)
       max_wn = std::max(max_wn, d->windows[w].words_per_hap);
-    const dim3 grid((d->num_samples + 255) / 256, d->num_windows), block(256);
+    const dim3 grid((unsigned)((4ull * d->num_samples + 255) / 256), d->num_windows), block(256);
     const size_t words = (size_t)d->num_samples * max_wn;
     const dim3 grid2((unsigned)((words + 255) / 256), d->num_windows);
     const auto *wt = reinterpret_cast<const glp_work_window *>(dev);
