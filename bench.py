@@ -132,12 +132,99 @@ def aux_knn(torch, steps, seed=11):
     same = bool((out[:ns].cpu().numpy().view(np.uint32) == want).all())
     log(f"kNN: {ms:.3f} ms/build, {words / (ms / 1e3):.3e} word-compares/s; "
         f"oracle {ns} samples in {cpu_s:.2f}s; first {ns} rows identical: {same}")
+    mh = aux_reference_panel_mh(torch, d, panel, out, steps, seed)
     return {"ms_per_build": ms, "unit": "word-compares/s", "value": words / (ms / 1e3),
+            "mh_klist": mh,
             "config": {"samples": N, "panel_haplotypes": R, "sites": M, "K": K},
             "identical_to_oracle_rows": ns if same else 0,
             "cpu_baseline": {"value": 2 * ns * H * wn / cpu_s, "unit": "word-compares/s",
                              "cores": 1, "kind": "port",
                              "sample": f"oracle kNN lists of {ns} samples ({cpu_s:.2f}s)"}}
+
+
+def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64):
+    """The MH update of the reference-panel workload: 10,000 samples whose
+    candidates are their K = 200 nearest haplotypes of panel + samples
+    (lists from the kNN build), 64 proposals each, mixed precision,
+    autotuned depth.  Cost unit: the headline's site-eval."""
+    from glphase_amd import PREC_MIXED, PROP_KLIST, host
+    from glphase_amd.engine import Job
+    N, M = d["gls"].shape[0], d["gls"].shape[1]
+    tran, pc, _ = host.model_params(panel.shape[0], d["r_morgans"])
+    cb, _ = host.vq_train(d["gls"], 256, key=seed, training_size=VQ_TRAINING, device=True)
+    packed = host.pack_codes(host.vq_codes(d["gls"], cb, device=True), 8)
+    job = Job([dict(panel=panel, packed=packed, tran=tran)], N, panel.shape[0], cb, pc, bits=8,
+              num_cycles=C, prop_mode=PROP_KLIST, klist=klist.cpu().numpy().view(np.uint32)[None],
+              precision=PREC_MIXED)
+    job.make_plan(0)
+    lanes = int(job.autotune(gen=0, pen=1.0, seed=seed)[0])
+    gen = [0]
+
+    def step():
+        job.run_batched(gen=gen[0], pen=1.0, seed=seed)
+        gen[0] += 1
+
+    secs, _ = time_steps(torch, step, steps, 1)
+    ms = secs / steps * 1e3
+    se = N * (C + 1) * M
+    log(f"reference-panel MH (K-lists): {ms:.3f} ms/step, {se / (ms / 1e3):.3e} SE/s, "
+        f"lanes/chain {lanes}")
+    job.close()
+    return {"ms_per_step": ms, "value": se / (ms / 1e3), "unit": "site-evals/s",
+            "lanes_per_chain": lanes,
+            "config": {"samples": N, "panel_haplotypes": panel.shape[0] - 2 * N, "sites": M,
+                       "proposals": C, "K": int(klist.shape[1])}}
+
+
+def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32):
+    """Uneven windows: 2,500 samples over 32 windows whose site counts are
+    lognormal (median 1,024, clipped to 256..4,096), 32 proposals per sample,
+    FP32 forward and FP32 log accumulation (the reference's arithmetic); one
+    persistent launch per generation with depths autotuned per window-size
+    bucket, the same through its HIP graph, and the plain path."""
+    from glphase_amd import PREC_F32, host, synth
+    from glphase_amd.engine import Job
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.round(np.exp(rng.normal(np.log(1024.0), 0.6, W))), 256, 4096).astype(int)
+    t0 = time.time()
+    data = [synth.make_dataset(N, int(M), depth=4.0, seed=seed + k) for k, M in enumerate(sizes)]
+    allg = np.concatenate([x["gls"].reshape(-1, 3) for x in data])
+    cb, _ = host.vq_train(allg, 256, key=seed, training_size=VQ_TRAINING, device=True)
+    del allg
+    wins = []
+    pc = None
+    for k, x in enumerate(data):
+        panel = synth.pack_haps(synth.initial_haps(x["gls"], seed=seed + 100 + k))
+        tran, pc, _ = host.model_params(2 * N, x["r_morgans"])
+        packed = host.pack_codes(host.vq_codes(x["gls"], cb, device=True), 8)
+        wins.append(dict(panel=panel, packed=packed, tran=tran))
+    del data
+    log(f"batching workload: {N} samples x {W} windows, sites {sizes.min()}..{sizes.max()} "
+        f"(total {int(sizes.sum())}) in {time.time() - t0:.1f}s")
+    se = N * (C + 1) * int(sizes.sum())
+    job = Job(wins, N, 2 * N, cb, pc, bits=8, num_cycles=C, precision=PREC_F32)
+    job.make_plan(0)
+    tuned = [int(q) for q in job.autotune(gen=0, pen=1.0, seed=seed)]
+    out = {"config": {"samples": N, "windows": W, "sites_total": int(sizes.sum()),
+                      "sites_min": int(sizes.min()), "sites_median": int(np.median(sizes)),
+                      "sites_max": int(sizes.max()), "proposals": C, "precision": "fp32"},
+           "lanes_per_window": tuned, "unit": "site-evals/s"}
+    for name, fn in (("batched", job.run_batched), ("batched_graph", job.run_batched_graph),
+                     ("plain", job.run_plain)):
+        gen = [0]
+
+        def step(fn=fn):
+            fn(gen=gen[0], pen=1.0, seed=seed)
+            gen[0] += 1
+
+        n = steps if name != "plain" else max(1, steps // 2)
+        secs, _ = time_steps(torch, step, n, 1)
+        ms = secs / n * 1e3
+        out[name] = {"ms_per_step": ms, "value": se / (ms / 1e3)}
+        log(f"batching {name}: {ms:.3f} ms/step, {se / (ms / 1e3):.3e} SE/s")
+    out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    job.close()
+    return out
 
 
 def aux_vq(torch, w, steps):
@@ -368,6 +455,7 @@ def main():
     if not args.no_aux:
         out["vq"] = aux_vq(torch, w, args.steps)
         out["knn"] = aux_knn(torch, args.steps)
+        out["batching"] = aux_batching(torch, args.steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
     print(json.dumps(out), flush=True)

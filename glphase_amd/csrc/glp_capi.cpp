@@ -339,9 +339,8 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
   // trials overwrite the windows' parents and log-likelihoods (inputs under
   // GLP_PROP_HOST): keep a copy and restore it after every trial
   const size_t pbytes = 16 * (size_t)a.N, lbytes = 8 * (size_t)a.N;
-  DevBuf save, sub;
+  DevBuf save;
   GLP_TRY(save.ensure((pbytes + lbytes) * p->nwin));
-  GLP_TRY(sub.ensure(4 * p->nwin));
   auto copy_all = [&](bool to_save) -> int {
     for (unsigned w = 0; w < p->nwin; ++w) {
       char *base = save.as<char>() + (pbytes + lbytes) * w;
@@ -371,39 +370,59 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
     }
     buckets[bucket_of[lg]].push_back(w);
   }
+  // every trial runs the whole plan (all windows share the device, as in a
+  // real run): first one depth for every window, then each bucket's depth in
+  // turn with the others held (coordinate descent on the measured time)
   int rc = GLP_OK;
-  for (auto &b : buckets) {
-    rc = hip_check(hipMemcpy(sub.p, b.data(), 4 * b.size(), hipMemcpyHostToDevice), "bucket copy");
-    if (rc != GLP_OK) break;
-    unsigned best = p->q[b[0]];
-    float best_ms = 1e30f;
-    static const unsigned kTrials[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 32};
-    for (unsigned Q : kTrials) {
-      if (rc != GLP_OK || Q > a.C + 1) break;
-      for (unsigned w : b) p->q[w] = Q;
-      rc = plan_upload_q(p);
-      const unsigned grid = plan_grid(p, b);
-      float ms = 0;
-      for (int rep = 0; rep < 2 && rc == GLP_OK; ++rep) {
-        rc = hip_check(hipEventRecord(e0, s), "event record");
-        if (rc == GLP_OK)
-          rc = launch_batched(a, p->wins.as<glp_window>(), sub.as<unsigned>(), p->qtab.as<unsigned>(),
-                              (unsigned)b.size(), p->wctr.as<unsigned>(),
-                              p->ctrs.as<unsigned long long>(), grid, s);
-        if (rc == GLP_OK) rc = hip_check(hipEventRecord(e1, s), "event record");
-        if (rc == GLP_OK) rc = hip_check(hipEventSynchronize(e1), "event sync");
-        float t = 0;
-        if (rc == GLP_OK) rc = hip_check(hipEventElapsedTime(&t, e0, e1), "event time");
-        ms = rep == 0 ? t : std::min(ms, t);
-        if (rc == GLP_OK) rc = copy_all(false);
-      }
-      if (rc == GLP_OK && ms < best_ms) {
-        best_ms = ms;
-        best = Q;
+  auto trial = [&](float *ms_out) -> int {
+    GLP_TRY(plan_upload_q(p));
+    const unsigned grid = plan_grid(p, p->order);
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      GLP_TRY(hip_check(hipEventRecord(e0, s), "event record"));
+      GLP_TRY(launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
+                             p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
+                             p->ctrs.as<unsigned long long>(), grid, s));
+      GLP_TRY(hip_check(hipEventRecord(e1, s), "event record"));
+      GLP_TRY(hip_check(hipEventSynchronize(e1), "event sync"));
+      float t = 0;
+      GLP_TRY(hip_check(hipEventElapsedTime(&t, e0, e1), "event time"));
+      ms = rep == 0 ? t : std::min(ms, t);
+      GLP_TRY(copy_all(false));
+    }
+    *ms_out = ms;
+    return GLP_OK;
+  };
+  static const unsigned kTrials[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 32};
+  float best_ms = 1e30f;
+  std::vector<unsigned> best_q = p->q;
+  for (unsigned Q : kTrials) {
+    if (rc != GLP_OK || Q > a.C + 1) break;
+    std::fill(p->q.begin(), p->q.end(), Q);
+    float ms = 0;
+    rc = trial(&ms);
+    if (rc == GLP_OK && ms < best_ms) {
+      best_ms = ms;
+      best_q = p->q;
+    }
+  }
+  if (buckets.size() > 1) {
+    for (auto &b : buckets) {
+      for (unsigned Q : kTrials) {
+        if (rc != GLP_OK || Q > a.C + 1) break;
+        if (Q == best_q[b[0]]) continue;
+        p->q = best_q;
+        for (unsigned w : b) p->q[w] = Q;
+        float ms = 0;
+        rc = trial(&ms);
+        if (rc == GLP_OK && ms < best_ms) {
+          best_ms = ms;
+          best_q = p->q;
+        }
       }
     }
-    for (unsigned w : b) p->q[w] = best;
   }
+  p->q = best_q;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc == GLP_OK) rc = plan_upload_q(p);

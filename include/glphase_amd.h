@@ -189,9 +189,11 @@ int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
 /* Sets passes per chain round per window (1..32). */
 int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
-/* Autotuner: for each window-size bucket (ceil(log2 sites)) runs one
- * generation per candidate lanes-per-chain (1..32, at most the proposals + 1
- * rounded up), times it with HIP events on `stream` and keeps the fastest.
+/* Autotuner: times whole generations of the plan (every window, as a real
+ * run shares the device) with HIP events on `stream`: first one
+ * lanes-per-chain for every window from 1..8, 10, 12, 16, 32 (at most the
+ * proposals + 1), then, with several window-size buckets (ceil(log2 sites)),
+ * each bucket's value in turn with the others held; keeps the fastest.
  * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
  * trial.  lanes_out (optional) receives the choice per window.
  * Synchronises `stream`. */
