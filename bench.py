@@ -48,7 +48,8 @@ def make_workload(N, M, C, depth, seed):
     log(f"workload N={N} M={M} C={C}: generated in {time.time() - t0:.1f}s (VQ {iters} iterations)")
     return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, N=N, M=M,
                 H=panel.shape[0], het_aa=np.ascontiguousarray(d["gls"][:, :, 1:3]),
-                gls=d["gls"], codes=codes, vq_iters=iters, vq_host_s=vq_host_s, seed=seed)
+                gls=d["gls"], codes=codes, vq_iters=iters, vq_host_s=vq_host_s, seed=seed,
+                truth=d["truth"])
 
 
 def time_steps(torch, fn, steps, warmup, stream_events=True):
@@ -224,6 +225,34 @@ def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32):
         log(f"batching {name}: {ms:.3f} ms/step, {se / (ms / 1e3):.3e} SE/s")
     out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
     job.close()
+    return out
+
+
+def aux_whole_run(torch, w, C, seed=7, burnin=56, sampling=200):
+    """A whole phasing run of the MH workload's samples (glphase_amd.phase:
+    Insti::estimate's schedule): initial haplotypes from the GLs, 56
+    annealed burn-in + 200 sampling generations of MH update (C proposals),
+    hmm_work, panel update and tallies, all on the device; genotype
+    discordance of the output calls against the simulated truth."""
+    from glphase_amd.phase import PhaseRun, called_genotypes
+    het_aa = np.ascontiguousarray(w["gls"][:, :, 1:3].transpose(1, 0, 2))
+    run = PhaseRun(het_aa, tran=w["tran"], cycles=C, burnin=burnin, sampling=sampling, seed=seed,
+                   training_size=VQ_TRAINING)
+    hsum = run.run()
+    a, b = called_genotypes(hsum, run.sn)
+    truth = w["truth"]
+    tdos = (truth[0::2].astype(np.int8) + truth[1::2].astype(np.int8)).T  # [M][N]
+    disc = float(((a + b) != tdos).mean())
+    g = w["gls"]
+    gl_disc = float((np.argmax(g, axis=2).T != tdos).mean())
+    gens = burnin + sampling
+    log(f"whole run: {gens} generations in {run.gen_s:.2f}s ({run.gen_s / gens * 1e3:.2f} ms each), "
+        f"setup {run.setup_s:.2f}s; discordance {disc:.4f} (GL argmax {gl_disc:.4f})")
+    out = {"generations": gens, "burnin": burnin, "sampling": sampling, "proposals": C,
+           "seconds": run.gen_s, "ms_per_generation": run.gen_s / gens * 1e3,
+           "setup_s": run.setup_s, "genotype_discordance": disc,
+           "gl_argmax_discordance": gl_disc}
+    run.close()
     return out
 
 
@@ -456,6 +485,7 @@ def main():
         out["vq"] = aux_vq(torch, w, args.steps)
         out["knn"] = aux_knn(torch, args.steps)
         out["batching"] = aux_batching(torch, args.steps)
+        out["whole_run"] = aux_whole_run(torch, w, C)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
     print(json.dumps(out), flush=True)
