@@ -95,9 +95,15 @@ def cpu_baseline(w, C, pen, seed, budget_s=12.0, precision=1):
         gens += 1
         chains += w["N"]
     dta = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": se_per_chain * chains / dta, "unit": "site-evals/s", "cores": cores,
-        "kind": "port",
+        "kind": "port", "cpu_model": model, "host_cpus": os.cpu_count(),
         "sample": f"{gens} generation(s) of all {w['N']} chains x {C + 1} evaluations x "
                   f"{w['M']} sites ({dta:.1f}s, OpenMP over samples)",
         "single_core": {"value": se_per_chain * n1 / dt1, "cores": 1,
@@ -254,6 +260,32 @@ def aux_whole_run(torch, w, C, seed=7, burnin=56, sampling=200):
            "gl_argmax_discordance": gl_disc}
     run.close()
     return out
+
+
+def aux_plain_small(torch, steps, seed=31, N=1000, M=1024, C=16):
+    """The plain path at 1,000 samples x 1,024 sites: one launch per MH
+    iteration, one proposal per sample per launch (C launches after the
+    initial one), FP32 arithmetic.  Per-launch time and site-evals/s."""
+    from glphase_amd import PREC_F32
+    p = make_workload(N, M, C, 4.0, seed)
+    from glphase_amd.engine import Job
+    job = Job([dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])], N, p["H"],
+              p["codebook"], p["pc"], bits=8, num_cycles=C, precision=PREC_F32)
+    job.make_plan(1)
+    gen = [0]
+
+    def step():
+        job.run_plain(gen=gen[0], pen=1.0, seed=seed)
+        gen[0] += 1
+
+    secs, _ = time_steps(torch, step, steps, 1)
+    ms = secs / steps * 1e3
+    se = N * (C + 1) * M
+    log(f"plain 1000 x 1024: {ms / (C + 1):.4f} ms/launch, {se / (ms / 1e3):.3e} SE/s")
+    job.close()
+    return {"ms_per_launch": ms / (C + 1), "launches_per_step": C + 1, "value": se / (ms / 1e3),
+            "unit": "site-evals/s",
+            "config": {"samples": N, "sites": M, "proposals_per_launch": 1, "precision": "fp32"}}
 
 
 def aux_vq(torch, w, steps):
@@ -485,6 +517,7 @@ def main():
         out["vq"] = aux_vq(torch, w, args.steps)
         out["knn"] = aux_knn(torch, args.steps)
         out["batching"] = aux_batching(torch, args.steps)
+        out["plain_small"] = aux_plain_small(torch, args.steps)
         out["whole_run"] = aux_whole_run(torch, w, C)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
