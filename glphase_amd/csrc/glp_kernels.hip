@@ -160,9 +160,11 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   const float ba = la * t0 + pt + lb * t2;    // b00 / b01
   const float bb = lb * t0 + pt + la * t2;    // b11 / b10
   const float na = ba * e.x, nb = bb * e.y;
-  float s = dpp<kDppEven>(na) + dpp<kDppOdd>(na);  // l00 + l01
-  s = dpp<kDppOdd>(nb) + s;                        // + l10
-  s = dpp<kDppEven>(nb) + s;                       // + l11
+  // l00 + l01: the partner's na plus the own one (IEEE addition commutes,
+  // so both lanes hold the reference's first partial sum bit for bit)
+  float s = dpp<kDppSwap>(na) + na;
+  s = dpp<kDppOdd>(nb) + s;   // + l10
+  s = dpp<kDppEven>(nb) + s;  // + l11
   const bool r = s < nrm;
   const float sc = r ? recip_small(s) : 1.0f;
   st.la = na * sc;
@@ -177,7 +179,7 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
 
 // The pair's l00 + l01 + l10 + l11 in the reference's order.
 __device__ __forceinline__ float pair_sum(const PairState &st) {
-  float s = dpp<kDppEven>(st.la) + dpp<kDppOdd>(st.la);
+  float s = dpp<kDppSwap>(st.la) + st.la;
   s = dpp<kDppOdd>(st.lb) + s;
   return dpp<kDppEven>(st.lb) + s;
 }
