@@ -115,15 +115,27 @@ __device__ __forceinline__ float dpp(float x) {
 // 1.0f / x rounded to nearest-even, for 2^-149 <= x < 2^-20.  The operand is
 // scaled by 2^64 (exact), inverted by v_rcp_f32 plus one Newton step with
 // fused multiply-adds, and scaled back (exact, or the same overflow to inf
-// as the division).  glp_selftest_recip checks it against the IEEE division
-// for every float of that range.  For x = 0 it gives NaN where the division
-// gives inf; a zero site sum turns the probabilities into NaN (0 * inf) in
-// the reference as well, so the returned likelihood is NaN either way.
+// as the division).  For x = 0 it gives NaN where the division gives inf; a
+// zero site sum turns the probabilities into NaN (0 * inf) in the reference
+// as well, so the returned likelihood is NaN either way.
 __device__ __forceinline__ float recip_small(float x) {
   const float xs = x * 0x1p64f;
   const float r = __builtin_amdgcn_rcpf(xs);
   const float e = __builtin_fmaf(-xs, r, 1.0f);
   return __builtin_fmaf(e, r, r) * 0x1p64f;
+}
+
+// The same without the scaling, two instructions shorter: rounded to
+// nearest-even for 2^-126 <= x < 2^-20 (normal x), NaN for subnormal x and
+// 0, whose input v_rcp_f32 flushes.  A NaN factor makes the pass's
+// likelihood NaN (NaN stays NaN through the rest of the recurrence), and the
+// pass is then redone with recip_small (score_set), so the returned value is
+// recip_small's in every case.  glp_selftest_recip checks both against the
+// IEEE division for every float of their ranges.
+__device__ __forceinline__ float recip_normal(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(e, r, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +163,7 @@ constexpr double kLn2 = 0.6931471805599453;
 
 // One site (impute.cpp:354-372) for the lane's two states; e = this lane's
 // two emissions.  ring: this lane's first ring slot (stride 64 floats).
-template <bool MIXED>
+template <bool MIXED, bool FAST>
 __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t0, float t1,
                                           float t2, float nrm, float *ring) {
   const float la = st.la, lb = st.lb;
@@ -166,7 +178,7 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   s = dpp<kDppOdd>(nb) + s;   // + l10
   s = dpp<kDppEven>(nb) + s;  // + l11
   const bool r = s < nrm;
-  const float sc = r ? recip_small(s) : 1.0f;
+  const float sc = r ? (FAST ? recip_normal(s) : recip_small(s)) : 1.0f;
   st.la = na * sc;
   st.lb = nb * sc;
   if (MIXED) {
@@ -211,7 +223,7 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
 // sites in the chunk (TAIL: n < 32), nrm0 = the threshold of the chunk's
 // first site (0 at the window's site 0, which only initialises,
 // impute.cpp:339-347).
-template <int BITS, bool MIXED, bool TAIL>
+template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
                                           const unsigned char *smem, const uint2 *spread,
@@ -243,7 +255,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
     for (int k = 0; k < 8; ++k) {
       const int s = 8 * g + k;
       if (TAIL && (unsigned)s >= n) break;
-      pair_step<MIXED>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
+      pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
     }
     if (MIXED) {
@@ -255,12 +267,23 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
   }
 }
 
+// Floats o and o + 1 of tran (n floats in all), 0 past the end: when n is
+// odd (odd site count) the last float stands alone.
+__device__ __forceinline__ float2 tran_pair(gptr_t<float> tran, unsigned o, unsigned n) {
+  float2 t = make_float2(0.0f, 0.0f);
+  if (o + 1u < n)
+    t = load_f2(tran + o);
+  else if (o < n)
+    t.x = tran[o];
+  return t;
+}
+
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
 // pair m0, m1; rows as 32-bit words), computed by a lane pair; both lanes
 // return it.  Every lane of the wavefront must call it together (M and tran
 // wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
 // LDS buffer wtr, one chunk ahead.
-template <int BITS, bool MIXED>
+template <int BITS, bool MIXED, bool FAST>
 __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p, const uint32_t *__restrict__ f1p,
                                const uint32_t *__restrict__ m0p, const uint32_t *__restrict__ m1p,
                                const uint32_t *__restrict__ codesp, size_t stride,
@@ -292,7 +315,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   // 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step gives
   // l_xy = 1/4 * e_xy exactly
   float2 tp = make_float2(0.0f, 0.0f);
-  if (lane < 48u && 2u * lane + 1u < ntr) tp = load_f2(tran + 2u * lane);
+  if (lane < 48u) tp = tran_pair(tran, 2u * lane, ntr);
   if (lane == 0u) tp = make_float2(1.0f, 0.0f);
   if (lane == 1u) tp.x = 0.0f;
   for (unsigned ch = 0; ch < nch; ++ch) {
@@ -300,9 +323,8 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     // prefetch the next chunk's tran, allele and code words
     const unsigned nx = ch + 1;
     {
-      const unsigned o = kTranFloats * nx + 2u * lane;
       tp = make_float2(0.0f, 0.0f);
-      if (lane < 48u && o + 1u < ntr) tp = load_f2(tran + o);
+      if (lane < 48u) tp = tran_pair(tran, kTranFloats * nx + 2u * lane, ntr);
     }
     uint32_t hn[4] = {0, 0, 0, 0}, cn[NW];
     if (nx < nch) {
@@ -320,9 +342,11 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     const unsigned n = M - base < 32u ? M - base : 32u;
     const float nrm0 = ch == 0 ? 0.0f : norm;
     if (n == 32)
-      fwd_chunk<BITS, MIXED, false>(st, h, cw, wtr, 32, smem, spread, halfbits, norm, nrm0, ring);
+      fwd_chunk<BITS, MIXED, false, FAST>(st, h, cw, wtr, 32, smem, spread, halfbits, norm, nrm0,
+                                          ring);
     else
-      fwd_chunk<BITS, MIXED, true>(st, h, cw, wtr, n, smem, spread, halfbits, norm, nrm0, ring);
+      fwd_chunk<BITS, MIXED, true, FAST>(st, h, cw, wtr, n, smem, spread, halfbits, norm, nrm0,
+                                         ring);
     if (!MIXED) flush_ring(st, ring);
 #pragma unroll
     for (int q = 0; q < 4; ++q) h[q] = hn[q];
@@ -350,9 +374,21 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
                                                 16u * lut_entries + kSpreadBytes);
   float *wtr = tables_end + kTranFloats * wave;
   float *ring = tables_end + kTranFloats * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
-  return forward_pass<BITS, MIXED>(hap_row(W, P[0]), hap_row(W, P[1]), hap_row(W, P[2]),
-                                   hap_row(W, P[3]), W.packed + sample, N, W.tran, W.num_sites,
-                                   smem, lut_entries, wtr, ring, norm);
+  const uint32_t *f0 = hap_row(W, P[0]), *f1 = hap_row(W, P[1]), *m0 = hap_row(W, P[2]),
+                 *m1 = hap_row(W, P[3]);
+  const double v = forward_pass<BITS, MIXED, true>(f0, f1, m0, m1, W.packed + sample, N, W.tran,
+                                                   W.num_sites, smem, lut_entries, wtr, ring,
+                                                   norm);
+  // a subnormal (or zero) site sum made the short reciprocal NaN: the
+  // wavefront redoes its passes with the exact one (recip_normal is exact
+  // for normal sums, so a value that is not NaN is already the exact one)
+  if (__builtin_expect(__any(v != v), 0)) {
+    const double x = forward_pass<BITS, MIXED, false>(f0, f1, m0, m1, W.packed + sample, N,
+                                                      W.tran, W.num_sites, smem, lut_entries, wtr,
+                                                      ring, norm);
+    return v != v ? x : v;
+  }
+  return v;
 }
 
 // MHSampler::SampleHap (MHSampler.hpp:82-96): accept a better or equal
@@ -653,13 +689,18 @@ __global__ void __launch_bounds__(256)
   if (valid) out[e] = v;
 }
 
-// Every float with bit pattern in [lo, hi): recip_small against the division.
+// Every float with bit pattern in [lo, hi): recip_small bit-equal to the
+// division; recip_normal bit-equal to it for normal x and NaN for subnormal x.
 __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *bad,
                                  uint32_t *first) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t b = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; b < hi; b += stride) {
     const float x = __uint_as_float((uint32_t)b);
-    if (__float_as_uint(recip_small(x)) != __float_as_uint(1.0f / x)) {
+    const uint32_t want = __float_as_uint(1.0f / x);
+    const float fast = recip_normal(x);
+    const bool ok = __float_as_uint(recip_small(x)) == want &&
+                    (x >= 0x1p-126f ? __float_as_uint(fast) == want : __builtin_isnan(fast));
+    if (!ok) {
       atomicAdd(bad, 1ull);
       atomicMin(first, (uint32_t)b);
     }

@@ -215,9 +215,11 @@ int glp_plan_graph_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
 int glp_plan_graph_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                              void *stream);
 
-/* Self-test of the device reciprocal the rescale step uses (no counterpart
- * in the reference): compares it with the IEEE division 1.0f / x for every
- * float x with bit pattern in [lo_bits, hi_bits) on the current device.
+/* Self-test of the device reciprocals the rescale step uses (no counterpart
+ * in the reference): compares them with the IEEE division 1.0f / x for every
+ * float x with bit pattern in [lo_bits, hi_bits) on the current device (the
+ * exact one bit-equal everywhere; the short one bit-equal for normal x and
+ * NaN for subnormal x, which sends the pass to the exact one).
  * mismatches receives the count, first_bad the lowest mismatching pattern
  * (0xFFFFFFFF if none).  Synchronous. */
 int glp_selftest_recip(uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,

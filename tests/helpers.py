@@ -73,3 +73,75 @@ def check_planted(parents, expect, samples):
         for pair in (parents[s][0:2], parents[s][2:4]):
             lo, hi = sorted(int(x) for x in pair)
             assert hi == e["other"] and lo in e["pair"], (s, list(parents[s]))
+
+
+def tiny_emission_problem(N=48, M=301, seed=0, row_scale=(1.0, 1.1, 1.2, 1.3)):
+    """A problem whose site sums reach the subnormal range: every emission is
+    about 2^-42 (the mutation matrix row idx is row_scale[idx] * 2^-42.9), so
+    the sum falls by that factor at each site.  After a rescale (sum near 1)
+    two sites of GL code 0 (Het = 1, emission 2c) keep it at or above the
+    rescale threshold 2^-84, and a third site of code 2 (AA = 1, emission c)
+    takes it below 2^-126."""
+    rng = np.random.default_rng(seed)
+    bits = 4
+    H = 2 * N
+    wn = (M + 63) // 64
+    panel = rng.integers(0, np.iinfo(np.uint64).max, (H, wn), dtype=np.uint64, endpoint=True)
+    if M % 64:
+        panel[:, -1] &= np.uint64((1 << (M % 64)) - 1)
+    cb = np.zeros((16, 2), np.float32)
+    cb[0] = (0.0, 1.0)
+    cb[1] = (0.0, 0.95)
+    cb[2] = (0.0, 0.0)
+    rr = rng.random(13).astype(np.float32)
+    cb[3:, 0] = rr
+    cb[3:, 1] = (rng.random(13).astype(np.float32) * (np.float32(1) - rr)).astype(np.float32)
+    prob = np.full(16, 0.02)
+    prob[:3] = (0.5, 0.08, 0.16)
+    codes = rng.choice(16, size=(N, M), p=prob / prob.sum()).astype(np.uint8)
+    packed = host.pack_codes(codes, bits)
+    c = np.float32(2.0 ** -42.9)
+    pc = np.array([[np.float32(s) * c] * 4 for s in row_scale], np.float32)
+    r = np.full(M, 1e-4)
+    r[0] = 0.0
+    tran, _, _ = host.model_params(H, r)
+    return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, codes=codes,
+                bits=bits, N=N, M=M, H=H)
+
+
+def count_subnormal_sums(p, samples, parents, emit_of):
+    """Float32 restatement of the site-sum sequence (impute.cpp:339-372
+    order), used only to check that a test reaches subnormal sums: the
+    number of passes with a rescale whose sum is below 2^-126."""
+    f32 = np.float32
+    norm = f32(np.finfo(np.float32).tiny) ** f32(2.0 / 3.0)
+    hits = 0
+    for s, P in zip(samples, parents):
+        e = emit_of(int(s))
+        rows = [p["panel"][int(h)] for h in P]
+
+        def bit(row, m):
+            return int((int(row[m >> 6]) >> (m & 63)) & 1)
+
+        def ix(m):
+            f0, f1, m0, m1 = (bit(r_, m) for r_ in rows)
+            return (f0 << 1) | m0, (f0 << 1) | m1, (f1 << 1) | m0, (f1 << 1) | m1
+
+        i = ix(0)
+        l = [f32(0.25) * e[0][i[k]] for k in range(4)]
+        found = False
+        for m in range(1, p["M"]):
+            t0, t1, t2 = p["tran"][m]
+            l00, l01, l10, l11 = l
+            b = [l00 * t0 + (l01 + l10) * t1 + l11 * t2, l01 * t0 + (l00 + l11) * t1 + l10 * t2,
+                 l10 * t0 + (l00 + l11) * t1 + l01 * t2, l11 * t0 + (l01 + l10) * t1 + l00 * t2]
+            i = ix(m)
+            l = [f32(b[k] * e[m][i[k]]) for k in range(4)]
+            sm = f32(f32(f32(l[0] + l[1]) + l[2]) + l[3])
+            if sm < norm:
+                if sm < f32(2.0 ** -126):
+                    found = True
+                sc = f32(1) / sm
+                l = [f32(x * sc) for x in l]
+        hits += int(found)
+    return hits

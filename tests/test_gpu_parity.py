@@ -307,3 +307,90 @@ def test_lanes_above_32_rejected():
     job = job_for(p, C=4)
     with pytest.raises(GlpError):
         job.make_plan(64)
+
+
+def test_subnormal_site_sums_hmm_like(orc):
+    # emissions near 2^-42 drive site sums below 2^-126 in every pass: the
+    # short reciprocal gives NaN there and the pass is redone with the exact
+    # one, so the values still equal the oracle's bit for bit
+    from glphase_amd._lib import c_f32p, c_u32p, c_u64p, check, lib, ptr
+    say("hmm_like with subnormal site sums")
+    p = helpers.tiny_emission_problem(seed=3)
+    lut = orc.emit_lut(p["codebook"], p["pc"])
+    rng = np.random.default_rng(4)
+    n = 64
+    samples = rng.integers(0, p["N"], n).astype(np.uint32)
+    parents = rng.integers(0, p["H"], (n, 4)).astype(np.uint32)
+    emit_of = lambda s: orc.sample_emit(p["packed"], s, p["M"], p["bits"], lut)  # noqa: E731
+    assert helpers.count_subnormal_sums(p, samples[:8], parents[:8], emit_of) > 0
+    L = lib()
+    ctx = ctypes.c_void_p()
+    check(L.glp_create(0, ctypes.byref(ctx)), "create")
+    try:
+        check(L.glp_set_tran(ctx, ptr(p["tran"], c_f32p), p["tran"].size), "tran")
+        check(L.glp_set_mutation_mat(ctx, ptr(np.ascontiguousarray(p["pc"]), c_f32p)), "pc")
+        check(L.glp_set_codebook(ctx, ptr(p["codebook"], c_f32p), 16), "cb")
+        pk = np.ascontiguousarray(p["packed"])
+        check(L.glp_set_packed_gls(ctx, ptr(pk, c_u32p), pk.size, 4), "packed")
+        out = np.zeros(n, np.float32)
+        panel = np.ascontiguousarray(p["panel"])
+        check(L.glp_hmm_like(ctx, ptr(panel, c_u64p), panel.size, p["M"], p["N"], n,
+                             ptr(samples, c_u32p), ptr(parents, c_u32p), ptr(out, c_f32p)),
+              "hmm_like")
+        want = np.array([orc.hmm_like_f32(p["panel"], parents[e], emit_of(int(samples[e])),
+                                          p["tran"]) for e in range(n)], np.float32)
+        assert np.isfinite(want).all()
+        assert np.array_equal(out, want)
+    finally:
+        L.glp_destroy(ctx)
+
+
+@pytest.mark.parametrize("prec", [PREC_F32, PREC_MIXED])
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_subnormal_site_sums_mh(orc, prec, form):
+    say(f"MH with subnormal site sums prec={prec} form={form}")
+    p = helpers.tiny_emission_problem(seed=5)
+    C, pen, seed = 24, 1.0, 77
+    job = job_for(p, C, precision=prec, trace=True)
+    if form == "plain":
+        job.run_plain(gen=2, pen=pen, seed=seed)
+    else:
+        job.make_plan(4)
+        job.run_batched(gen=2, pen=pen, seed=seed)
+    compare_run(orc, p, job, C, pen, seed, prec, gen=2)
+
+
+@pytest.mark.parametrize("M", [1, 3, 31, 33, 95, 97, 301])
+def test_hmm_like_odd_site_count_strong_transitions(orc, M):
+    # with an odd site count the last site's t2 is the last float of tran,
+    # alone in its pair; r = 0.3 makes every transition term count
+    from glphase_amd._lib import c_f32p, c_u32p, c_u64p, check, lib, ptr
+    say(f"hmm_like odd M={M}, r=0.3")
+    p = helpers.synthetic(16, M, bits=8, seed=M, training_size=4096)
+    r = np.float32(0.3)
+    p["tran"][1:] = (np.float32(1 - r) * np.float32(1 - r), r * np.float32(1 - r), r * r)
+    L = lib()
+    ctx = ctypes.c_void_p()
+    check(L.glp_create(0, ctypes.byref(ctx)), "create")
+    try:
+        check(L.glp_set_tran(ctx, ptr(p["tran"], c_f32p), p["tran"].size), "tran")
+        check(L.glp_set_mutation_mat(ctx, ptr(np.ascontiguousarray(p["pc"]), c_f32p)), "pc")
+        check(L.glp_set_codebook(ctx, ptr(p["codebook"], c_f32p), 256), "cb")
+        pk = np.ascontiguousarray(p["packed"])
+        check(L.glp_set_packed_gls(ctx, ptr(pk, c_u32p), pk.size, 8), "packed")
+        rng = np.random.default_rng(M + 1)
+        n = 64
+        samples = rng.integers(0, p["N"], n).astype(np.uint32)
+        parents = rng.integers(0, p["H"], (n, 4)).astype(np.uint32)
+        out = np.zeros(n, np.float32)
+        panel = np.ascontiguousarray(p["panel"])
+        check(L.glp_hmm_like(ctx, ptr(panel, c_u64p), panel.size, M, p["N"], n,
+                             ptr(samples, c_u32p), ptr(parents, c_u32p), ptr(out, c_f32p)),
+              "hmm_like")
+        lut = orc.emit_lut(p["codebook"], p["pc"])
+        want = np.array([orc.hmm_like_f32(p["panel"], parents[e],
+                                          orc.sample_emit(p["packed"], int(samples[e]), M, 8, lut),
+                                          p["tran"]) for e in range(n)], np.float32)
+        assert np.array_equal(out, want)
+    finally:
+        L.glp_destroy(ctx)
