@@ -49,6 +49,29 @@ __device__ __forceinline__ void store_f4(float *p, float a, float b, float c, fl
   v.w = d;
   *(gmut_t<glp_f4v>)p = v;  // a C cast: the one that may change address space
 }
+
+// 1.0f / x rounded to nearest-even, for 2^-149 <= x < 2^-20.  The operand is
+// scaled by 2^64 (exact), inverted by v_rcp_f32 plus one Newton step with
+// fused multiply-adds, and scaled back (exact, or the same overflow to inf
+// as the division).  For x = 0 it gives NaN where the division gives inf.
+__device__ __forceinline__ float recip_small(float x) {
+  const float xs = x * 0x1p64f;
+  const float r = __builtin_amdgcn_rcpf(xs);
+  const float e = __builtin_fmaf(-xs, r, 1.0f);
+  return __builtin_fmaf(e, r, r) * 0x1p64f;
+}
+
+// The same without the scaling, two instructions shorter: rounded to
+// nearest-even for 2^-126 <= x <= 2^126 (normal x and result), NaN for
+// subnormal x and for 0 (v_rcp_f32 flushes a subnormal input), 0 past
+// 2^126.  Callers detect the NaN (it carries through their recurrences) and
+// redo the work with an exact division.  glp_selftest_recip checks both
+// reciprocals against the IEEE division for every float of these ranges.
+__device__ __forceinline__ float recip_normal(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(e, r, r);
+}
 }  // namespace glp
 #endif
 

@@ -112,32 +112,6 @@ __device__ __forceinline__ float dpp(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 
-// 1.0f / x rounded to nearest-even, for 2^-149 <= x < 2^-20.  The operand is
-// scaled by 2^64 (exact), inverted by v_rcp_f32 plus one Newton step with
-// fused multiply-adds, and scaled back (exact, or the same overflow to inf
-// as the division).  For x = 0 it gives NaN where the division gives inf; a
-// zero site sum turns the probabilities into NaN (0 * inf) in the reference
-// as well, so the returned likelihood is NaN either way.
-__device__ __forceinline__ float recip_small(float x) {
-  const float xs = x * 0x1p64f;
-  const float r = __builtin_amdgcn_rcpf(xs);
-  const float e = __builtin_fmaf(-xs, r, 1.0f);
-  return __builtin_fmaf(e, r, r) * 0x1p64f;
-}
-
-// The same without the scaling, two instructions shorter: rounded to
-// nearest-even for 2^-126 <= x < 2^-20 (normal x), NaN for subnormal x and
-// 0, whose input v_rcp_f32 flushes.  A NaN factor makes the pass's
-// likelihood NaN (NaN stays NaN through the rest of the recurrence), and the
-// pass is then redone with recip_small (score_set), so the returned value is
-// recip_small's in every case.  glp_selftest_recip checks both against the
-// IEEE division for every float of their ranges.
-__device__ __forceinline__ float recip_normal(float x) {
-  const float r = __builtin_amdgcn_rcpf(x);
-  const float e = __builtin_fmaf(-x, r, 1.0f);
-  return __builtin_fmaf(e, r, r);
-}
-
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
@@ -690,7 +664,8 @@ __global__ void __launch_bounds__(256)
 }
 
 // Every float with bit pattern in [lo, hi): recip_small bit-equal to the
-// division; recip_normal bit-equal to it for normal x and NaN for subnormal x.
+// division below 2^-20; recip_normal bit-equal to it on [2^-126, 2^126] and
+// NaN for subnormal x.
 __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *bad,
                                  uint32_t *first) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -698,8 +673,10 @@ __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *b
     const float x = __uint_as_float((uint32_t)b);
     const uint32_t want = __float_as_uint(1.0f / x);
     const float fast = recip_normal(x);
-    const bool ok = __float_as_uint(recip_small(x)) == want &&
-                    (x >= 0x1p-126f ? __float_as_uint(fast) == want : __builtin_isnan(fast));
+    const bool ok_small = !(x < 0x1p-20f) || __float_as_uint(recip_small(x)) == want;
+    const bool ok_fast = x < 0x1p-126f ? __builtin_isnan(fast)
+                                       : (x > 0x1p126f || __float_as_uint(fast) == want);
+    const bool ok = ok_small && ok_fast;
     if (!ok) {
       atomicAdd(bad, 1ull);
       atomicMin(first, (uint32_t)b);

@@ -5,7 +5,8 @@
 // Arithmetic contract: every float operation of impute.cpp:380-485 and of the
 // emission / genotype-probability builder of Insti::initialize
 // (insti.cpp:1345-1378) is kept in the reference's order, compiled with
-// -ffp-contract=off, and 1.0f / x is the IEEE division; powf(x, S) is the
+// -ffp-contract=off, and 1.0f / x is the IEEE division (or a reciprocal
+// proven equal to it where it is used); powf(x, S) is the
 // float of the FP64 power (x itself at S = 1).  The sampled haplotypes equal
 // the CPU oracle's bit for bit, given the same Philox stream.
 #include <hip/hip_runtime.h>
@@ -65,24 +66,40 @@ __device__ __forceinline__ void load_pc_rows(float4 *s_pc, const WorkArgs &a) {
 }
 
 // One block of the recursions: sites 8 blk .. 8 blk + 7 (aligned, so a block
-// lies inside one 64-site allele word), their GL pairs and transitions.
+// lies inside one 64-site allele word), their GL pairs and transitions, and
+// the lane's two parent words of that 64-site word.
 struct WorkBlock {
   float2 q[kWorkBlock];
   float t[3 * kWorkBlock];
+  uint64_t wx, wy;
+  float bt[kWorkBlock];  // the forward recursion's beta of the block's sites
 };
 
-// Loads block blk's inputs for sites in [lo, hi] (others zero).
-__device__ __forceinline__ void load_block(WorkBlock &b, int blk, int lo, int hi,
-                                           gptr_t<float> q2, gptr_t<float> tran, unsigned N,
-                                           unsigned i) {
+// Loads block blk's inputs.  Site indices are clamped into [0, hi] (the
+// window), so every load goes to a valid address and none sits behind a
+// branch: one wait then covers a whole block, a block ahead.  Sites of the
+// block outside the window get a copy of a real site's inputs, which the
+// recursions never use.  The parent words come first, since a block's
+// recursion needs them before its sums.
+template <bool BETA>
+__device__ __forceinline__ void load_block(WorkBlock &b, int blk, int hi, gptr_t<float> q2,
+                                           gptr_t<float> tran, gptr_t<uint64_t> rx,
+                                           gptr_t<uint64_t> ry, gptr_t<float> beta, unsigned N,
+                                           unsigned i, unsigned st) {
+  const int top = hi / kWorkBlock;  // the window's last block
+  const int cb = blk < 0 ? 0 : (blk > top ? top : blk);
+  const unsigned word = (unsigned)(kWorkBlock * cb) >> 6;
+  b.wx = rx[word];
+  b.wy = ry[word];
 #pragma unroll
   for (int k = 0; k < kWorkBlock; ++k) {
-    const int m = kWorkBlock * blk + k;
-    const bool in = blk >= 0 && m >= lo && m <= hi;
-    b.q[k] = in ? load_f2(q2 + 2 * ((size_t)m * N + i)) : make_float2(0, 0);
-    b.t[3 * k] = in ? tran[3 * m] : 0.0f;
-    b.t[3 * k + 1] = in ? tran[3 * m + 1] : 0.0f;
-    b.t[3 * k + 2] = in ? tran[3 * m + 2] : 0.0f;
+    const int m = kWorkBlock * cb + k;
+    const int mc = m > hi ? hi : m;
+    b.q[k] = load_f2(q2 + 2 * ((size_t)mc * N + i));
+    b.t[3 * k] = tran[3 * mc];
+    b.t[3 * k + 1] = tran[3 * mc + 1];
+    b.t[3 * k + 2] = tran[3 * mc + 2];
+    if (BETA) b.bt[k] = beta[4 * ((size_t)mc * N + i) + st];
   }
 }
 
@@ -105,6 +122,140 @@ __device__ __forceinline__ float quad_sum(float x) {
   return s + qperm<kQ3>(x);
 }
 
+// One lane's view of a sample for the two recursions.
+struct WorkLane {
+  gptr_t<uint64_t> rx, ry;  // this state's parents f_x, m_y
+  gptr_t<float> q2, tran;
+  float *beta;              // [M][N] x 4
+  gptr_t<float> gbeta;
+  const float4 *s_pc;       // rows pc[s][0..3] in LDS
+  unsigned M, N, i, st;
+};
+
+// 1.0f / x: the IEEE division, or (FAST) the 3-instruction reciprocal, which
+// equals it for normal x up to 2^126 and is NaN below (and 0 above) -- a NaN
+// or zero sum leaves a NaN or all-zero state that the caller detects.
+template <bool FAST>
+__device__ __forceinline__ float work_recip(float x) {
+  return FAST ? recip_normal(x) : 1.0f / x;
+}
+
+// Backward step over block blk (sites 8 blk + 7 down to 8 blk, those in
+// [1, M-1]) of impute.cpp:398-416: site m turns beta_m into beta_{m-1},
+// written to the workspace.
+template <bool FAST>
+__device__ __forceinline__ void bwd_block(const WorkLane &L, const WorkBlock &cur, int blk,
+                                          float &b) {
+  constexpr int B = kWorkBlock;
+  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
+  float e[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const unsigned m = (unsigned)(B * blk + k);
+    const unsigned row = (hbit64(cur.wx, m) << 1) | hbit64(cur.wy, m);
+    e[k] = work_emit(L.s_pc[row], work_prob(cur.q[k]));
+  }
+#pragma unroll
+  for (int k = B - 1; k >= 0; --k) {
+    const int mi = B * blk + k;
+    if (mi > (int)M - 1 || mi < 1) continue;
+    const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
+    const float bb = b * e[k];
+    // bb_s t0 + (pair-cross sum) t1 + bb_{3-s} t2; the cross sum is
+    // bb01 + bb10 for s = 00, 11 and bb00 + bb11 for s = 01, 10
+    const float cross = qperm<kQSwap1>(bb) + qperm<kQSwap2>(bb);
+    const float n = bb * t0 + cross * t1 + qperm<kQOpp>(bb) * t2;
+    const float sum = work_recip<FAST>(quad_sum(n));
+    b = n * sum;
+    L.beta[4 * ((size_t)(mi - 1) * N + i) + st] = b;
+  }
+}
+
+// Backward recursion (impute.cpp:398-416) over the window, blocks from the
+// last to the first, each block's inputs loaded a block ahead into the
+// other of two buffers (no copy between them, so no wait for the load just
+// issued).  Returns whether the last beta is a proper one (its quad sum
+// positive), which fails exactly when a FAST reciprocal met a subnormal,
+// zero or huge sum.
+template <bool FAST>
+__device__ __forceinline__ bool work_backward(const WorkLane &L) {
+  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
+  const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
+  float b = 1;
+  L.beta[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
+  WorkBlock x, y;
+  load_block<false>(x, top_blk, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+  for (int blk = top_blk;;) {
+    load_block<false>(y, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    bwd_block<FAST>(L, x, blk, b);
+    if (--blk < 0) break;
+    load_block<false>(x, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    bwd_block<FAST>(L, y, blk, b);
+    if (--blk < 0) break;
+  }
+  return quad_sum(b) > 0.0f;
+}
+
+// Forward step over block blk (sites 8 blk .. 8 blk + 7, those below M) of
+// impute.cpp:418-455; each site's slot of the workspace takes the state
+// posterior l * beta (beta read with the block, a block ahead).
+template <bool FAST>
+__device__ __forceinline__ void fwd_block(const WorkLane &L, const WorkBlock &cur, int blk,
+                                          float &l) {
+  constexpr int B = kWorkBlock;
+  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
+  // forward coefficient of state j's value in this state's update:
+  // t[popcount(st ^ j)] (impute.cpp:435-445)
+  const bool own[4] = {st == 0, st == 1, st == 2, st == 3};
+  const bool opp[4] = {st == 3, st == 2, st == 1, st == 0};
+  float e[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const unsigned m = (unsigned)(B * blk + k);
+    const unsigned row = (hbit64(cur.wx, m) << 1) | hbit64(cur.wy, m);
+    e[k] = work_emit(L.s_pc[row], work_prob(cur.q[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const unsigned m = (unsigned)(B * blk + k);
+    if (m >= M) break;
+    if (m) {
+      const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
+      float c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = own[j] ? t0 : (opp[j] ? t2 : t1);
+      const float n = qperm<kQ0>(l) * c[0] + qperm<kQ1>(l) * c[1] + qperm<kQ2>(l) * c[2] +
+                      qperm<kQ3>(l) * c[3];
+      l = n * e[k];
+    } else {
+      l = 0.25f * e[k];
+    }
+    const float sum = work_recip<FAST>(quad_sum(l));
+    l *= sum;
+    L.beta[4 * ((size_t)m * N + i) + st] = l * cur.bt[k];
+  }
+}
+
+// Forward recursion (impute.cpp:418-455), sites 0 .. M-1, with the same two
+// buffers as work_backward.  Returns whether the last state is a proper one.
+template <bool FAST>
+__device__ __forceinline__ bool work_forward(const WorkLane &L) {
+  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
+  const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
+  float l = 0;
+  WorkBlock x, y;
+  load_block<true>(x, 0, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+  for (int blk = 0;;) {
+    load_block<true>(y, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    fwd_block<FAST>(L, x, blk, l);
+    if (++blk > top_blk) break;
+    load_block<true>(x, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    fwd_block<FAST>(L, y, blk, l);
+    if (++blk > top_blk) break;
+  }
+  return quad_sum(l) > 0.0f;
+}
+
 // The two recursions of impute.cpp:398-470 with one lane per (window,
 // sample, state): lane s of a quad holds state s's value, the cross-state
 // terms come by quad DPP moves in the reference's operand order, and the
@@ -114,7 +265,9 @@ __device__ __forceinline__ float quad_sum(float x) {
 // scaled forward pass then stores each site's state posteriors
 // (l00 b00, l01 b01, l10 b10, l11 b11) over its beta.  Sites go in aligned
 // blocks of 8 whose GL pairs, transitions and (forward) beta are loaded a
-// block ahead.  Parent indices past the panel are clamped (the MH update
+// block ahead.  The recursions run with the short reciprocal first; a
+// wavefront in which some sample met a sum outside its exact range redoes
+// both with the IEEE division, so every stored value is the division's.  Parent indices past the panel are clamped (the MH update
 // never produces them).
 __global__ void __launch_bounds__(256)
     k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
@@ -126,107 +279,32 @@ __global__ void __launch_bounds__(256)
   const unsigned i = g >> 2, st = g & 3u;
   if (i >= a.N) return;  // whole quads leave together
   const glp_work_window W = wins[w];
-  const unsigned M = W.num_sites, wn = W.words_per_hap, N = a.N;
-  float *beta = reinterpret_cast<float *>(ws + beta_off[w]);  // [M][N] x 4
-  const auto gbeta = gmem(beta);
-  const auto q2 = gmem(W.het_aa), tran = gmem(W.tran);
-  // this state's two parents: f_x (slot x) and m_y (slot 2 + y)
-  gptr_t<uint64_t> rx, ry;
+  WorkLane L;
+  L.M = W.num_sites;
+  L.N = a.N;
+  L.i = i;
+  L.st = st;
+  L.beta = reinterpret_cast<float *>(ws + beta_off[w]);
+  L.gbeta = gmem(L.beta);
+  L.q2 = gmem(W.het_aa);
+  L.tran = gmem(W.tran);
+  L.s_pc = s_pc;
   {
+    const unsigned wn = W.words_per_hap;
     unsigned hx = gmem(W.parents)[4 * (size_t)i + (st >> 1)];
     unsigned hy = gmem(W.parents)[4 * (size_t)i + 2 + (st & 1u)];
     hx = hx < a.H ? hx : a.H - 1;
     hy = hy < a.H ? hy : a.H - 1;
-    rx = gmem(W.panel) + (size_t)hx * wn;
-    ry = gmem(W.panel) + (size_t)hy * wn;
+    L.rx = gmem(W.panel) + (size_t)hx * wn;
+    L.ry = gmem(W.panel) + (size_t)hy * wn;
   }
-  // forward coefficient of state j's value in this state's update:
-  // t[popcount(st ^ j)] (impute.cpp:435-445)
-  const bool own[4] = {st == 0, st == 1, st == 2, st == 3};
-  const bool opp[4] = {st == 3, st == 2, st == 1, st == 0};
-  constexpr int B = kWorkBlock;
-  const int top_blk = (int)(M - 1) / B;
-
-  // backward recursion (impute.cpp:398-416): site m = M-1 .. 1 turns beta_m
-  // into beta_{m-1}
-  float b = 1;
-  beta[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
-  WorkBlock cur, nxt;
-  load_block(nxt, top_blk, 1, (int)M - 1, q2, tran, N, i);
-  for (int blk = top_blk; blk >= 0; --blk) {
-    cur = nxt;
-    load_block(nxt, blk - 1, 1, (int)M - 1, q2, tran, N, i);
-    const unsigned word = (unsigned)(B * blk) >> 6;
-    const uint64_t wx = rx[word], wy = ry[word];
-    float e[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const unsigned m = (unsigned)(B * blk + k);
-      const unsigned row = (hbit64(wx, m) << 1) | hbit64(wy, m);
-      e[k] = work_emit(s_pc[row], work_prob(cur.q[k]));
-    }
-#pragma unroll
-    for (int k = B - 1; k >= 0; --k) {
-      const int mi = B * blk + k;
-      if (mi > (int)M - 1 || mi < 1) continue;
-      const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-      const float bb = b * e[k];
-      // bb_s t0 + (pair-cross sum) t1 + bb_{3-s} t2; the cross sum is
-      // bb01 + bb10 for s = 00, 11 and bb00 + bb11 for s = 01, 10
-      const float cross = qperm<kQSwap1>(bb) + qperm<kQSwap2>(bb);
-      const float n = bb * t0 + cross * t1 + qperm<kQOpp>(bb) * t2;
-      const float sum = 1.0f / quad_sum(n);
-      b = n * sum;
-      beta[4 * ((size_t)(mi - 1) * N + i) + st] = b;
-    }
-  }
-
-  // forward recursion (impute.cpp:418-455), sites 0 .. M-1; each site's beta
-  // is read (a block ahead) before its slot takes the posterior
-  float l = 0;
-  float bn[B];
-  load_block(nxt, 0, 0, (int)M - 1, q2, tran, N, i);
-#pragma unroll
-  for (int k = 0; k < B; ++k) bn[k] = (unsigned)k < M ? gbeta[4 * ((size_t)k * N + i) + st] : 0.0f;
-  for (int blk = 0; blk <= top_blk; ++blk) {
-    cur = nxt;
-    float bc[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      bc[k] = bn[k];
-      const unsigned m = (unsigned)(B * (blk + 1) + k);
-      bn[k] = m < M ? gbeta[4 * ((size_t)m * N + i) + st] : 0.0f;
-    }
-    load_block(nxt, blk + 1, 0, (int)M - 1, q2, tran, N, i);
-    const unsigned word = (unsigned)(B * blk) >> 6;
-    const uint64_t wx = rx[word], wy = ry[word];
-    float e[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const unsigned m = (unsigned)(B * blk + k);
-      const unsigned row = (hbit64(wx, m) << 1) | hbit64(wy, m);
-      e[k] = work_emit(s_pc[row], work_prob(cur.q[k]));
-    }
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const unsigned m = (unsigned)(B * blk + k);
-      if (m >= M) break;
-      if (m) {
-        const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-        float c[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) c[j] = own[j] ? t0 : (opp[j] ? t2 : t1);
-        const float n = qperm<kQ0>(l) * c[0] + qperm<kQ1>(l) * c[1] + qperm<kQ2>(l) * c[2] +
-                        qperm<kQ3>(l) * c[3];
-        l = n * e[k];
-      } else {
-        l = 0.25f * e[k];
-      }
-      const float sum = 1.0f / quad_sum(l);
-      l *= sum;
-      beta[4 * ((size_t)m * N + i) + st] = l * bc[k];
-    }
-  }
+  // the forward recursion stores the posteriors over its input beta, so a
+  // failed forward recursion is redone from the backward one
+  if (__builtin_expect(!__any(!work_backward<true>(L)), 1) &&
+      __builtin_expect(!__any(!work_forward<true>(L)), 1))
+    return;
+  work_backward<false>(L);
+  work_forward<false>(L);
 }
 
 // The sampling of impute.cpp:457-484, one lane per (window, sample, 64-site

@@ -215,11 +215,12 @@ int glp_plan_graph_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
 int glp_plan_graph_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                              void *stream);
 
-/* Self-test of the device reciprocals the rescale step uses (no counterpart
- * in the reference): compares them with the IEEE division 1.0f / x for every
- * float x with bit pattern in [lo_bits, hi_bits) on the current device (the
- * exact one bit-equal everywhere; the short one bit-equal for normal x and
- * NaN for subnormal x, which sends the pass to the exact one).
+/* Self-test of the device reciprocals of the rescale and resampling steps
+ * (no counterpart in the reference): compares them with the IEEE division
+ * 1.0f / x for every float x with bit pattern in [lo_bits, hi_bits) on the
+ * current device (the scaled one bit-equal below 2^-20; the short one
+ * bit-equal on [2^-126, 2^126] and NaN for subnormal x, which sends the
+ * work to an exact division).
  * mismatches receives the count, first_bad the lowest mismatching pattern
  * (0xFFFFFFFF if none).  Synchronous. */
 int glp_selftest_recip(uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,

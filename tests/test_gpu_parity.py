@@ -294,10 +294,12 @@ def test_rescale_reciprocal_is_exact():
     say("reciprocal self-test")
     norm = np.float32(np.finfo(np.float32).tiny) ** np.float32(2.0 / 3.0)
     hi = int(np.array([norm], np.float32).view(np.uint32)[0]) + 1
-    bad, first = ctypes.c_uint64(), ctypes.c_uint32()
-    check(lib().glp_selftest_recip(1, hi, ctypes.byref(bad), ctypes.byref(first)),
-          "glp_selftest_recip")
-    assert bad.value == 0, f"{bad.value} mismatches, first at 0x{first.value:08x}"
+    # and on to 2^126 for the short reciprocal (hmm_work's site sums)
+    for lo_bits, hi_bits in ((1, hi), (hi, 0x7E800001)):
+        bad, first = ctypes.c_uint64(), ctypes.c_uint32()
+        check(lib().glp_selftest_recip(lo_bits, hi_bits, ctypes.byref(bad), ctypes.byref(first)),
+              "glp_selftest_recip")
+        assert bad.value == 0, f"{bad.value} mismatches, first at 0x{first.value:08x}"
 
 
 @pytest.mark.gpu

@@ -61,6 +61,28 @@ def test_hmm_work_matches_oracle(orc, N, M, S):
         assert (got[:, -1] >> np.uint64(M % 64) == 0).all()
 
 
+@pytest.mark.parametrize("S", [1.0, 0.49])
+def test_hmm_work_subnormal_sums(orc, S):
+    # mutation-matrix entries near 2^-127.7 make every emission, and so most
+    # site sums of both recursions, subnormal or just above: the short
+    # reciprocal gives NaN there, the wavefront redoes the recursion with the
+    # IEEE division, and the sampled haplotypes still equal the oracle's
+    say(f"hmm_work with subnormal site sums S={S}")
+    from glphase_amd.engine import Resampler
+    p = helpers.synthetic(48, 300, seed=17)
+    c = np.float32(2.0 ** -127.7)
+    p["pc"] = np.array([[np.float32(r) * c] * 4 for r in (1.0, 1.1, 1.2, 1.3)], np.float32)
+    rng = np.random.default_rng(18)
+    parents = rng.integers(0, p["H"], (p["N"], 4)).astype(np.uint32)
+    job = job_with_parents(p, parents)
+    rs = Resampler(job, [het_aa_of(p)], p["pc"])
+    seed, gen = 31, 2
+    rs.run(pen=S, seed=seed, gen=gen)
+    got = rs.hnew()[0]
+    want = oracle_work(orc, p["panel"], parents, het_aa_of(p), p["tran"], p["pc"], S, seed, 0, gen)
+    assert (got == want).all()
+
+
 def test_generations_end_to_end(orc):
     # three generations of batched MH + resampling + panel swap, annealed pen
     say("generations end to end")
