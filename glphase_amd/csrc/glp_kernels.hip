@@ -74,7 +74,7 @@ __device__ __forceinline__ float cr_logf(float x) {
 // entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
 // in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
 constexpr unsigned kSpreadBytes = 4 * 256 * 8;
-constexpr unsigned kTranFloats = 96;   // per wave: tran of one 32-site chunk
+constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk (a wave has two slots)
 constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
 
 __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
@@ -192,11 +192,12 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
   }
 }
 
-// 32 sites of a chunk.  h = the 4 parents' 32-bit allele words, cw = the
-// chunk's code words, tr = the wave's LDS copy of the chunk's tran, n =
-// sites in the chunk (TAIL: n < 32), nrm0 = the threshold of the chunk's
-// first site (0 at the window's site 0, which only initialises,
-// impute.cpp:339-347).
+// The sites of a chunk without the group pipeline (used for a window's
+// last chunk when it is shorter than 32 sites).  h = the 4 parents' 32-bit
+// allele words, cw = the chunk's code words, tr = the wave's LDS copy of the
+// chunk's tran, n = sites in the chunk (TAIL: n < 32), nrm0 = the threshold
+// of the chunk's first site (0 at the window's site 0, which only
+// initialises, impute.cpp:339-347).
 template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
@@ -239,6 +240,96 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       st.pexp += ex;
     }
   }
+}
+
+// LDS inputs of one 8-site group of a full chunk, loaded a group ahead of
+// its recurrence: the parents' spread entries, the sites' tran rows and the
+// lane's emissions.
+struct GroupLds {
+  uint2 sp[4];
+  float4 t4[6];
+  float2 e[8];
+};
+
+// First half of group g's loads: spread entries (from the chunk's allele
+// words h) and tran rows (from the chunk's LDS slot tr).
+__device__ __forceinline__ void group_load_a(GroupLds &G, const uint32_t (&h)[4], const float *tr,
+                                             int g, const uint2 *spread) {
+#pragma unroll
+  for (int p = 0; p < 4; ++p) G.sp[p] = spread[p * 256 + ((h[p] >> (8 * g)) & 255u)];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) G.t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
+}
+
+// Second half: the emissions, whose addresses need the spread entries.
+template <int BITS>
+__device__ __forceinline__ void group_load_b(GroupLds &G, const uint32_t (&cw)[BITS], int g,
+                                             uint32_t halfbits, const unsigned char *smem) {
+  constexpr int CPW = 32 / BITS;
+  const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x | halfbits,
+                          G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y | halfbits};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int s = 8 * g + k;
+    const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
+    G.e[k] = *reinterpret_cast<const float2 *>(smem + a);
+  }
+}
+
+// Sites k0 .. k0 + 3 of group g, from its loaded inputs; after the 8th, the
+// MIXED product's exponent is split off.
+template <bool MIXED, bool FAST>
+__device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, int g, int k0,
+                                            float norm, float nrm0, float *ring) {
+  const float *tg = reinterpret_cast<const float *>(G.t4);
+#pragma unroll
+  for (int k = k0; k < k0 + 4; ++k)
+    pair_step<MIXED, FAST>(st, G.e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2],
+                           (g == 0 && k == 0) ? nrm0 : norm, ring);
+  if (MIXED && k0 == 4) {
+    // at most 8 factors below 2^127 since the last split: no overflow
+    const int ex = __builtin_amdgcn_frexp_exp(st.prod);
+    st.prod = __builtin_amdgcn_frexp_mant(st.prod);
+    st.pexp += ex;
+  }
+}
+
+// A full 32-site chunk as a software pipeline over its 4 groups: while one
+// group's recurrence runs, the next group's LDS loads are in flight (spread
+// and tran before its first half, emissions before its second half), in
+// two buffers A and B that the groups alternate between.  On entry A holds
+// group 0's inputs; during group 3 the next chunk's tran goes into its LDS
+// slot trn (from tpn) and A takes the next chunk's group 0 (from hn, cn).
+// LDS reads past the window's end only read the tables.
+template <int BITS, bool MIXED, bool FAST>
+__device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, GroupLds &B,
+                                               const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
+                                               const float *tr, const uint32_t (&hn)[4],
+                                               const uint32_t (&cn)[BITS], float *trn, float2 tpn,
+                                               const unsigned char *smem, const uint2 *spread,
+                                               uint32_t halfbits, float norm, float nrm0,
+                                               float *ring) {
+  const unsigned lane = threadIdx.x & 63u;
+  group_load_a(B, h, tr, 1, spread);
+  group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
+  group_load_b<BITS>(B, cw, 1, halfbits, smem);
+  group_steps<MIXED, FAST>(st, A, 0, 4, norm, nrm0, ring);
+
+  group_load_a(A, h, tr, 2, spread);
+  group_steps<MIXED, FAST>(st, B, 1, 0, norm, nrm0, ring);
+  group_load_b<BITS>(A, cw, 2, halfbits, smem);
+  group_steps<MIXED, FAST>(st, B, 1, 4, norm, nrm0, ring);
+
+  group_load_a(B, h, tr, 3, spread);
+  group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
+  group_load_b<BITS>(B, cw, 3, halfbits, smem);
+  group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
+
+  if (lane < 48u) reinterpret_cast<float2 *>(trn)[lane] = tpn;
+  group_load_a(A, hn, trn, 0, spread);
+  group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
+  group_load_b<BITS>(A, cn, 0, halfbits, smem);
+  group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
 // Floats o and o + 1 of tran (n floats in all), 0 past the end: when n is
@@ -292,14 +383,19 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   if (lane < 48u) tp = tran_pair(tran, 2u * lane, ntr);
   if (lane == 0u) tp = make_float2(1.0f, 0.0f);
   if (lane == 1u) tp.x = 0.0f;
+  // chunk ch's tran sits in LDS slot ch & 1 of the wave's two; a full
+  // chunk's last group writes the next chunk's slot
+  if (lane < 48u) reinterpret_cast<float2 *>(wtr)[lane] = tp;
+  GroupLds A, B;
+  if (M >= 32u) {
+    group_load_a(A, h, wtr, 0, spread);
+    group_load_b<BITS>(A, cw, 0, halfbits, smem);
+  }
   for (unsigned ch = 0; ch < nch; ++ch) {
-    if (lane < 48u) reinterpret_cast<float2 *>(wtr)[lane] = tp;
     // prefetch the next chunk's tran, allele and code words
     const unsigned nx = ch + 1;
-    {
-      tp = make_float2(0.0f, 0.0f);
-      if (lane < 48u) tp = tran_pair(tran, kTranFloats * nx + 2u * lane, ntr);
-    }
+    tp = make_float2(0.0f, 0.0f);
+    if (lane < 48u) tp = tran_pair(tran, kTranFloats * nx + 2u * lane, ntr);
     uint32_t hn[4] = {0, 0, 0, 0}, cn[NW];
     if (nx < nch) {
       hn[0] = f0[nx];
@@ -315,11 +411,12 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     const unsigned base = ch << 5;
     const unsigned n = M - base < 32u ? M - base : 32u;
     const float nrm0 = ch == 0 ? 0.0f : norm;
+    float *tr = wtr + kTranFloats * (ch & 1u), *trn = wtr + kTranFloats * ((ch + 1u) & 1u);
     if (n == 32)
-      fwd_chunk<BITS, MIXED, false, FAST>(st, h, cw, wtr, 32, smem, spread, halfbits, norm, nrm0,
-                                          ring);
-    else
-      fwd_chunk<BITS, MIXED, true, FAST>(st, h, cw, wtr, n, smem, spread, halfbits, norm, nrm0,
+      fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, h, cw, tr, hn, cn, trn, tp, smem, spread,
+                                        halfbits, norm, nrm0, ring);
+    else  // the window's last chunk, shorter than 32 sites
+      fwd_chunk<BITS, MIXED, true, FAST>(st, h, cw, tr, n, smem, spread, halfbits, norm, nrm0,
                                          ring);
     if (!MIXED) flush_ring(st, ring);
 #pragma unroll
@@ -336,8 +433,8 @@ __device__ __forceinline__ const uint32_t *hap_row(const glp_window &W, unsigned
   return reinterpret_cast<const uint32_t *>(W.panel + (size_t)h * W.words_per_hap);
 }
 
-// LDS layout after the two tables: per wave, kTranFloats floats of tran,
-// then (F32 only) per wave a ring of kRingSlots x 64 floats.
+// LDS layout after the two tables: per wave, two slots of kTranFloats
+// floats of tran, then (F32 only) per wave a ring of kRingSlots x 64 floats.
 template <int BITS, bool MIXED>
 __device__ __forceinline__ double score_set(const glp_window &W, const unsigned (&P)[4],
                                             unsigned sample, unsigned N,
@@ -346,8 +443,9 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
   const unsigned wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   float *tables_end = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) +
                                                 16u * lut_entries + kSpreadBytes);
-  float *wtr = tables_end + kTranFloats * wave;
-  float *ring = tables_end + kTranFloats * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
+  float *wtr = tables_end + 2u * kTranFloats * wave;
+  float *ring =
+      tables_end + 2u * kTranFloats * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
   const uint32_t *f0 = hap_row(W, P[0]), *f1 = hap_row(W, P[1]), *m0 = hap_row(W, P[2]),
                  *m1 = hap_row(W, P[3]);
   const double v = forward_pass<BITS, MIXED, true>(f0, f1, m0, m1, W.packed + sample, N, W.tran,
@@ -689,7 +787,7 @@ __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *b
 // ---------------------------------------------------------------------------
 size_t lds_bytes(const KArgs &a) {
   const size_t waves = 4;  // 256-thread blocks
-  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * kTranFloats * 4;
+  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * kTranFloats * 4;
   if (a.precision == kPrecF32) b += waves * kRingSlots * 64 * 4;
   return b;
 }
