@@ -72,7 +72,6 @@ struct WorkBlock {
   float2 q[kWorkBlock];
   float t[3 * kWorkBlock];
   uint64_t wx, wy;
-  float bt[kWorkBlock];  // the forward recursion's beta of the block's sites
 };
 
 // Loads block blk's inputs.  Site indices are clamped into [0, hi] (the
@@ -81,11 +80,9 @@ struct WorkBlock {
 // block outside the window get a copy of a real site's inputs, which the
 // recursions never use.  The parent words come first, since a block's
 // recursion needs them before its sums.
-template <bool BETA>
 __device__ __forceinline__ void load_block(WorkBlock &b, int blk, int hi, gptr_t<float> q2,
                                            gptr_t<float> tran, gptr_t<uint64_t> rx,
-                                           gptr_t<uint64_t> ry, gptr_t<float> beta, unsigned N,
-                                           unsigned i, unsigned st) {
+                                           gptr_t<uint64_t> ry, unsigned N, unsigned i) {
   const int top = hi / kWorkBlock;  // the window's last block
   const int cb = blk < 0 ? 0 : (blk > top ? top : blk);
   const unsigned word = (unsigned)(kWorkBlock * cb) >> 6;
@@ -99,7 +96,6 @@ __device__ __forceinline__ void load_block(WorkBlock &b, int blk, int hi, gptr_t
     b.t[3 * k] = tran[3 * mc];
     b.t[3 * k + 1] = tran[3 * mc + 1];
     b.t[3 * k + 2] = tran[3 * mc + 2];
-    if (BETA) b.bt[k] = beta[4 * ((size_t)mc * N + i) + st];
   }
 }
 
@@ -126,8 +122,7 @@ __device__ __forceinline__ float quad_sum(float x) {
 struct WorkLane {
   gptr_t<uint64_t> rx, ry;  // this state's parents f_x, m_y
   gptr_t<float> q2, tran;
-  float *beta;              // [M][N] x 4
-  gptr_t<float> gbeta;
+  float *out;               // [M][N] x 4: beta (backward) or scaled l (forward)
   const float4 *s_pc;       // rows pc[s][0..3] in LDS
   unsigned M, N, i, st;
 };
@@ -167,7 +162,7 @@ __device__ __forceinline__ void bwd_block(const WorkLane &L, const WorkBlock &cu
     const float n = bb * t0 + cross * t1 + qperm<kQOpp>(bb) * t2;
     const float sum = work_recip<FAST>(quad_sum(n));
     b = n * sum;
-    L.beta[4 * ((size_t)(mi - 1) * N + i) + st] = b;
+    L.out[4 * ((size_t)(mi - 1) * N + i) + st] = b;
   }
 }
 
@@ -182,14 +177,14 @@ __device__ __forceinline__ bool work_backward(const WorkLane &L) {
   const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
   const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
   float b = 1;
-  L.beta[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
+  L.out[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
   WorkBlock x, y;
-  load_block<false>(x, top_blk, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+  load_block(x, top_blk, hi, L.q2, L.tran, L.rx, L.ry, N, i);
   for (int blk = top_blk;;) {
-    load_block<false>(y, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    load_block(y, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
     bwd_block<FAST>(L, x, blk, b);
     if (--blk < 0) break;
-    load_block<false>(x, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    load_block(x, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
     bwd_block<FAST>(L, y, blk, b);
     if (--blk < 0) break;
   }
@@ -197,8 +192,8 @@ __device__ __forceinline__ bool work_backward(const WorkLane &L) {
 }
 
 // Forward step over block blk (sites 8 blk .. 8 blk + 7, those below M) of
-// impute.cpp:418-455; each site's slot of the workspace takes the state
-// posterior l * beta (beta read with the block, a block ahead).
+// impute.cpp:418-455; each site's slot of the forward table takes the
+// scaled state probability l.
 template <bool FAST>
 __device__ __forceinline__ void fwd_block(const WorkLane &L, const WorkBlock &cur, int blk,
                                           float &l) {
@@ -232,51 +227,56 @@ __device__ __forceinline__ void fwd_block(const WorkLane &L, const WorkBlock &cu
     }
     const float sum = work_recip<FAST>(quad_sum(l));
     l *= sum;
-    L.beta[4 * ((size_t)m * N + i) + st] = l * cur.bt[k];
+    L.out[4 * ((size_t)m * N + i) + st] = l;
   }
 }
 
 // Forward recursion (impute.cpp:418-455), sites 0 .. M-1, with the same two
-// buffers as work_backward.  Returns whether the last state is a proper one.
+// buffers as work_backward.  It needs no beta, so it runs beside the
+// backward recursion.  Returns whether the last state is a proper one.
 template <bool FAST>
 __device__ __forceinline__ bool work_forward(const WorkLane &L) {
   const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
   const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
   float l = 0;
   WorkBlock x, y;
-  load_block<true>(x, 0, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+  load_block(x, 0, hi, L.q2, L.tran, L.rx, L.ry, N, i);
   for (int blk = 0;;) {
-    load_block<true>(y, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    load_block(y, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
     fwd_block<FAST>(L, x, blk, l);
     if (++blk > top_blk) break;
-    load_block<true>(x, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, L.gbeta, N, i, st);
+    load_block(x, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
     fwd_block<FAST>(L, y, blk, l);
     if (++blk > top_blk) break;
   }
   return quad_sum(l) > 0.0f;
 }
 
-// The two recursions of impute.cpp:398-470 with one lane per (window,
-// sample, state): lane s of a quad holds state s's value, the cross-state
-// terms come by quad DPP moves in the reference's operand order, and the
-// four lanes compute each site's normaliser together.  The scaled backward
-// pass writes beta to the workspace ([M][N] float4, site-major: a quad
-// stores one float4, a wavefront 16 samples' contiguous float4s); the
-// scaled forward pass then stores each site's state posteriors
-// (l00 b00, l01 b01, l10 b10, l11 b11) over its beta.  Sites go in aligned
-// blocks of 8 whose GL pairs, transitions and (forward) beta are loaded a
-// block ahead.  The recursions run with the short reciprocal first; a
-// wavefront in which some sample met a sum outside its exact range redoes
-// both with the IEEE division, so every stored value is the division's.  Parent indices past the panel are clamped (the MH update
-// never produces them).
+// The two recursions of impute.cpp:398-470, side by side: one lane per
+// (window, sample, state, recursion); even wavefronts run the backward
+// recursion of 16 samples, odd ones the forward recursion of the same 16
+// (each wavefront takes one code path).  Lane s of a quad holds state s's
+// value, the cross-state terms come by quad DPP moves in the reference's
+// operand order, and the four lanes compute each site's normaliser
+// together.  The backward recursion writes beta to the first table of the
+// window's workspace, the forward one the scaled l to the second ([M][N]
+// float4 each, site-major: a quad stores one float4, a wavefront 16
+// samples' contiguous float4s); the sampling step multiplies them into the
+// posteriors l * beta, the reference's product.  Sites go in aligned blocks
+// of 8 whose GL pairs, transitions and parent words are loaded a block
+// ahead.  A recursion runs with the short reciprocal first; a wavefront in
+// which some sample met a sum outside its exact range redoes it with the
+// IEEE division, so every stored value is the division's.  Parent indices
+// past the panel are clamped (the MH update never produces them).
 __global__ void __launch_bounds__(256)
-    k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
+    k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ ws_off,
                     float4 *__restrict__ ws, WorkArgs a) {
   __shared__ float4 s_pc[4];  // rows pc[s][0..3]
   load_pc_rows(s_pc, a);
   const unsigned w = blockIdx.y;
   const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned i = g >> 2, st = g & 3u;
+  const unsigned wave = g >> 6, fwd = wave & 1u;
+  const unsigned i = (wave >> 1) * 16u + ((g & 63u) >> 2), st = g & 3u;
   if (i >= a.N) return;  // whole quads leave together
   const glp_work_window W = wins[w];
   WorkLane L;
@@ -284,8 +284,7 @@ __global__ void __launch_bounds__(256)
   L.N = a.N;
   L.i = i;
   L.st = st;
-  L.beta = reinterpret_cast<float *>(ws + beta_off[w]);
-  L.gbeta = gmem(L.beta);
+  L.out = reinterpret_cast<float *>(ws + ws_off[w] + (fwd ? (size_t)L.M * a.N : 0));
   L.q2 = gmem(W.het_aa);
   L.tran = gmem(W.tran);
   L.s_pc = s_pc;
@@ -298,24 +297,23 @@ __global__ void __launch_bounds__(256)
     L.rx = gmem(W.panel) + (size_t)hx * wn;
     L.ry = gmem(W.panel) + (size_t)hy * wn;
   }
-  // the forward recursion stores the posteriors over its input beta, so a
-  // failed forward recursion is redone from the backward one
-  if (__builtin_expect(!__any(!work_backward<true>(L)), 1) &&
-      __builtin_expect(!__any(!work_forward<true>(L)), 1))
-    return;
-  work_backward<false>(L);
-  work_forward<false>(L);
+  if (fwd) {
+    if (__builtin_expect(__any(!work_forward<true>(L)), 0)) work_forward<false>(L);
+  } else {
+    if (__builtin_expect(__any(!work_backward<true>(L)), 0)) work_backward<false>(L);
+  }
 }
 
 // The sampling of impute.cpp:457-484, one lane per (window, sample, 64-site
 // word), sample fastest so a wavefront's reads of a site coalesce: from the
-// posteriors p (written by k_hmm_work_pass) and the GL pair, the genotype
+// posteriors p = l * beta (the two tables k_hmm_work_pass wrote) and the GL
+// pair, the genotype
 // weights c_g = (t_g * sum_s p_s pc[s][g])^S, one Philox word per site, the
 // genotype choice, and the two new haplotype words.  POW: S != 1 (powf is
 // the identity at S = 1).
 template <bool POW>
 __global__ void __launch_bounds__(256)
-    k_hmm_work_sample(const glp_work_window *__restrict__ wins, const size_t *__restrict__ beta_off,
+    k_hmm_work_sample(const glp_work_window *__restrict__ wins, const size_t *__restrict__ ws_off,
                       const float4 *__restrict__ ws, WorkArgs a) {
   __shared__ float4 s_pc[4];
   load_pc_rows(s_pc, a);
@@ -325,7 +323,8 @@ __global__ void __launch_bounds__(256)
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)N * wn) return;
   const unsigned i = (unsigned)(t % N), wd = (unsigned)(t / N);
-  const auto post = gmem(reinterpret_cast<const float *>(ws + beta_off[w]));
+  const auto beta = gmem(reinterpret_cast<const float *>(ws + ws_off[w]));
+  const auto lfwd = gmem(reinterpret_cast<const float *>(ws + ws_off[w] + (size_t)M * N));
   const auto q2 = gmem(W.het_aa);
   uint64_t hw[4];
 #pragma unroll
@@ -339,7 +338,9 @@ __global__ void __launch_bounds__(256)
   uint64_t wa = 0, wb = 0;
   U4 x{0, 0, 0, 0};
   for (unsigned m = 64u * wd; m < m1; ++m) {
-    const float4 p = load_f4(post + 4 * ((size_t)m * N + i));
+    const float4 bq = load_f4(beta + 4 * ((size_t)m * N + i));
+    const float4 lq = load_f4(lfwd + 4 * ((size_t)m * N + i));
+    const float4 p = make_float4(lq.x * bq.x, lq.y * bq.y, lq.z * bq.z, lq.w * bq.w);
     const float3 tq = work_prob(load_f2(q2 + 2 * ((size_t)m * N + i)));
     const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
                    m1b = hbit64(hw[3], m);
@@ -394,7 +395,7 @@ extern "C" size_t glp_work_workspace_bytes(const glp_work_desc *d) {
   if (!d || !d->windows) return 0;
   size_t b = 0;
   for (unsigned w = 0; w < d->num_windows; ++w)
-    b += (size_t)d->windows[w].num_sites * d->num_samples * sizeof(float4);
+    b += 2 * (size_t)d->windows[w].num_sites * d->num_samples * sizeof(float4);
   return b;
 }
 
@@ -415,7 +416,7 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
   size_t o = 0;
   for (unsigned w = 0; w < d->num_windows; ++w) {
     off[w] = o;
-    o += (size_t)d->windows[w].num_sites * d->num_samples;
+    o += 2 * (size_t)d->windows[w].num_sites * d->num_samples;
   }
   // a copy from pageable memory has left the host buffer when it returns
   rc = hip_check(hipMemcpyAsync(dev, host, tbl + offb, hipMemcpyHostToDevice, s), "table copy");
@@ -431,7 +432,9 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
     unsigned max_wn = 0;
     for (unsigned w = 0; w < d->num_windows; ++w)
       max_wn = std::max(max_wn, d->windows[w].words_per_hap);
-    const dim3 grid((unsigned)((4ull * d->num_samples + 255) / 256), d->num_windows), block(256);
+    // two wavefronts (backward, forward) per 16 samples
+    const size_t waves = 2 * (((size_t)d->num_samples + 15) / 16);
+    const dim3 grid((unsigned)((64 * waves + 255) / 256), d->num_windows), block(256);
     const size_t words = (size_t)d->num_samples * max_wn;
     const dim3 grid2((unsigned)((words + 255) / 256), d->num_windows);
     const auto *wt = reinterpret_cast<const glp_work_window *>(dev);

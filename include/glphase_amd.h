@@ -252,8 +252,9 @@ typedef struct {
   void *workspace;                /* device, >= glp_work_workspace_bytes */
 } glp_work_desc;
 
-/* Device workspace the resampling needs: 16 B per sample and site of every
- * window (the scaled backward probabilities). */
+/* Device workspace the resampling needs: 32 B per sample and site of every
+ * window (the scaled backward and forward probabilities, computed side by
+ * side). */
 size_t glp_work_workspace_bytes(const glp_work_desc *desc);
 
 /* Impute::hmm_work for every sample of every window: scaled backward
