@@ -27,6 +27,7 @@ constexpr unsigned kKnnCountChunk = 128;  // haplotypes per allele-count block
 constexpr unsigned kKnnMaxHalf = 1024;   // ceil(K / 2) limit
 constexpr unsigned kKnnQpb = 32;         // queries per distance block
 constexpr unsigned kKnnBins = 2560;      // distance histogram bins per row
+constexpr unsigned kSelChunks = 16;      // 64-pair chunks of a row in flight per select pass
 constexpr unsigned kKnnMaxWords = 1024;  // 65,536 sites
 
 // Alt-allele counts of the 64 sites of word w over haplotypes [h0, h0 + kKnnCountChunk).
@@ -140,6 +141,8 @@ __device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k,
 //     it to the threshold distance t and the count of distances below t;
 //  2. one pass in index order keeps every d < t and the first `need` with
 //     d == t, placed by ballot prefix counts (so ties go by index);
+// each pass has kSelChunks 32-bit loads per lane in flight (the passes are
+// bound by the latency of the row's reads, not by their bytes);
 //  3. a rank sort of the k keys.
 __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
                                 unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
@@ -151,17 +154,17 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
   const unsigned np = H >> 1;
   for (unsigned b = lane; b < kKnnBins; b += 64) hist[b] = 0;
   wave_lds_sync();
-  for (unsigned base = 0; base < np; base += 256) {
-    uint32_t v[4];
-    bool ok[4];
+  for (unsigned base = 0; base < np; base += 64 * kSelChunks) {
+    uint32_t v[kSelChunks];
+    bool ok[kSelChunks];
 #pragma unroll
-    for (unsigned j = 0; j < 4; ++j) {
+    for (unsigned j = 0; j < kSelChunks; ++j) {
       const unsigned p = base + 64 * j + lane;
       ok[j] = p < np && p != u;
       v[j] = ok[j] ? row2[p] : 0u;
     }
 #pragma unroll
-    for (unsigned j = 0; j < 4; ++j)
+    for (unsigned j = 0; j < kSelChunks; ++j)
       if (ok[j]) {
         atomicAdd(&hist[(v[j] & 0xFFFFu) >> shift], 1u);
         atomicAdd(&hist[(v[j] >> 16) >> shift], 1u);
@@ -175,17 +178,17 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
     hist[lane] = 0;
     wave_lds_sync();
     const unsigned lo_mask = (1u << shift) - 1;
-    for (unsigned base = 0; base < np; base += 256) {
-      uint32_t v[4];
-      bool ok[4];
+    for (unsigned base = 0; base < np; base += 64 * kSelChunks) {
+      uint32_t v[kSelChunks];
+      bool ok[kSelChunks];
 #pragma unroll
-      for (unsigned j = 0; j < 4; ++j) {
+      for (unsigned j = 0; j < kSelChunks; ++j) {
         const unsigned p = base + 64 * j + lane;
         ok[j] = p < np && p != u;
         v[j] = ok[j] ? row2[p] : 0u;
       }
 #pragma unroll
-      for (unsigned j = 0; j < 4; ++j)
+      for (unsigned j = 0; j < kSelChunks; ++j)
         if (ok[j]) {
           const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
           if ((d0 >> shift) == t) atomicAdd(&hist[d0 & lo_mask], 1u);
@@ -202,18 +205,19 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
   const uint64_t lt = lanes_below();
   unsigned nless = 0, nties = 0;
   // index order: 64-pair chunks ascending, lanes ascending, element 2p
-  // before 2p + 1; four chunks' loads are in flight together
-  for (unsigned base = 0; base < np && (nless < below || nties < need); base += 256) {
-    uint32_t v[4];
-    bool ok[4];
+  // before 2p + 1; kSelChunks chunks' loads are in flight together
+  for (unsigned base = 0; base < np && (nless < below || nties < need);
+       base += 64 * kSelChunks) {
+    uint32_t v[kSelChunks];
+    bool ok[kSelChunks];
 #pragma unroll
-    for (unsigned j = 0; j < 4; ++j) {
+    for (unsigned j = 0; j < kSelChunks; ++j) {
       const unsigned p = base + 64 * j + lane;
       ok[j] = p < np && p != u;
       v[j] = ok[j] ? row2[p] : 0u;
     }
 #pragma unroll
-    for (unsigned j = 0; j < 4; ++j) {
+    for (unsigned j = 0; j < kSelChunks; ++j) {
       const unsigned p = base + 64 * j + lane;
       const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
       const bool l0 = ok[j] && d0 < t, l1 = ok[j] && d1 < t;
