@@ -3,8 +3,9 @@
 //
 // Three steps: per-site variant allele counts over the panel give the mask
 // of kept sites (CalculateVarAfs / CutDownHaps); a masked, word-major copy of
-// the panel feeds an all-pairs Hamming kernel that writes 16-bit distances
-// of a batch of query haplotypes (the samples' own) against the panel; one
+// the panel feeds an all-pairs Hamming kernel on the matrix cores that
+// writes 16-bit distances of a batch of query haplotypes (the samples' own)
+// against the panel; one
 // wavefront per query row then selects its neighbours from a histogram of
 // the row's distances, an index-ordered compaction of the ties at the
 // threshold and a rank sort by (distance, index), and one workgroup per
@@ -25,9 +26,10 @@ namespace glp {
 constexpr unsigned kKnnBatch = 4096;     // query haplotypes per distance batch
 constexpr unsigned kKnnCountChunk = 128;  // haplotypes per allele-count block
 constexpr unsigned kKnnMaxHalf = 1024;   // ceil(K / 2) limit
-constexpr unsigned kKnnQpb = 32;         // queries per distance block
 constexpr unsigned kKnnBins = 2560;      // distance histogram bins per row
 constexpr unsigned kSelChunks = 16;      // 64-pair chunks of a row in flight per select pass
+constexpr unsigned kKnnKc = 8;           // 64-site words per staged chunk of the distance kernel
+constexpr size_t kKnnDistLds = 2 * kKnnKc * 384 * 8;  // its two staging buffers (48 KB)
 constexpr unsigned kKnnMaxWords = 1024;  // 65,536 sites
 
 // Alt-allele counts of the 64 sites of word w over haplotypes [h0, h0 + kKnnCountChunk).
@@ -64,32 +66,170 @@ __global__ void __launch_bounds__(256)
   if (h < H) pt[(size_t)w * H + h] = panel[(size_t)h * wn + w] & mask[w];
 }
 
-// Distances of queries q0 + blockIdx.y * kKnnQpb + q (q < kKnnQpb, below
-// q0 + nq) to every haplotype h: d[(query - q0) * H + h].  A query's words are
-// the same for every lane, so they come through the scalar unit in one run of
-// kKnnQpb words; each lane streams its haplotype's words once per kKnnQpb
-// queries.  The last block's run may pass its last query, and the last word
-// row's may pass the panel's end: pt is followed by kKnnQpb words of padding.
+// Masked allele count of every haplotype: pc[h] = sum_w popcount(pt[w][h]).
 __global__ void __launch_bounds__(256)
-    k_knn_dist(const uint64_t *__restrict__ pt, unsigned H, unsigned wn, unsigned q0,
-               unsigned nq, uint16_t *__restrict__ d) {
-  const unsigned qb = blockIdx.y * kKnnQpb, last = nq - 1 - qb;  // grid: qb < nq
+    k_knn_popc(const uint64_t *__restrict__ pt, unsigned H, unsigned wn, unsigned *__restrict__ pc) {
   const unsigned h = blockIdx.x * 256u + threadIdx.x;
   if (h >= H) return;
-  unsigned acc[kKnnQpb];
+  const auto p = gmem(pt);
+  unsigned c = 0;
+  for (unsigned w = 0; w < wn; ++w) c += (unsigned)__popcll(p[(size_t)w * H + h]);
+  pc[h] = c;
+}
+
+// The distances as a matrix product on the matrix cores: with a, b the 0/1
+// vectors of two masked haplotypes, popcount(a ^ b) = |a| + |b| - 2 a.b, and
+// the a.b of a tile of queries x haplotypes is an int8 product accumulated
+// in int32 (v_mfma_i32_32x32x32_i8), exact.
+//  - A block covers 128 queries x 256 haplotypes; each of its 4 wavefronts
+//    owns 128 queries x 64 haplotypes (4 x 2 blocks of 32 x 32).
+//  - The block stages kKnnKc 64-site words of its 384 rows at a time in LDS
+//    with direct global-to-LDS loads (global_load_lds_dwordx4: 1 KB of the
+//    word-major panel per wavefront load), double-buffered: chunk c + 1 is in
+//    flight while chunk c's products run.
+//  - For a 32-row block, lane l reads the word of row l % 32 and expands its
+//    bits 16 (l / 32) .. + 15 and 32 + 16 (l / 32) .. + 15 into the two
+//    16-byte operands of the word's two 32-site steps: query operands
+//    through a 256-entry LDS table (byte -> 8 bytes of 0 / 1), haplotype
+//    operands by multiplies (a nibble n spreads to bytes by
+//    (n * 0x00204081) & 0x01010101), which splits the work between the LDS
+//    and VALU pipes.
+//  - Operand layout (lane l holds row l % 32, sites 16 (l / 32) .. + 15) and
+//    result layout (lane l, register r: row 8 (r / 4) + 4 (l / 32) + r % 4,
+//    column l % 32) are the instruction's.
+// Rows past the batch or the panel read a valid row (clamped) and are not
+// stored.
+typedef int knn_v4i __attribute__((ext_vector_type(4)));
+typedef int knn_v16i __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void gbl_void_t;
+
+__device__ __forceinline__ knn_v4i knn_expand_lut(const uint2 *lut, uint64_t w, unsigned off) {
+  const unsigned x = (unsigned)(w >> off);
+  const uint2 lo = lut[x & 255u], hi = lut[(x >> 8) & 255u];
+  knn_v4i v;
+  v.x = (int)lo.x;
+  v.y = (int)lo.y;
+  v.z = (int)hi.x;
+  v.w = (int)hi.y;
+  return v;
+}
+
+__device__ __forceinline__ knn_v4i knn_expand_mul(uint64_t w, unsigned off) {
+  const unsigned x = (unsigned)(w >> off);
+  knn_v4i v;
+  v.x = (int)((__builtin_amdgcn_ubfe(x, 0, 4) * 0x00204081u) & 0x01010101u);
+  v.y = (int)((__builtin_amdgcn_ubfe(x, 4, 4) * 0x00204081u) & 0x01010101u);
+  v.z = (int)((__builtin_amdgcn_ubfe(x, 8, 4) * 0x00204081u) & 0x01010101u);
+  v.w = (int)((__builtin_amdgcn_ubfe(x, 12, 4) * 0x00204081u) & 0x01010101u);
+  return v;
+}
+
+// Stage chunk c (words c kKnnKc .. + kKnnKc - 1, clamped into the window) of
+// the block's rows into buffer buf ([kKnnKc][384] words: query rows 0..127,
+// then haplotype rows 0..255).  Wavefront v issues loads t = kPer v .. kPer
+// v + kPer - 1: word t / 3, part t % 3 (queries, haplotypes 0..127, 128..255).
+__device__ __forceinline__ void knn_stage(uint64_t *buf, const uint64_t *pt, unsigned H,
+                                          unsigned wn, unsigned c, unsigned qrow, unsigned hrow) {
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  constexpr unsigned kPer = 3 * kKnnKc / 4;  // loads per wavefront
 #pragma unroll
-  for (unsigned q = 0; q < kKnnQpb; ++q) acc[q] = 0;
-  for (unsigned w = 0; w < wn; ++w) {
-    const uint64_t *row = pt + (size_t)w * H;
-    const uint64_t c = row[h];
-    const uint64_t *qw = row + q0 + qb;
-#pragma unroll
-    for (unsigned q = 0; q < kKnnQpb; ++q)
-      acc[q] += (unsigned)__popcll(c ^ qw[q]);
+  for (unsigned m = 0; m < kPer; ++m) {
+    const unsigned t = kPer * wave + m, wl = t / 3u, part = t % 3u;
+    unsigned w = c * kKnnKc + wl;
+    w = w < wn ? w : wn - 1;
+    const unsigned base = part == 0 ? qrow : hrow + 128u * (part - 1u);
+    unsigned r = base + 2u * lane;
+    r = r < H - 2u ? r : H - 2u;  // an even pair of rows inside the panel
+    __builtin_amdgcn_global_load_lds((gbl_void_t *)(pt + (size_t)w * H + r),
+                                     (lds_void_t *)(buf + wl * 384u + 128u * part), 16, 0, 0);
   }
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_knn_dist(const uint64_t *__restrict__ pt, const unsigned *__restrict__ pc, unsigned H,
+               unsigned wn, unsigned q0, unsigned nq, uint16_t *__restrict__ d) {
+  extern __shared__ __align__(16) uint64_t s_stage[];  // 2 x kKnnKc x 384 words
+  __shared__ uint2 lut[256];
+  __shared__ unsigned s_pa[128];  // |a| of the block's queries
+  const unsigned qt = blockIdx.y * 128u;          // first query of the block
+  const unsigned hb0 = blockIdx.x * 256u;         // first haplotype of the block
+  const unsigned nc = (wn + kKnnKc - 1) / kKnnKc;  // chunks
+  knn_stage(s_stage, pt, H, wn, 0, q0 + qt, hb0);
+  if (threadIdx.x < 128u) {
+    const unsigned q = qt + threadIdx.x;
+    s_pa[threadIdx.x] = pc[q0 + (q < nq ? q : nq - 1)];
+  }
+  for (unsigned b = threadIdx.x; b < 256u; b += blockDim.x) {
+    uint32_t lo = 0, hi = 0;
 #pragma unroll
-  for (unsigned q = 0; q < kKnnQpb; ++q)
-    if (q <= last) d[(size_t)(qb + q) * H + h] = (uint16_t)acc[q];
+    for (unsigned i = 0; i < 4; ++i) {
+      lo |= ((b >> i) & 1u) << (8 * i);
+      hi |= ((b >> (i + 4)) & 1u) << (8 * i);
+    }
+    lut[b] = make_uint2(lo, hi);
+  }
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const unsigned row = lane & 31u, half = lane >> 5, off = 16u * half;
+  const unsigned ht = hb0 + wave * 64u;  // first haplotype of the wavefront
+  unsigned pb[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const unsigned h = ht + 32u * j + row;
+    pb[j] = pc[h < H ? h : H - 1];
+  }
+  knn_v16i acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
+  for (unsigned c = 0; c < nc; ++c) {
+    // chunk c has landed (this wavefront's loads), every wavefront is past
+    // chunk c - 1 (so its buffer may take chunk c + 1)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (c + 1 < nc)
+      knn_stage(s_stage + ((c + 1) & 1u) * (kKnnKc * 384u), pt, H, wn, c + 1, q0 + qt, hb0);
+    const uint64_t *buf = s_stage + (c & 1u) * (kKnnKc * 384u);
+    const unsigned nw = wn - c * kKnnKc < kKnnKc ? wn - c * kKnnKc : kKnnKc;
+#pragma unroll 2
+    for (unsigned wl = 0; wl < nw; ++wl) {
+      const uint64_t *wr = buf + wl * 384u;
+      uint64_t wa[4], wb[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) wa[i] = wr[32u * i + row];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) wb[j] = wr[128u + 64u * wave + 32u * j + row];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        knn_v4i fa[4], fb[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa[i] = knn_expand_lut(lut, wa[i], off + 32u * s);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) fb[j] = knn_expand_mul(wb[j], off + 32u * s);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // d = |a| + |b| - 2 a.b; lane l holds column l % 32 of each block
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const unsigned h = ht + 32u * j + row;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const unsigned qi = 32u * i + 8u * (r >> 2) + 4u * half + (r & 3), q = qt + qi;
+        if (q < nq && h < H)
+          d[(size_t)q * H + h] = (uint16_t)(s_pa[qi] + pb[j] - 2u * (unsigned)acc[i][j][r]);
+      }
+  }
 }
 
 // Orders a wavefront's LDS accesses before and after this point (the
@@ -291,7 +431,7 @@ using namespace glp;
 namespace {
 
 struct KnnLayout {
-  size_t mask, counts, pt, dist, total;
+  size_t mask, counts, pt, pc, dist, total;
 };
 
 KnnLayout knn_layout(const glp_knn_desc *d) {
@@ -301,7 +441,8 @@ KnnLayout knn_layout(const glp_knn_desc *d) {
   L.mask = 0;
   L.counts = (wn * 8 + 255) & ~(size_t)255;
   L.pt = L.counts + ((64 * wn * 4 + 255) & ~(size_t)255);
-  L.dist = L.pt + (((wn * H + kKnnQpb) * 8 + 255) & ~(size_t)255);  // padding: k_knn_dist
+  L.pc = L.pt + ((wn * H * 8 + 255) & ~(size_t)255);
+  L.dist = L.pc + ((H * 4 + 255) & ~(size_t)255);
   L.total = L.dist + batch * H * 2;
   return L;
 }
@@ -344,6 +485,7 @@ extern "C" int glp_knn_build(const glp_knn_desc *d, void *stream) {
   uint64_t *mask = reinterpret_cast<uint64_t *>(ws + L.mask);
   unsigned *counts = reinterpret_cast<unsigned *>(ws + L.counts);
   uint64_t *pt = reinterpret_cast<uint64_t *>(ws + L.pt);
+  unsigned *pc = reinterpret_cast<unsigned *>(ws + L.pc);
   uint16_t *dist = reinterpret_cast<uint16_t *>(ws + L.dist);
   const unsigned kout = glp_knn_k_out(d->K, H), kh = kout / 2;
   const unsigned kb = std::min(2 * kh, H - 2);
@@ -356,6 +498,7 @@ extern "C" int glp_knn_build(const glp_knn_desc *d, void *stream) {
                      d->using_maf, mask);
   hipLaunchKernelGGL(k_knn_transpose, dim3((H + 255) / 256, wn), dim3(256), 0, s, d->panel, H, wn,
                      mask, pt);
+  hipLaunchKernelGGL(k_knn_popc, dim3((H + 255) / 256), dim3(256), 0, s, pt, H, wn, pc);
   rc = hip_check(hipGetLastError(), "kNN setup launch");
   // distances are at most M: bins of 2^shift distances keep them below kKnnBins
   unsigned shift = 0;
@@ -365,11 +508,15 @@ extern "C" int glp_knn_build(const glp_knn_desc *d, void *stream) {
     rc = hip_check(hipFuncSetAttribute((const void *)&k_knn_select,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                    "dynamic LDS attribute");
+  if (rc == GLP_OK)
+    rc = hip_check(hipFuncSetAttribute((const void *)&k_knn_dist,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kKnnDistLds),
+                   "dynamic LDS attribute");
   const unsigned per = kKnnBatch / 2;  // samples per batch
   for (unsigned u0 = 0; u0 < d->num_samples && rc == GLP_OK; u0 += per) {
     const unsigned nu = std::min(per, d->num_samples - u0), nq = 2 * nu;
-    hipLaunchKernelGGL(k_knn_dist, dim3((H + 255) / 256, (nq + kKnnQpb - 1) / kKnnQpb), dim3(256),
-                       0, s, pt, H, wn, 2 * u0, nq, dist);
+    hipLaunchKernelGGL(k_knn_dist, dim3((H + 255) / 256, (nq + 127) / 128), dim3(256),
+                       kKnnDistLds, s, pt, pc, H, wn, 2 * u0, nq, dist);
     hipLaunchKernelGGL(k_knn_select, dim3(nu), dim3(128), lds, s, dist, H, u0, kh, kb, kout,
                        shift, d->klist);
     rc = hip_check(hipGetLastError(), "kNN batch launch");
