@@ -43,10 +43,11 @@ def test_knn_matches_oracle(orc, H, M, nsamp, K, lb, ub, maf):
     say(f"kNN H={H} M={M} K={K}")
     rng = np.random.default_rng(H * M)
     wn = (M + 63) // 64
-    base = rng.integers(0, 2**63, (6, wn), dtype=np.uint64)
-    noise = (rng.integers(0, 2**63, (H, wn), dtype=np.uint64)
-             & rng.integers(0, 2**63, (H, wn), dtype=np.uint64)
-             & rng.integers(0, 2**63, (H, wn), dtype=np.uint64))
+    # full 64-bit words: every bit position of a word, bit 63 included
+    base = rng.integers(0, 2**64, (6, wn), dtype=np.uint64)
+    noise = (rng.integers(0, 2**64, (H, wn), dtype=np.uint64)
+             & rng.integers(0, 2**64, (H, wn), dtype=np.uint64)
+             & rng.integers(0, 2**64, (H, wn), dtype=np.uint64))
     haps = base[rng.integers(0, 6, H)] ^ noise
     if M % 64:
         haps[:, -1] &= np.uint64((1 << (M % 64)) - 1)
