@@ -41,6 +41,11 @@ __device__ __forceinline__ float4 load_f4(gptr_t<float> p) {
   const glp_f4v v = *reinterpret_cast<gptr_t<glp_f4v>>(p);
   return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef unsigned glp_u4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 load_u4(gptr_t<uint32_t> p) {
+  const glp_u4v v = *reinterpret_cast<gptr_t<glp_u4v>>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ void store_f4(float *p, float a, float b, float c, float d) {
   glp_f4v v;
   v.x = a;

@@ -27,7 +27,7 @@ constexpr unsigned kKnnBatch = 4096;     // query haplotypes per distance batch
 constexpr unsigned kKnnCountChunk = 128;  // haplotypes per allele-count block
 constexpr unsigned kKnnMaxHalf = 1024;   // ceil(K / 2) limit
 constexpr unsigned kKnnBins = 2560;      // distance histogram bins per row
-constexpr unsigned kSelChunks = 16;      // 64-pair chunks of a row in flight per select pass
+constexpr unsigned kSelVecs = 8;         // 16-byte reads per lane in flight per select step
 constexpr unsigned kKnnKc = 8;           // 64-site words per staged chunk of the distance kernel
 constexpr size_t kKnnDistLds = 2 * kKnnKc * 384 * 8;  // its two staging buffers (48 KB)
 constexpr unsigned kKnnMaxWords = 1024;  // 65,536 sites
@@ -99,6 +99,12 @@ __global__ void __launch_bounds__(256)
 //    column l % 32) are the instruction's.
 // Rows past the batch or the panel read a valid row (clamped) and are not
 // stored.
+// Row stride of the distance block: H rounded up to 8 distances, so a row
+// starts 16-byte aligned (the select kernel reads 8 distances at a time).
+__host__ __device__ __forceinline__ size_t knn_row_stride(unsigned H) {
+  return ((size_t)H + 7) & ~(size_t)7;
+}
+
 typedef int knn_v4i __attribute__((ext_vector_type(4)));
 typedef int knn_v16i __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -227,7 +233,8 @@ __global__ void __launch_bounds__(256, 2)
       for (int r = 0; r < 16; ++r) {
         const unsigned qi = 32u * i + 8u * (r >> 2) + 4u * half + (r & 3), q = qt + qi;
         if (q < nq && h < H)
-          d[(size_t)q * H + h] = (uint16_t)(s_pa[qi] + pb[j] - 2u * (unsigned)acc[i][j][r]);
+          d[(size_t)q * knn_row_stride(H) + h] =
+              (uint16_t)(s_pa[qi] + pb[j] - 2u * (unsigned)acc[i][j][r]);
       }
   }
 }
@@ -272,107 +279,135 @@ __device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k,
   return (unsigned)__shfl((int)bin, (int)src, 64);
 }
 
-// The k nearest haplotypes of row dr (length H; haplotypes 2u and 2u + 1
-// excluded) by (distance, index), as keys (d << 32 | h) sorted into out[0, k).
-// One wavefront per row, no workgroup barrier:
+// The k nearest haplotypes of row dr (H distances, padded to a multiple of
+// 8; haplotypes 2u and 2u + 1 excluded) by (distance, index), as keys
+// (d << 32 | h) sorted into out[0, k).  One wavefront per row, no workgroup
+// barrier:
 //  1. a histogram of d >> shift (exact distances when shift = 0) in the
 //     wavefront's own kKnnBins counters gives the threshold bin; with
 //     shift > 0, a second histogram of the low bits inside that bin refines
 //     it to the threshold distance t and the count of distances below t;
 //  2. one pass in index order keeps every d < t and the first `need` with
 //     d == t, placed by ballot prefix counts (so ties go by index);
-// each pass has kSelChunks 32-bit loads per lane in flight (the passes are
-// bound by the latency of the row's reads, not by their bytes);
 //  3. a rank sort of the k keys.
+// A lane reads 8 distances (16 bytes) at a time and has kSelVecs such reads
+// in flight per step of a pass: the passes are bound by the latency of the
+// row's reads, not by their bytes.
+__device__ __forceinline__ unsigned knn_elem(const uint4 &v, unsigned t) {
+  const unsigned w = t < 2 ? v.x : t < 4 ? v.y : t < 6 ? v.z : v.w;
+  return (t & 1u) ? w >> 16 : w & 0xFFFFu;
+}
+
 __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
                                 unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
                                 uint64_t *out) {
   const unsigned lane = threadIdx.x & 63u;
-  // distances 2p, 2p + 1 as one 32-bit word (H is even, so rows are 4-byte
-  // aligned); pair u is the sample's own two haplotypes, which never count
-  const auto row2 = gmem(reinterpret_cast<const uint32_t *>(dr));
-  const unsigned np = H >> 1;
+  const auto row = gmem(reinterpret_cast<const uint32_t *>(dr));
+  const unsigned nv = (H + 7u) >> 3;  // 8-distance vectors of the row
+  // element e of vector p: 8 p + t, kept when inside the row and not one of
+  // the sample's own haplotypes
+  auto keep = [&](unsigned p, unsigned t) {
+    const unsigned e = 8u * p + t;
+    return e < H && (e >> 1) != u;
+  };
   for (unsigned b = lane; b < kKnnBins; b += 64) hist[b] = 0;
   wave_lds_sync();
-  for (unsigned base = 0; base < np; base += 64 * kSelChunks) {
-    uint32_t v[kSelChunks];
-    bool ok[kSelChunks];
+  for (unsigned base = 0; base < nv; base += 64 * kSelVecs) {
+    uint4 v[kSelVecs];
 #pragma unroll
-    for (unsigned j = 0; j < kSelChunks; ++j) {
+    for (unsigned j = 0; j < kSelVecs; ++j) {
       const unsigned p = base + 64 * j + lane;
-      ok[j] = p < np && p != u;
-      v[j] = ok[j] ? row2[p] : 0u;
+      v[j] = p < nv ? load_u4(row + 4 * p) : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
-    for (unsigned j = 0; j < kSelChunks; ++j)
-      if (ok[j]) {
-        atomicAdd(&hist[(v[j] & 0xFFFFu) >> shift], 1u);
-        atomicAdd(&hist[(v[j] >> 16) >> shift], 1u);
+    for (unsigned j = 0; j < kSelVecs; ++j) {
+      const unsigned p = base + 64 * j + lane;
+      if (p < nv) {
+#pragma unroll
+        for (unsigned t = 0; t < 8; ++t)
+          if (keep(p, t)) atomicAdd(&hist[knn_elem(v[j], t) >> shift], 1u);
       }
+    }
   }
   wave_lds_sync();
   unsigned below;
-  unsigned t = wave_find_bin(hist, kKnnBins, k, &below);
+  unsigned thr = wave_find_bin(hist, kKnnBins, k, &below);
   if (shift) {
     wave_lds_sync();
     hist[lane] = 0;
     wave_lds_sync();
     const unsigned lo_mask = (1u << shift) - 1;
-    for (unsigned base = 0; base < np; base += 64 * kSelChunks) {
-      uint32_t v[kSelChunks];
-      bool ok[kSelChunks];
+    for (unsigned base = 0; base < nv; base += 64 * kSelVecs) {
+      uint4 v[kSelVecs];
 #pragma unroll
-      for (unsigned j = 0; j < kSelChunks; ++j) {
+      for (unsigned j = 0; j < kSelVecs; ++j) {
         const unsigned p = base + 64 * j + lane;
-        ok[j] = p < np && p != u;
-        v[j] = ok[j] ? row2[p] : 0u;
+        v[j] = p < nv ? load_u4(row + 4 * p) : make_uint4(0, 0, 0, 0);
       }
 #pragma unroll
-      for (unsigned j = 0; j < kSelChunks; ++j)
-        if (ok[j]) {
-          const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
-          if ((d0 >> shift) == t) atomicAdd(&hist[d0 & lo_mask], 1u);
-          if ((d1 >> shift) == t) atomicAdd(&hist[d1 & lo_mask], 1u);
+      for (unsigned j = 0; j < kSelVecs; ++j) {
+        const unsigned p = base + 64 * j + lane;
+        if (p < nv) {
+#pragma unroll
+          for (unsigned t = 0; t < 8; ++t) {
+            const unsigned dv = knn_elem(v[j], t);
+            if (keep(p, t) && (dv >> shift) == thr) atomicAdd(&hist[dv & lo_mask], 1u);
+          }
         }
+      }
     }
     wave_lds_sync();
     unsigned below_lo;
     const unsigned lo = wave_find_bin(hist, 64, k - below, &below_lo);
-    t = (t << shift) | lo;
+    thr = (thr << shift) | lo;
     below += below_lo;
   }
-  const unsigned need = k - below;  // ties at t to take, lowest indices first
+  const unsigned need = k - below;  // ties at thr to take, lowest indices first
   const uint64_t lt = lanes_below();
   unsigned nless = 0, nties = 0;
-  // index order: 64-pair chunks ascending, lanes ascending, element 2p
-  // before 2p + 1; kSelChunks chunks' loads are in flight together
-  for (unsigned base = 0; base < np && (nless < below || nties < need);
-       base += 64 * kSelChunks) {
-    uint32_t v[kSelChunks];
-    bool ok[kSelChunks];
+  // index order: 512-element steps ascending, lanes ascending, elements
+  // 8 p .. 8 p + 7 of a lane in order
+  for (unsigned base = 0; base < nv && (nless < below || nties < need);
+       base += 64 * kSelVecs) {
+    uint4 v[kSelVecs];
 #pragma unroll
-    for (unsigned j = 0; j < kSelChunks; ++j) {
+    for (unsigned j = 0; j < kSelVecs; ++j) {
       const unsigned p = base + 64 * j + lane;
-      ok[j] = p < np && p != u;
-      v[j] = ok[j] ? row2[p] : 0u;
+      v[j] = p < nv ? load_u4(row + 4 * p) : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
-    for (unsigned j = 0; j < kSelChunks; ++j) {
+    for (unsigned j = 0; j < kSelVecs; ++j) {
       const unsigned p = base + 64 * j + lane;
-      const unsigned d0 = v[j] & 0xFFFFu, d1 = v[j] >> 16;
-      const bool l0 = ok[j] && d0 < t, l1 = ok[j] && d1 < t;
-      const bool e0 = ok[j] && d0 == t, e1 = ok[j] && d1 == t;
-      const uint64_t bl0 = __ballot(l0), bl1 = __ballot(l1);
-      const uint64_t be0 = __ballot(e0), be1 = __ballot(e1);
-      const unsigned pl = nless + (unsigned)__popcll(bl0 & lt) + (unsigned)__popcll(bl1 & lt);
-      if (l0) tmp[pl] = ((uint64_t)d0 << 32) | (2u * p);
-      if (l1) tmp[pl + (l0 ? 1u : 0u)] = ((uint64_t)d1 << 32) | (2u * p + 1u);
-      const unsigned pe = nties + (unsigned)__popcll(be0 & lt) + (unsigned)__popcll(be1 & lt);
-      if (e0 && pe < need) tmp[below + pe] = ((uint64_t)d0 << 32) | (2u * p);
-      const unsigned pe1 = pe + (e0 ? 1u : 0u);
-      if (e1 && pe1 < need) tmp[below + pe1] = ((uint64_t)d1 << 32) | (2u * p + 1u);
-      nless += (unsigned)(__popcll(bl0) + __popcll(bl1));
-      nties += (unsigned)(__popcll(be0) + __popcll(be1));
+      unsigned ml = 0, me = 0;  // this lane's elements below / at the threshold
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t) {
+        const unsigned dv = knn_elem(v[j], t);
+        const bool ok = p < nv && keep(p, t);
+        ml |= (ok && dv < thr) ? 1u << t : 0u;
+        me |= (ok && dv == thr) ? 1u << t : 0u;
+      }
+      // elements of lower lanes come first
+      unsigned cl = 0, ce = 0, tl = 0, te = 0;
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t) {
+        const uint64_t bl = __ballot((ml >> t) & 1u), be = __ballot((me >> t) & 1u);
+        cl += (unsigned)__popcll(bl & lt);
+        ce += (unsigned)__popcll(be & lt);
+        tl += (unsigned)__popcll(bl);
+        te += (unsigned)__popcll(be);
+      }
+      unsigned pl = nless + cl, pe = nties + ce;
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t) {
+        const uint64_t key = ((uint64_t)knn_elem(v[j], t) << 32) | (8u * p + t);
+        if ((ml >> t) & 1u) tmp[pl++] = key;
+        if ((me >> t) & 1u) {
+          if (pe < need) tmp[below + pe] = key;
+          ++pe;
+        }
+      }
+      nless += tl;
+      nties += te;
     }
   }
   wave_lds_sync();
@@ -397,7 +432,7 @@ __global__ void __launch_bounds__(128)
   uint64_t *a_tmp = s_keys, *a_out = a_tmp + kh, *b_tmp = a_out + kh, *b_out = b_tmp + kb;
   unsigned *hist = reinterpret_cast<unsigned *>(b_out + kb);
   const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, u = u0 + blockIdx.x;
-  const uint16_t *row = d + (size_t)(2 * blockIdx.x + wave) * H;
+  const uint16_t *row = d + (size_t)(2 * blockIdx.x + wave) * knn_row_stride(H);
   if (wave == 0)
     knn_select_wave(row, H, u, kh, shift, hist, a_tmp, a_out);
   else
@@ -443,7 +478,7 @@ KnnLayout knn_layout(const glp_knn_desc *d) {
   L.pt = L.counts + ((64 * wn * 4 + 255) & ~(size_t)255);
   L.pc = L.pt + ((wn * H * 8 + 255) & ~(size_t)255);
   L.dist = L.pc + ((H * 4 + 255) & ~(size_t)255);
-  L.total = L.dist + batch * H * 2;
+  L.total = L.dist + batch * knn_row_stride((unsigned)H) * 2;
   return L;
 }
 

@@ -38,7 +38,11 @@ def test_knn_reference_known_answer():
 @pytest.mark.parametrize("H,M,nsamp,K,lb,ub,maf", [(40, 100, 20, 10, 0.0, 1.0, False),
                                                    (300, 700, 150, 40, 0.05, 0.5, True),
                                                    (64, 64, 10, 61, 0.0, 1.0, False),
-                                                   (5000, 1000, 1200, 200, 0.01, 1.0, False)])
+                                                   (5000, 1000, 1200, 200, 0.01, 1.0, False),
+                                                   # M >= 2,560: histogram bins of 2 distances,
+                                                   # then the refining histogram in the bin
+                                                   (600, 3000, 300, 30, 0.0, 1.0, False),
+                                                   (402, 9000, 201, 24, 0.0, 1.0, True)])
 def test_knn_matches_oracle(orc, H, M, nsamp, K, lb, ub, maf):
     say(f"kNN H={H} M={M} K={K}")
     rng = np.random.default_rng(H * M)
