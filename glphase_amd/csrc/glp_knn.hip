@@ -281,8 +281,8 @@ __device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k,
 
 // The k nearest haplotypes of row dr (H distances, padded to a multiple of
 // 8; haplotypes 2u and 2u + 1 excluded) by (distance, index), as keys
-// (d << 32 | h) sorted into out[0, k).  One wavefront per row, no workgroup
-// barrier:
+// (d << 32 | h) sorted into out[0, k), over the whole row (the fallback of
+// knn_select_wave).  One wavefront per row, no workgroup barrier:
 //  1. a histogram of d >> shift (exact distances when shift = 0) in the
 //     wavefront's own kKnnBins counters gives the threshold bin; with
 //     shift > 0, a second histogram of the low bits inside that bin refines
@@ -298,7 +298,7 @@ __device__ __forceinline__ unsigned knn_elem(const uint4 &v, unsigned t) {
   return (t & 1u) ? w >> 16 : w & 0xFFFFu;
 }
 
-__device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
+__device__ void knn_select_full(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
                                 unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
                                 uint64_t *out) {
   const unsigned lane = threadIdx.x & 63u;
@@ -409,6 +409,157 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
       nless += tl;
       nties += te;
     }
+  }
+  wave_lds_sync();
+  for (unsigned i = lane; i < k; i += 64) {
+    const uint64_t key = tmp[i];
+    unsigned r = 0;
+    for (unsigned j = 0; j < k; ++j) r += tmp[j] < key;
+    out[r] = key;
+  }
+  wave_lds_sync();
+}
+
+// Sum over the wavefront.
+__device__ __forceinline__ unsigned wave_sum(unsigned x) {
+  for (unsigned o = 32; o; o >>= 1) x += (unsigned)__shfl_xor((int)x, (int)o, 64);
+  return x;
+}
+
+constexpr unsigned kKnnCand = kKnnBins / 2;  // candidate keys the histogram's LDS can hold
+
+// The same selection, usually from one pass over the row:
+//  1. a histogram of the distances of every 8th 8-distance vector (an
+//     eighth of the row) gives a bound T with about twice k of the row's
+//     distances at or below it (the sample's rank 2 ceil(k ns / nk) + 16);
+//  2. one pass in index order compacts every kept distance <= T into the
+//     histogram's memory (at most kKnnCand keys; a step without one costs a
+//     ballot);
+//  3. if there are at least k of them (and they fit), the k-th smallest
+//     distance t comes from a binary search over [0, T] with wavefront
+//     counts, then the keys with d < t and the first k - below with d == t
+//     (still in index order) are rank sorted.  Every key the exact
+//     selection keeps has d <= t <= T, so the result is the same;
+// otherwise (too few or too many candidates) it runs knn_select_full.
+__device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
+                                unsigned k, unsigned shift, unsigned *hist, uint64_t *tmp,
+                                uint64_t *out) {
+  const unsigned lane = threadIdx.x & 63u;
+  const auto row = gmem(reinterpret_cast<const uint32_t *>(dr));
+  const unsigned nv = (H + 7u) >> 3;
+  auto keep = [&](unsigned p, unsigned t) {
+    const unsigned e = 8u * p + t;
+    return e < H && (e >> 1) != u;
+  };
+  // 1. the sample
+  for (unsigned b = lane; b < kKnnBins; b += 64) hist[b] = 0;
+  wave_lds_sync();
+  unsigned ns = 0;
+  const unsigned nsv = (nv + 7u) >> 3;  // sampled vectors: p = 8 m
+  for (unsigned base = 0; base < nsv; base += 64 * kSelVecs) {
+    uint4 v[kSelVecs];
+#pragma unroll
+    for (unsigned j = 0; j < kSelVecs; ++j) {
+      const unsigned m = base + 64 * j + lane;
+      v[j] = m < nsv ? load_u4(row + 32 * m) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (unsigned j = 0; j < kSelVecs; ++j) {
+      const unsigned m = base + 64 * j + lane;
+      if (m < nsv) {
+#pragma unroll
+        for (unsigned t = 0; t < 8; ++t)
+          if (keep(8u * m, t)) {
+            atomicAdd(&hist[knn_elem(v[j], t) >> shift], 1u);
+            ++ns;
+          }
+      }
+    }
+  }
+  ns = wave_sum(ns);
+  wave_lds_sync();
+  const unsigned nk = H - 2;  // kept distances of the row
+  const unsigned rs = 2u * (unsigned)(((uint64_t)k * ns + nk - 1) / nk) + 16u;
+  if (rs > ns) {  // a sample too small to bound anything
+    knn_select_full(dr, H, u, k, shift, hist, tmp, out);
+    return;
+  }
+  unsigned below_s;
+  const unsigned tb = wave_find_bin(hist, kKnnBins, rs, &below_s);
+  const unsigned T = (tb << shift) | ((1u << shift) - 1u);
+  wave_lds_sync();
+  // 2. every kept d <= T, in index order
+  uint64_t *cand = reinterpret_cast<uint64_t *>(hist);
+  const uint64_t lt = lanes_below();
+  unsigned cnt = 0;
+  for (unsigned base = 0; base < nv; base += 64 * kSelVecs) {
+    uint4 v[kSelVecs];
+#pragma unroll
+    for (unsigned j = 0; j < kSelVecs; ++j) {
+      const unsigned p = base + 64 * j + lane;
+      v[j] = p < nv ? load_u4(row + 4 * p) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (unsigned j = 0; j < kSelVecs; ++j) {
+      const unsigned p = base + 64 * j + lane;
+      unsigned mc = 0;
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t)
+        mc |= (p < nv && keep(p, t) && knn_elem(v[j], t) <= T) ? 1u << t : 0u;
+      if (__ballot(mc != 0u) == 0ull) continue;  // wavefront-uniform
+      unsigned cl = 0, tl = 0;
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t) {
+        const uint64_t bt = __ballot((mc >> t) & 1u);
+        cl += (unsigned)__popcll(bt & lt);
+        tl += (unsigned)__popcll(bt);
+      }
+      unsigned pos = cnt + cl;
+#pragma unroll
+      for (unsigned t = 0; t < 8; ++t)
+        if ((mc >> t) & 1u) {
+          if (pos < kKnnCand) cand[pos] = ((uint64_t)knn_elem(v[j], t) << 32) | (8u * p + t);
+          ++pos;
+        }
+      cnt += tl;
+    }
+  }
+  wave_lds_sync();
+  if (cnt < k || cnt > kKnnCand) {
+    knn_select_full(dr, H, u, k, shift, hist, tmp, out);
+    return;
+  }
+  // 3. the k-th smallest distance t among the candidates: the least t with
+  // count(d <= t) >= k
+  unsigned lo = 0, hi = T;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) >> 1;
+    unsigned c = 0;
+    for (unsigned i = lane; i < cnt; i += 64) c += (unsigned)(cand[i] >> 32) <= mid;
+    if (wave_sum(c) >= k)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  const unsigned thr = lo;
+  unsigned below = 0;
+  for (unsigned i = lane; i < cnt; i += 64) below += (unsigned)(cand[i] >> 32) < thr;
+  below = wave_sum(below);
+  const unsigned need = k - below;
+  // keys with d < thr, then the first `need` with d == thr, in index order
+  unsigned nless = 0, nties = 0;
+  for (unsigned base = 0; base < cnt; base += 64) {
+    const unsigned i = base + lane;
+    const uint64_t key = i < cnt ? cand[i] : ~0ull;
+    const unsigned dv = (unsigned)(key >> 32);
+    const bool l = i < cnt && dv < thr, e = i < cnt && dv == thr;
+    const uint64_t bl = __ballot(l), be = __ballot(e);
+    const unsigned pl = nless + (unsigned)__popcll(bl & lt);
+    const unsigned pe = nties + (unsigned)__popcll(be & lt);
+    if (l) tmp[pl] = key;
+    if (e && pe < need) tmp[below + pe] = key;
+    nless += (unsigned)__popcll(bl);
+    nties += (unsigned)__popcll(be);
   }
   wave_lds_sync();
   for (unsigned i = lane; i < k; i += 64) {
