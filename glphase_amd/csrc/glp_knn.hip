@@ -8,7 +8,7 @@
 // against the panel; one
 // wavefront per query row then selects its neighbours from a histogram of
 // the row's distances, an index-ordered compaction of the ties at the
-// threshold and a rank sort by (distance, index), and one workgroup per
+// threshold and a bitonic sort by (distance, index), and one workgroup per
 // sample interleaves A's and B's lists as ClusterHaps does (kNN.cpp:176-199).
 #include <hip/hip_runtime.h>
 
@@ -279,6 +279,34 @@ __device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k,
   return (unsigned)__shfl((int)bin, (int)src, 64);
 }
 
+// Sorts the k (distinct) keys of tmp[0, k) into out[0, k), where out =
+// tmp + k: a bitonic sort of tmp[0, P), P = k rounded up to a power of two
+// (at most 2k, so the padding lies in out), padded with the largest key;
+// then the first k move to out.  The keys are distinct (their low word is
+// the haplotype), so the order is the rank order by (distance, index).
+__device__ void knn_sort_keys(uint64_t *tmp, unsigned k, uint64_t *out) {
+  const unsigned lane = threadIdx.x & 63u;
+  unsigned P = 1;
+  while (P < k) P <<= 1;
+  for (unsigned i = k + lane; i < P; i += 64) tmp[i] = ~0ull;
+  wave_lds_sync();
+  for (unsigned size = 2; size <= P; size <<= 1)
+    for (unsigned stride = size >> 1; stride > 0; stride >>= 1) {
+      for (unsigned i = lane; i < P / 2; i += 64) {
+        const unsigned a = 2 * stride * (i / stride) + (i % stride), b = a + stride;
+        const bool up = (a & size) == 0;
+        const uint64_t ka = tmp[a], kb = tmp[b];
+        if ((ka > kb) == up) {
+          tmp[a] = kb;
+          tmp[b] = ka;
+        }
+      }
+      wave_lds_sync();
+    }
+  for (unsigned i = lane; i < k; i += 64) out[i] = tmp[i];
+  wave_lds_sync();
+}
+
 // The k nearest haplotypes of row dr (H distances, padded to a multiple of
 // 8; haplotypes 2u and 2u + 1 excluded) by (distance, index), as keys
 // (d << 32 | h) sorted into out[0, k), over the whole row (the fallback of
@@ -289,7 +317,7 @@ __device__ unsigned wave_find_bin(const unsigned *hist, unsigned nb, unsigned k,
 //     it to the threshold distance t and the count of distances below t;
 //  2. one pass in index order keeps every d < t and the first `need` with
 //     d == t, placed by ballot prefix counts (so ties go by index);
-//  3. a rank sort of the k keys.
+//  3. a sort of the k keys (knn_sort_keys).
 // A lane reads 8 distances (16 bytes) at a time and has kSelVecs such reads
 // in flight per step of a pass: the passes are bound by the latency of the
 // row's reads, not by their bytes.
@@ -411,13 +439,7 @@ __device__ void knn_select_full(const uint16_t *__restrict__ dr, unsigned H, uns
     }
   }
   wave_lds_sync();
-  for (unsigned i = lane; i < k; i += 64) {
-    const uint64_t key = tmp[i];
-    unsigned r = 0;
-    for (unsigned j = 0; j < k; ++j) r += tmp[j] < key;
-    out[r] = key;
-  }
-  wave_lds_sync();
+  knn_sort_keys(tmp, k, out);
 }
 
 // Sum over the wavefront.
@@ -438,7 +460,7 @@ constexpr unsigned kKnnCand = kKnnBins / 2;  // candidate keys the histogram's L
 //  3. if there are at least k of them (and they fit), the k-th smallest
 //     distance t comes from a binary search over [0, T] with wavefront
 //     counts, then the keys with d < t and the first k - below with d == t
-//     (still in index order) are rank sorted.  Every key the exact
+//     (still in index order) are sorted.  Every key the exact
 //     selection keeps has d <= t <= T, so the result is the same;
 // otherwise (too few or too many candidates) it runs knn_select_full.
 __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, unsigned u,
@@ -562,13 +584,7 @@ __device__ void knn_select_wave(const uint16_t *__restrict__ dr, unsigned H, uns
     nties += (unsigned)__popcll(be);
   }
   wave_lds_sync();
-  for (unsigned i = lane; i < k; i += 64) {
-    const uint64_t key = tmp[i];
-    unsigned r = 0;
-    for (unsigned j = 0; j < k; ++j) r += tmp[j] < key;
-    out[r] = key;
-  }
-  wave_lds_sync();
+  knn_sort_keys(tmp, k, out);
 }
 
 // One workgroup of two wavefronts per sample u in [u0, u0 + nu): wavefront 0
@@ -592,6 +608,20 @@ __global__ void __launch_bounds__(128)
   if (wave) return;
   unsigned *out = klist + (size_t)u * kout;
   for (unsigned j = lane; j < kh; j += 64) out[2 * j] = (unsigned)a_out[j];
+  // A's haplotypes in an open-addressing set (the histograms' LDS, free
+  // now): at most half full, so a probe takes a step or two
+  unsigned *set = hist;
+  unsigned ts = 2;
+  while (ts < 2 * kh) ts <<= 1;  // <= 2 kKnnMaxHalf = 2048 <= kKnnBins
+  const unsigned tmask = ts - 1, tshift = 32u - (unsigned)__builtin_ctz(ts);
+  for (unsigned j = lane; j < ts; j += 64) set[j] = 0xFFFFFFFFu;
+  wave_lds_sync();
+  for (unsigned j = lane; j < kh; j += 64) {
+    const unsigned idx = (unsigned)a_out[j];
+    unsigned slot = (idx * 2654435761u) >> tshift;
+    while (atomicCAS(&set[slot], 0xFFFFFFFFu, idx) != 0xFFFFFFFFu) slot = (slot + 1) & tmask;
+  }
+  wave_lds_sync();
   const uint64_t lt = lanes_below();
   unsigned placed = 0;
   for (unsigned base = 0; base < kb && placed < kh; base += 64) {
@@ -600,8 +630,9 @@ __global__ void __launch_bounds__(128)
     unsigned idx = 0;
     if (i < kb) {
       idx = (unsigned)b_out[i];
-      fresh = true;
-      for (unsigned j = 0; j < kh && fresh; ++j) fresh = (unsigned)a_out[j] != idx;
+      unsigned slot = (idx * 2654435761u) >> tshift, v;
+      while ((v = set[slot]) != idx && v != 0xFFFFFFFFu) slot = (slot + 1) & tmask;
+      fresh = v != idx;
     }
     const uint64_t bal = __ballot(fresh);
     const unsigned pos = placed + (unsigned)__popcll(bal & lt);
