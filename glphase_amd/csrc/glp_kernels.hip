@@ -115,25 +115,28 @@ __device__ __forceinline__ float dpp(float x) {
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
-// State of one lane: its two probabilities, and the record of rescale
-// factors (impute.cpp:367-372).  The factor of a site that does not rescale
-// is 1.0f, so a rescale is a select rather than a branch and the probability
-// update (x * 1.0f == x) is exact either way.
-//   F32 (the reference's arithmetic): the factors go to a per-lane LDS ring
-//   and their correctly rounded logs are subtracted, oldest first, at the end
-//   of each 32-site chunk -- the same float sum in the same order.
-//   MIXED: the factors are multiplied into an FP64 product whose binary
-//   exponent is split off every 8 sites, and the score is
-//   log(last sum) - (log(mantissa) + exponent * ln 2) in FP64.
+// State of one lane: its two probabilities, and the record of rescales
+// (impute.cpp:367-372).
+//   F32 (the reference's arithmetic): a site whose sum falls below norm
+//   multiplies its probabilities by 1/sum (the factor of a site that does
+//   not rescale is 1.0f, so a rescale is a select rather than a branch and
+//   x * 1.0f == x keeps the others exact); the factors go to a per-lane LDS
+//   ring and their correctly rounded logs are subtracted, oldest first, at
+//   the end of each 32-site chunk -- the same float sum in the same order.
+//   MIXED: the FP32 recurrence rescales at the same sites, by the exact
+//   power of two 2^84 (norm = FLT_MIN^(2/3) = 2^-84, so the rescaled sum
+//   stays below 1), which rounds nothing; the lane counts its rescales and
+//   the score is log(last sum) - count * 84 ln 2 in FP64.
 struct PairState {
   float la, lb;
-  double prod;
-  int pexp;
+  unsigned nres;  // MIXED: rescales so far
   float fscore;
   unsigned nq;
 };
 
 constexpr double kLn2 = 0.6931471805599453;
+constexpr float kRescale = 0x1p84f;            // MIXED rescale factor
+constexpr double kRescaleLog = 84.0 * kLn2;    // its natural log, in FP64
 
 // One site (impute.cpp:354-372) for the lane's two states; e = this lane's
 // two emissions.  ring: this lane's first ring slot (stride 64 floats).
@@ -152,12 +155,15 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   s = dpp<kDppOdd>(nb) + s;   // + l10
   s = dpp<kDppEven>(nb) + s;  // + l11
   const bool r = s < nrm;
-  const float sc = r ? (FAST ? recip_normal(s) : recip_small(s)) : 1.0f;
-  st.la = na * sc;
-  st.lb = nb * sc;
   if (MIXED) {
-    st.prod *= (double)sc;
+    const float sc = r ? kRescale : 1.0f;
+    st.la = na * sc;
+    st.lb = nb * sc;
+    st.nres += r ? 1u : 0u;
   } else {
+    const float sc = r ? (FAST ? recip_normal(s) : recip_small(s)) : 1.0f;
+    st.la = na * sc;
+    st.lb = nb * sc;
     ring[64u * st.nq] = sc;
     st.nq += r ? 1u : 0u;
   }
@@ -233,12 +239,6 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
     }
-    if (MIXED) {
-      // at most 8 factors below 2^127 since the last split: no overflow
-      const int ex = __builtin_amdgcn_frexp_exp(st.prod);
-      st.prod = __builtin_amdgcn_frexp_mant(st.prod);
-      st.pexp += ex;
-    }
   }
 }
 
@@ -276,8 +276,7 @@ __device__ __forceinline__ void group_load_b(GroupLds &G, const uint32_t (&cw)[B
   }
 }
 
-// Sites k0 .. k0 + 3 of group g, from its loaded inputs; after the 8th, the
-// MIXED product's exponent is split off.
+// Sites k0 .. k0 + 3 of group g, from its loaded inputs.
 template <bool MIXED, bool FAST>
 __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, int g, int k0,
                                             float norm, float nrm0, float *ring) {
@@ -286,12 +285,6 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
   for (int k = k0; k < k0 + 4; ++k)
     pair_step<MIXED, FAST>(st, G.e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2],
                            (g == 0 && k == 0) ? nrm0 : norm, ring);
-  if (MIXED && k0 == 4) {
-    // at most 8 factors below 2^127 since the last split: no overflow
-    const int ex = __builtin_amdgcn_frexp_exp(st.prod);
-    st.prod = __builtin_amdgcn_frexp_mant(st.prod);
-    st.pexp += ex;
-  }
 }
 
 // A full 32-site chunk as a software pipeline over its 4 groups: while one
@@ -366,8 +359,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   const unsigned ntr = 3u * M;
   PairState st;
   st.la = st.lb = 0.25f;
-  st.prod = 1.0;
-  st.pexp = 0;
+  st.nres = 0;
   st.fscore = 0.0f;
   st.nq = 0;
   const unsigned nch = (M + 31u) >> 5;
@@ -425,7 +417,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     for (int q = 0; q < NW; ++q) cw[q] = cn[q];
   }
   const float last = pair_sum(st);
-  if (MIXED) return log((double)last) - (log(st.prod) + (double)st.pexp * kLn2);
+  if (MIXED) return log((double)last) - (double)st.nres * kRescaleLog;
   return (double)(st.fscore + cr_logf(last));
 }
 

@@ -192,13 +192,13 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
   return score + orc_logf_cr(l00 + l01 + l10 + l11);
 }
 
-/* The same FP32 recurrence and rescale decisions; the scale factors (1.0f at
- * sites that do not rescale) are multiplied into an FP64 product whose binary
- * exponent is split off after every 8th site and after the last one, and
- * the score is log(last sum) - (log(mantissa) + exponent * ln 2) in FP64.
- * Mathematically this is the reference's score -= log(factor) sum
- * (impute.cpp:367-374) carried in FP64; the split points keep the product
- * finite (at most 8 factors below 2^127 between splits). */
+/* The same FP32 recurrence and rescale decisions (impute.cpp:354-372), but a
+ * site whose sum falls below norm multiplies its probabilities by the exact
+ * power of two 2^84 instead of by 1/sum. norm = FLT_MIN^(2/3) = 2^-84, so the
+ * rescaled sum stays below 1 and the product rounds nothing. The score is
+ * log(last sum) - (number of rescales) * 84 ln 2 in FP64: the reference's
+ * score -= log(factor) sum (impute.cpp:367-374) with every factor exact and
+ * the sum of logs carried exactly as an integer count. */
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M) {
@@ -206,18 +206,12 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                  *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
   const float norm = orc_norm();
   const float *e = emit, *t = tran;
-  float sum;
-  double prod = 1.0;
-  int pexp = 0, ex;
+  unsigned nres = 0;
   float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
         l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
   float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
         l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
   float b00, b01, b10, b11;
-  if (M == 1) {
-    prod = frexp(prod, &ex);
-    pexp += ex;
-  }
   e += 4;
   t += 3;
   for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
@@ -229,20 +223,15 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
     l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
     l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
     l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
-    if ((sum = l00 + l01 + l10 + l11) < norm) {
-      sum = 1.0f / sum;
-      prod *= (double)sum;
-      l00 *= sum;
-      l01 *= sum;
-      l10 *= sum;
-      l11 *= sum;
-    }
-    if ((m & 7u) == 7u || m == M - 1) {
-      prod = frexp(prod, &ex);
-      pexp += ex;
+    if (l00 + l01 + l10 + l11 < norm) {
+      l00 *= 0x1p84f;
+      l01 *= 0x1p84f;
+      l10 *= 0x1p84f;
+      l11 *= 0x1p84f;
+      nres++;
     }
   }
-  return log((double)(l00 + l01 + l10 + l11)) - (log(prod) + (double)pexp * ORC_LN2);
+  return log((double)(l00 + l01 + l10 + l11)) - (double)nres * (84.0 * ORC_LN2);
 }
 
 /* FP64 forward oracle: the same model, normalised at every site. */
