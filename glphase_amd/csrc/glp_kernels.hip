@@ -123,20 +123,19 @@ __device__ __forceinline__ float dpp(float x) {
 //   x * 1.0f == x keeps the others exact); the factors go to a per-lane LDS
 //   ring and their correctly rounded logs are subtracted, oldest first, at
 //   the end of each 32-site chunk -- the same float sum in the same order.
-//   MIXED: the FP32 recurrence rescales at the same sites, by the exact
-//   power of two 2^84 (norm = FLT_MIN^(2/3) = 2^-84, so the rescaled sum
-//   stays below 1), which rounds nothing; the lane counts its rescales and
-//   the score is log(last sum) - count * 84 ln 2 in FP64.
+//   MIXED: the FP32 recurrence forms no site sums.  After every 4th site
+//   (sites 3, 7, 11, ...) the pair scales its four probabilities by the
+//   power of two 2^k that brings their sum into [1, 2) (k >= 0, since the
+//   sum never exceeds 2), which rounds nothing; the lane adds up the k and
+//   the score is log(last sum) - (sum of k) ln 2 in FP64.
 struct PairState {
   float la, lb;
-  unsigned nres;  // MIXED: rescales so far
+  unsigned nres;  // MIXED: sum of the scale exponents so far
   float fscore;
   unsigned nq;
 };
 
 constexpr double kLn2 = 0.6931471805599453;
-constexpr float kRescale = 0x1p84f;            // MIXED rescale factor
-constexpr double kRescaleLog = 84.0 * kLn2;    // its natural log, in FP64
 
 // One site (impute.cpp:354-372) for the lane's two states; e = this lane's
 // two emissions.  ring: this lane's first ring slot (stride 64 floats).
@@ -149,24 +148,35 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   const float ba = la * t0 + pt + lb * t2;    // b00 / b01
   const float bb = lb * t0 + pt + la * t2;    // b11 / b10
   const float na = ba * e.x, nb = bb * e.y;
+  if (MIXED) {
+    st.la = na;
+    st.lb = nb;
+    return;
+  }
   // l00 + l01: the partner's na plus the own one (IEEE addition commutes,
   // so both lanes hold the reference's first partial sum bit for bit)
   float s = dpp<kDppSwap>(na) + na;
   s = dpp<kDppOdd>(nb) + s;   // + l10
   s = dpp<kDppEven>(nb) + s;  // + l11
   const bool r = s < nrm;
-  if (MIXED) {
-    const float sc = r ? kRescale : 1.0f;
-    st.la = na * sc;
-    st.lb = nb * sc;
-    st.nres += r ? 1u : 0u;
-  } else {
-    const float sc = r ? (FAST ? recip_normal(s) : recip_small(s)) : 1.0f;
-    st.la = na * sc;
-    st.lb = nb * sc;
-    ring[64u * st.nq] = sc;
-    st.nq += r ? 1u : 0u;
-  }
+  const float sc = r ? (FAST ? recip_normal(s) : recip_small(s)) : 1.0f;
+  st.la = na * sc;
+  st.lb = nb * sc;
+  ring[64u * st.nq] = sc;
+  st.nq += r ? 1u : 0u;
+}
+
+// MIXED, after every 4th site: scale the pair's probabilities by 2^k with
+// (l00 + l11) + (l01 + l10) in [1, 2) after it (frexp exponent e, k = 1 - e;
+// the sum is below 2, so k >= 0 and the scaling is exact).  Both lanes form
+// the same sum (addition commutes), so the pair scales together.
+__device__ __forceinline__ void mixed_scale(PairState &st) {
+  const float p = st.la + st.lb;
+  const float s = dpp<kDppSwap>(p) + p;
+  const int k = 1 - __builtin_amdgcn_frexp_exp(s);
+  st.la = __builtin_amdgcn_ldexpf(st.la, k);
+  st.lb = __builtin_amdgcn_ldexpf(st.lb, k);
+  st.nres += (unsigned)k;
 }
 
 // The pair's l00 + l01 + l10 + l11 in the reference's order.
@@ -238,6 +248,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       if (TAIL && (unsigned)s >= n) break;
       pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
+      if (MIXED && (k & 3) == 3) mixed_scale(st);
     }
   }
 }
@@ -285,6 +296,7 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
   for (int k = k0; k < k0 + 4; ++k)
     pair_step<MIXED, FAST>(st, G.e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2],
                            (g == 0 && k == 0) ? nrm0 : norm, ring);
+  if (MIXED) mixed_scale(st);
 }
 
 // A full 32-site chunk as a software pipeline over its 4 groups: while one
@@ -417,7 +429,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     for (int q = 0; q < NW; ++q) cw[q] = cn[q];
   }
   const float last = pair_sum(st);
-  if (MIXED) return log((double)last) - (double)st.nres * kRescaleLog;
+  if (MIXED) return log((double)last) - (double)st.nres * kLn2;
   return (double)(st.fscore + cr_logf(last));
 }
 

@@ -192,21 +192,22 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
   return score + orc_logf_cr(l00 + l01 + l10 + l11);
 }
 
-/* The same FP32 recurrence and rescale decisions (impute.cpp:354-372), but a
- * site whose sum falls below norm multiplies its probabilities by the exact
- * power of two 2^84 instead of by 1/sum. norm = FLT_MIN^(2/3) = 2^-84, so the
- * rescaled sum stays below 1 and the product rounds nothing. The score is
- * log(last sum) - (number of rescales) * 84 ln 2 in FP64: the reference's
- * score -= log(factor) sum (impute.cpp:367-374) with every factor exact and
- * the sum of logs carried exactly as an integer count. */
+/* The reference's FP32 recurrence (impute.cpp:354-366) without its per-site
+ * rescale test (impute.cpp:367-372). Instead, after every 4th site (m = 3, 7,
+ * 11, ...) the four probabilities are scaled by 2^k, where e is the frexp
+ * exponent of (l00 + l11) + (l01 + l10) and k = 1 - e brings that sum into
+ * [1, 2). The probabilities never sum to 2 or more, so k >= 0 and the
+ * scaling rounds nothing. The score is log(last sum) - (sum of k) ln 2 in
+ * FP64: the reference's score -= log(factor) sum (impute.cpp:367-374) with
+ * every factor exact and the sum of logs carried as an integer. */
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M) {
   const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
                  *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
-  const float norm = orc_norm();
   const float *e = emit, *t = tran;
-  unsigned nres = 0;
+  long ksum = 0;
+  int ex;
   float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
         l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
   float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
@@ -223,15 +224,17 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
     l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
     l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
     l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
-    if (l00 + l01 + l10 + l11 < norm) {
-      l00 *= 0x1p84f;
-      l01 *= 0x1p84f;
-      l10 *= 0x1p84f;
-      l11 *= 0x1p84f;
-      nres++;
+    if ((m & 3u) == 3u) {
+      (void)frexpf((l00 + l11) + (l01 + l10), &ex);
+      const int k = 1 - ex;
+      l00 = ldexpf(l00, k);
+      l01 = ldexpf(l01, k);
+      l10 = ldexpf(l10, k);
+      l11 = ldexpf(l11, k);
+      ksum += k;
     }
   }
-  return log((double)(l00 + l01 + l10 + l11)) - (double)nres * (84.0 * ORC_LN2);
+  return log((double)(l00 + l01 + l10 + l11)) - (double)ksum * ORC_LN2;
 }
 
 /* FP64 forward oracle: the same model, normalised at every site. */
