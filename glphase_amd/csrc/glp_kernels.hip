@@ -123,7 +123,8 @@ __device__ __forceinline__ float dpp(float x) {
 //   x * 1.0f == x keeps the others exact); the factors go to a per-lane LDS
 //   ring and their correctly rounded logs are subtracted, oldest first, at
 //   the end of each 32-site chunk -- the same float sum in the same order.
-//   MIXED: the FP32 recurrence forms no site sums.  After every 4th site
+//   MIXED: the FP32 recurrence, with l * t0 and l' * t2 fused into the
+//   partner term by two FMAs per state, forms no site sums.  After every 4th site
 //   (sites 3, 7, 11, ...) the pair scales its four probabilities by the
 //   power of two 2^k that brings their sum into [1, 2) (k >= 0, since the
 //   sum never exceeds 2), which rounds nothing; the lane adds up the k and
@@ -145,14 +146,15 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   const float la = st.la, lb = st.lb;
   const float pown = la + lb;                 // l00 + l11 (even) / l01 + l10 (odd)
   const float pt = dpp<kDppSwap>(pown) * t1;  // the partner's pair sum * t1
+  if (MIXED) {
+    // the same sums with the two products of each fused into FMAs
+    st.la = __builtin_fmaf(lb, t2, __builtin_fmaf(la, t0, pt)) * e.x;  // b00 / b01
+    st.lb = __builtin_fmaf(la, t2, __builtin_fmaf(lb, t0, pt)) * e.y;  // b11 / b10
+    return;
+  }
   const float ba = la * t0 + pt + lb * t2;    // b00 / b01
   const float bb = lb * t0 + pt + la * t2;    // b11 / b10
   const float na = ba * e.x, nb = bb * e.y;
-  if (MIXED) {
-    st.la = na;
-    st.lb = nb;
-    return;
-  }
   // l00 + l01: the partner's na plus the own one (IEEE addition commutes,
   // so both lanes hold the reference's first partial sum bit for bit)
   float s = dpp<kDppSwap>(na) + na;
@@ -173,7 +175,7 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
 __device__ __forceinline__ void mixed_scale(PairState &st) {
   const float p = st.la + st.lb;
   const float s = dpp<kDppSwap>(p) + p;
-  const int k = 1 - __builtin_amdgcn_frexp_exp(s);
+  const int k = 1 - __builtin_amdgcn_frexp_expf(s);
   st.la = __builtin_amdgcn_ldexpf(st.la, k);
   st.lb = __builtin_amdgcn_ldexpf(st.lb, k);
   st.nres += (unsigned)k;
@@ -191,6 +193,15 @@ __device__ __forceinline__ void flush_ring(PairState &st, const float *ring) {
   for (unsigned k = 0; __any(k < st.nq); ++k)
     if (k < st.nq) st.fscore -= cr_logf(ring[64u * k]);
   st.nq = 0;
+}
+
+// The emission half-entry at byte a of the kernel's dynamic LDS.  The kernels
+// of this file declare no static LDS (checked on the host before each
+// launch), so their dynamic LDS starts at LDS address 0 and a is the address
+// itself: the read takes it as is, with no base to add.
+__device__ __forceinline__ float2 lut_entry(uint32_t a) {
+  const glp_f2v v = *reinterpret_cast<const __attribute__((address_space(3))) glp_f2v *>((uintptr_t)a);
+  return make_float2(v.x, v.y);
 }
 
 // Byte address of site s's emission half-entry: byte 0 = (nib << 4) | 8 *
@@ -240,7 +251,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       const int s = 8 * g + k;
       if (TAIL && (unsigned)s >= n) break;
       const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
-      e[k] = *reinterpret_cast<const float2 *>(smem + a);
+      e[k] = lut_entry(a);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -283,7 +294,7 @@ __device__ __forceinline__ void group_load_b(GroupLds &G, const uint32_t (&cw)[B
   for (int k = 0; k < 8; ++k) {
     const int s = 8 * g + k;
     const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
-    G.e[k] = *reinterpret_cast<const float2 *>(smem + a);
+    G.e[k] = lut_entry(a);
   }
 }
 
@@ -796,7 +807,12 @@ size_t lds_bytes(const KArgs &a) {
   return b;
 }
 
+// Sets a kernel's dynamic LDS limit, after checking that it declares no
+// static LDS: its emission-table reads take LDS addresses from 0.
 static int allow_lds(const void *fn, size_t bytes) {
+  hipFuncAttributes fa;
+  GLP_TRY_K(hip_check(hipFuncGetAttributes(&fa, fn), "kernel attributes"));
+  if (fa.sharedSizeBytes != 0) return set_error(GLP_ERR_HIP, "kernel declares static LDS");
   return hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
                    "dynamic LDS attribute");
 }
@@ -855,8 +871,7 @@ const void *batched_kernel_fn(unsigned bits, bool mixed) { return batched_kernel
 
 int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu) {
   const void *fn = batched_kernel(bits, mixed);
-  GLP_TRY_K(hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
-                      "dynamic LDS attribute"));
+  GLP_TRY_K(allow_lds(fn, smem));
   return hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 256, smem),
                    "occupancy query");
 }

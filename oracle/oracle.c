@@ -192,8 +192,10 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
   return score + orc_logf_cr(l00 + l01 + l10 + l11);
 }
 
-/* The reference's FP32 recurrence (impute.cpp:354-366) without its per-site
- * rescale test (impute.cpp:367-372). Instead, after every 4th site (m = 3, 7,
+/* The reference's FP32 recurrence (impute.cpp:354-366), with each state's
+ * products l * t0 and l' * t2 fused with the (pair sum) * t1 term into two
+ * FMAs, and without the per-site rescale test (impute.cpp:367-372).
+ * Instead, after every 4th site (m = 3, 7,
  * 11, ...) the four probabilities are scaled by 2^k, where e is the frexp
  * exponent of (l00 + l11) + (l01 + l10) and k = 1 - e brings that sum into
  * [1, 2). The probabilities never sum to 2 or more, so k >= 0 and the
@@ -208,6 +210,7 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
   const float *e = emit, *t = tran;
   long ksum = 0;
   int ex;
+  float pe, po;
   float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
         l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
   float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
@@ -216,10 +219,12 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
   e += 4;
   t += 3;
   for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
-    b00 = l00 * t[0] + (l01 + l10) * t[1] + l11 * t[2];
-    b01 = l01 * t[0] + (l00 + l11) * t[1] + l10 * t[2];
-    b10 = l10 * t[0] + (l00 + l11) * t[1] + l01 * t[2];
-    b11 = l11 * t[0] + (l01 + l10) * t[1] + l00 * t[2];
+    pe = (l00 + l11) * t[1];
+    po = (l01 + l10) * t[1];
+    b00 = fmaf(l11, t[2], fmaf(l00, t[0], po));
+    b01 = fmaf(l10, t[2], fmaf(l01, t[0], pe));
+    b10 = fmaf(l01, t[2], fmaf(l10, t[0], pe));
+    b11 = fmaf(l00, t[2], fmaf(l11, t[0], po));
     l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
     l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
     l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
