@@ -74,7 +74,8 @@ __device__ __forceinline__ float cr_logf(float x) {
 // entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
 // in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
 constexpr unsigned kSpreadBytes = 4 * 256 * 8;
-constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk (a wave has two slots)
+constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk
+constexpr unsigned kTranSlot = 128;    // a wave's LDS slot for it (a float pair per lane; a wave has two)
 constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
 
 __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
@@ -341,22 +342,54 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
   group_load_b<BITS>(B, cw, 3, halfbits, smem);
   group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
 
-  if (lane < 48u) reinterpret_cast<float2 *>(trn)[lane] = tpn;
+  // every lane writes (lanes 48-63 into the slot's unread tail), so that the
+  // write, and with it the tran load, is not under a branch
+  reinterpret_cast<float2 *>(trn)[lane] = tpn;
   group_load_a(A, hn, trn, 0, spread);
   group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
   group_load_b<BITS>(A, cn, 0, halfbits, smem);
   group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
-// Floats o and o + 1 of tran (n floats in all), 0 past the end: when n is
-// odd (odd site count) the last float stands alone.
+// Floats o and o + 1 of tran (n floats in all).  Both loads always run, from
+// addresses clamped to the last float, so that no load sits under a branch
+// (when n is odd the last float stands alone: its partner is a copy of it).
+// Values past the end are never used in arithmetic: the sites they belong
+// to are not stepped.
 __device__ __forceinline__ float2 tran_pair(gptr_t<float> tran, unsigned o, unsigned n) {
-  float2 t = make_float2(0.0f, 0.0f);
-  if (o + 1u < n)
-    t = load_f2(tran + o);
-  else if (o < n)
-    t.x = tran[o];
-  return t;
+  const float x = tran[o < n ? o : n - 1u];
+  const float y = tran[o + 1u < n ? o + 1u : n - 1u];
+  return make_float2(x, y);
+}
+
+// The global inputs of chunk c: the 4 parents' allele words, the code
+// words and this lane's pair of tran floats (lanes 0-47 carry a chunk's 96).
+// Every load runs for every c: past the window's end the addresses are
+// clamped and the values are not used, except by reads of the LDS tables.
+template <int BITS>
+struct ChunkIn {
+  uint32_t h[4];
+  uint32_t cw[BITS];
+  float2 tp;
+};
+
+template <int BITS>
+__device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsigned nch,
+                                           gptr_t<uint32_t> f0, gptr_t<uint32_t> f1,
+                                           gptr_t<uint32_t> m0, gptr_t<uint32_t> m1,
+                                           gptr_t<uint32_t> codes, size_t stride, unsigned nwords,
+                                           gptr_t<float> tran, unsigned ntr, unsigned lane) {
+  const unsigned cc = c < nch ? c : nch - 1u;
+  in.tp = tran_pair(tran, kTranFloats * c + 2u * lane, ntr);
+  in.h[0] = f0[cc];
+  in.h[1] = f1[cc];
+  in.h[2] = m0[cc];
+  in.h[3] = m1[cc];
+#pragma unroll
+  for (int q = 0; q < BITS; ++q) {
+    const unsigned wq = cc * BITS + q;
+    in.cw[q] = codes[(wq < nwords ? wq : nwords - 1u) * stride];
+  }
 }
 
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
@@ -374,7 +407,6 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   const auto f0 = gmem(f0p), f1 = gmem(f1p), m0 = gmem(m0p), m1 = gmem(m1p);
   const auto codes = gmem(codesp);
   const auto tran = gmem(tranp);
-  constexpr int NW = BITS;  // code words per 32 sites
   constexpr int CPW = 32 / BITS;
   const uint2 *spread = reinterpret_cast<const uint2 *>(smem + 16u * lut_entries);
   const unsigned lane = threadIdx.x & 63u;
@@ -387,57 +419,44 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   st.nq = 0;
   const unsigned nch = (M + 31u) >> 5;
   const unsigned nwords = (M + CPW - 1) / CPW;
-  uint32_t h[4] = {f0[0], f1[0], m0[0], m1[0]};
-  uint32_t cw[NW];
-#pragma unroll
-  for (int q = 0; q < NW; ++q) cw[q] = (unsigned)q < nwords ? codes[q * stride] : 0u;
-  // lane l < 48 carries floats 2l, 2l+1 of a chunk's 96 tran values; site
-  // 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step gives
-  // l_xy = 1/4 * e_xy exactly
-  float2 tp = make_float2(0.0f, 0.0f);
-  if (lane < 48u) tp = tran_pair(tran, 2u * lane, ntr);
-  if (lane == 0u) tp = make_float2(1.0f, 0.0f);
-  if (lane == 1u) tp.x = 0.0f;
+  // chunk ch's global inputs sit in buffer X (ch even) or Y (ch odd): the
+  // loop body is written out twice, so that the next chunk's loads land in
+  // the other buffer while this one is used, with no register copies
+  ChunkIn<BITS> X, Y;
+  load_chunk<BITS>(X, 0, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
+  // site 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step
+  // gives l_xy = 1/4 * e_xy exactly
+  if (lane == 0u) X.tp = make_float2(1.0f, 0.0f);
+  if (lane == 1u) X.tp.x = 0.0f;
   // chunk ch's tran sits in LDS slot ch & 1 of the wave's two; a full
   // chunk's last group writes the next chunk's slot
-  if (lane < 48u) reinterpret_cast<float2 *>(wtr)[lane] = tp;
+  reinterpret_cast<float2 *>(wtr)[lane] = X.tp;
   GroupLds A, B;
   if (M >= 32u) {
-    group_load_a(A, h, wtr, 0, spread);
-    group_load_b<BITS>(A, cw, 0, halfbits, smem);
+    group_load_a(A, X.h, wtr, 0, spread);
+    group_load_b<BITS>(A, X.cw, 0, halfbits, smem);
   }
-  for (unsigned ch = 0; ch < nch; ++ch) {
-    // prefetch the next chunk's tran, allele and code words
-    const unsigned nx = ch + 1;
-    tp = make_float2(0.0f, 0.0f);
-    if (lane < 48u) tp = tran_pair(tran, kTranFloats * nx + 2u * lane, ntr);
-    uint32_t hn[4] = {0, 0, 0, 0}, cn[NW];
-    if (nx < nch) {
-      hn[0] = f0[nx];
-      hn[1] = f1[nx];
-      hn[2] = m0[nx];
-      hn[3] = m1[nx];
-    }
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-      const unsigned wq = nx * NW + q;
-      cn[q] = wq < nwords ? codes[wq * stride] : 0u;
-    }
+  auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
+    load_chunk<BITS>(nxt, ch + 1u, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
+    // keep the loads here, a whole chunk ahead of their use (the scheduler
+    // would otherwise sink the tran loads down to the LDS write that uses them)
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned base = ch << 5;
     const unsigned n = M - base < 32u ? M - base : 32u;
     const float nrm0 = ch == 0 ? 0.0f : norm;
-    float *tr = wtr + kTranFloats * (ch & 1u), *trn = wtr + kTranFloats * ((ch + 1u) & 1u);
+    float *tr = wtr + kTranSlot * (ch & 1u), *trn = wtr + kTranSlot * ((ch + 1u) & 1u);
     if (n == 32)
-      fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, h, cw, tr, hn, cn, trn, tp, smem, spread,
-                                        halfbits, norm, nrm0, ring);
+      fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, cur.h, cur.cw, tr, nxt.h, nxt.cw, trn, nxt.tp,
+                                        smem, spread, halfbits, norm, nrm0, ring);
     else  // the window's last chunk, shorter than 32 sites
-      fwd_chunk<BITS, MIXED, true, FAST>(st, h, cw, tr, n, smem, spread, halfbits, norm, nrm0,
-                                         ring);
+      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, smem, spread, halfbits, norm,
+                                         nrm0, ring);
     if (!MIXED) flush_ring(st, ring);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) h[q] = hn[q];
-#pragma unroll
-    for (int q = 0; q < NW; ++q) cw[q] = cn[q];
+  };
+  for (unsigned ch = 0; ch < nch; ch += 2u) {
+    chunk(ch, X, Y);
+    if (ch + 1u == nch) break;
+    chunk(ch + 1u, Y, X);
   }
   const float last = pair_sum(st);
   if (MIXED) return log((double)last) - (double)st.nres * kLn2;
@@ -448,7 +467,7 @@ __device__ __forceinline__ const uint32_t *hap_row(const glp_window &W, unsigned
   return reinterpret_cast<const uint32_t *>(W.panel + (size_t)h * W.words_per_hap);
 }
 
-// LDS layout after the two tables: per wave, two slots of kTranFloats
+// LDS layout after the two tables: per wave, two slots of kTranSlot
 // floats of tran, then (F32 only) per wave a ring of kRingSlots x 64 floats.
 template <int BITS, bool MIXED>
 __device__ __forceinline__ double score_set(const glp_window &W, const unsigned (&P)[4],
@@ -458,9 +477,9 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
   const unsigned wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   float *tables_end = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) +
                                                 16u * lut_entries + kSpreadBytes);
-  float *wtr = tables_end + 2u * kTranFloats * wave;
+  float *wtr = tables_end + 2u * kTranSlot * wave;
   float *ring =
-      tables_end + 2u * kTranFloats * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
+      tables_end + 2u * kTranSlot * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
   const uint32_t *f0 = hap_row(W, P[0]), *f1 = hap_row(W, P[1]), *m0 = hap_row(W, P[2]),
                  *m1 = hap_row(W, P[3]);
   const double v = forward_pass<BITS, MIXED, true>(f0, f1, m0, m1, W.packed + sample, N, W.tran,
@@ -802,7 +821,7 @@ __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *b
 // ---------------------------------------------------------------------------
 size_t lds_bytes(const KArgs &a) {
   const size_t waves = 4;  // 256-thread blocks
-  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * kTranFloats * 4;
+  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * kTranSlot * 4;
   if (a.precision == kPrecF32) b += waves * kRingSlots * 64 * 4;
   return b;
 }
