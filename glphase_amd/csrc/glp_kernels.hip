@@ -265,30 +265,29 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
   }
 }
 
-// LDS inputs of one 8-site group of a full chunk, loaded a group ahead of
-// its recurrence: the parents' spread entries, the sites' tran rows and the
-// lane's emissions.
+// LDS inputs of one 8-site group of a full chunk: the parents' spread
+// entries, the sites' tran rows and the lane's emissions.
 struct GroupLds {
   uint2 sp[4];
   float4 t4[6];
   float2 e[8];
 };
 
-// First half of group g's loads: spread entries (from the chunk's allele
-// words h) and tran rows (from the chunk's LDS slot tr).
-__device__ __forceinline__ void group_load_a(GroupLds &G, const uint32_t (&h)[4], const float *tr,
-                                             int g, const uint2 *spread) {
+// Group g's spread entries, from the chunk's allele words h.
+__device__ __forceinline__ void group_load_sp(GroupLds &G, const uint32_t (&h)[4], int g,
+                                              const uint2 *spread) {
 #pragma unroll
   for (int p = 0; p < 4; ++p) G.sp[p] = spread[p * 256 + ((h[p] >> (8 * g)) & 255u)];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) G.t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
 }
 
-// Second half: the emissions, whose addresses need the spread entries.
+// Group g's tran rows (from the chunk's LDS slot tr) and emissions, whose
+// addresses need its spread entries.
 template <int BITS>
-__device__ __forceinline__ void group_load_b(GroupLds &G, const uint32_t (&cw)[BITS], int g,
-                                             uint32_t halfbits, const unsigned char *smem) {
+__device__ __forceinline__ void group_load_et(GroupLds &G, const uint32_t (&cw)[BITS],
+                                              const float *tr, int g, uint32_t halfbits) {
   constexpr int CPW = 32 / BITS;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) G.t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
   const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x | halfbits,
                           G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y | halfbits};
 #pragma unroll
@@ -311,43 +310,43 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
   if (MIXED) mixed_scale(st);
 }
 
-// A full 32-site chunk as a software pipeline over its 4 groups: while one
-// group's recurrence runs, the next group's LDS loads are in flight (spread
-// and tran before its first half, emissions before its second half), in
-// two buffers A and B that the groups alternate between.  On entry A holds
-// group 0's inputs; during group 3 the next chunk's tran goes into its LDS
-// slot trn (from tpn) and A takes the next chunk's group 0 (from hn, cn).
-// LDS reads past the window's end only read the tables.
+// A full 32-site chunk as a software pipeline over its 4 groups, in two
+// buffers A and B that the groups alternate between.  While group g runs,
+// group g + 1's tran rows and emissions are read (a whole group ahead of
+// their use) and, from its middle, group g + 2's spread entries (which the
+// emission addresses need).  On entry A holds group 0's inputs and B group
+// 1's spread entries; at exit the same holds for the next chunk (allele
+// words hn, code words cn, tran in LDS slot trn, written from tpn during
+// group 2).  LDS reads past the window's end only read the tables.
 template <int BITS, bool MIXED, bool FAST>
 __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, GroupLds &B,
                                                const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
                                                const float *tr, const uint32_t (&hn)[4],
                                                const uint32_t (&cn)[BITS], float *trn, float2 tpn,
-                                               const unsigned char *smem, const uint2 *spread,
-                                               uint32_t halfbits, float norm, float nrm0,
-                                               float *ring) {
+                                               const uint2 *spread, uint32_t halfbits, float norm,
+                                               float nrm0, float *ring) {
   const unsigned lane = threadIdx.x & 63u;
-  group_load_a(B, h, tr, 1, spread);
+  group_load_et<BITS>(B, cw, tr, 1, halfbits);
   group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
-  group_load_b<BITS>(B, cw, 1, halfbits, smem);
+  group_load_sp(A, h, 2, spread);
   group_steps<MIXED, FAST>(st, A, 0, 4, norm, nrm0, ring);
 
-  group_load_a(A, h, tr, 2, spread);
+  group_load_et<BITS>(A, cw, tr, 2, halfbits);
   group_steps<MIXED, FAST>(st, B, 1, 0, norm, nrm0, ring);
-  group_load_b<BITS>(A, cw, 2, halfbits, smem);
+  group_load_sp(B, h, 3, spread);
   group_steps<MIXED, FAST>(st, B, 1, 4, norm, nrm0, ring);
 
-  group_load_a(B, h, tr, 3, spread);
+  group_load_et<BITS>(B, cw, tr, 3, halfbits);
   group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
-  group_load_b<BITS>(B, cw, 3, halfbits, smem);
-  group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
-
   // every lane writes (lanes 48-63 into the slot's unread tail), so that the
   // write, and with it the tran load, is not under a branch
   reinterpret_cast<float2 *>(trn)[lane] = tpn;
-  group_load_a(A, hn, trn, 0, spread);
+  group_load_sp(A, hn, 0, spread);
+  group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
+
+  group_load_et<BITS>(A, cn, trn, 0, halfbits);
   group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
-  group_load_b<BITS>(A, cn, 0, halfbits, smem);
+  group_load_sp(B, hn, 1, spread);
   group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
@@ -433,8 +432,9 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   reinterpret_cast<float2 *>(wtr)[lane] = X.tp;
   GroupLds A, B;
   if (M >= 32u) {
-    group_load_a(A, X.h, wtr, 0, spread);
-    group_load_b<BITS>(A, X.cw, 0, halfbits, smem);
+    group_load_sp(A, X.h, 0, spread);
+    group_load_et<BITS>(A, X.cw, wtr, 0, halfbits);
+    group_load_sp(B, X.h, 1, spread);
   }
   auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
     load_chunk<BITS>(nxt, ch + 1u, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
@@ -447,7 +447,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     float *tr = wtr + kTranSlot * (ch & 1u), *trn = wtr + kTranSlot * ((ch + 1u) & 1u);
     if (n == 32)
       fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, cur.h, cur.cw, tr, nxt.h, nxt.cw, trn, nxt.tp,
-                                        smem, spread, halfbits, norm, nrm0, ring);
+                                        spread, halfbits, norm, nrm0, ring);
     else  // the window's last chunk, shorter than 32 sites
       fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, smem, spread, halfbits, norm,
                                          nrm0, ring);
