@@ -178,6 +178,10 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
       return set_error(GLP_ERR_ARG, "words_per_hap must be ceil(num_sites / 64)");
     if (!W.panel || !W.packed || !W.tran || !W.parents || !W.like)
       return set_error(GLP_ERR_ARG, "window with a null pointer");
+    // the kernels index a window's packed codes with 32-bit word offsets
+    const uint64_t code_words = ((uint64_t)W.num_sites * d->bits_per_code + 31) / 32;
+    if (code_words * d->num_samples >= (1ull << 32))
+      return set_error(GLP_ERR_ARG, "a window's packed codes exceed 2^32 words");
   }
   a.N = d->num_samples;
   a.H = d->num_haps;

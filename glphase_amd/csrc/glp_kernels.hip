@@ -384,10 +384,14 @@ __device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsign
   in.h[1] = f1[cc];
   in.h[2] = m0[cc];
   in.h[3] = m1[cc];
+  // word offsets step by stride from the chunk's first word, clamped to the
+  // window's last word (one add and one min per word)
+  const unsigned last = (nwords - 1u) * (unsigned)stride;
+  unsigned off = cc * BITS * (unsigned)stride;
 #pragma unroll
   for (int q = 0; q < BITS; ++q) {
-    const unsigned wq = cc * BITS + q;
-    in.cw[q] = codes[(wq < nwords ? wq : nwords - 1u) * stride];
+    in.cw[q] = codes[off < last ? off : last];
+    off += (unsigned)stride;
   }
 }
 
