@@ -73,7 +73,7 @@ __device__ __forceinline__ float cr_logf(float x) {
 // Spread table: for parent p and an allele byte b of 8 consecutive sites,
 // entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
 // in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
-constexpr unsigned kSpreadBytes = 4 * 256 * 8;
+constexpr unsigned kSpreadBytes = 2 * 4 * 256 * 8;  // spread tables: even lanes', odd lanes'
 constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk
 constexpr unsigned kTranSlot = 128;    // a wave's LDS slot for it (a float pair per lane; a wave has two)
 constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
@@ -83,13 +83,19 @@ __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__re
   float4 *lut = reinterpret_cast<float4 *>(smem);
   for (unsigned q = threadIdx.x; q < lut_entries; q += blockDim.x) lut[q] = g_lut[q];
   uint2 *spread = reinterpret_cast<uint2 *>(smem + 16u * lut_entries);
-  for (unsigned q = threadIdx.x; q < 1024u; q += blockDim.x) {
-    const unsigned p = q >> 8, b = q & 255u;
+  // two sets of 4 tables (one per parent), the odd lanes' set with the
+  // odd-lane bit 8 set in every byte of parent 0's entries
+  for (unsigned q = threadIdx.x; q < 2048u; q += blockDim.x) {
+    const unsigned p = (q >> 8) & 3u, b = q & 255u;
     uint32_t lo = 0, hi = 0;
 #pragma unroll
     for (unsigned i = 0; i < 4; ++i) {
       lo |= ((b >> i) & 1u) << (8 * i + 4 + p);
       hi |= ((b >> (i + 4)) & 1u) << (8 * i + 4 + p);
+    }
+    if (q >= 1024u && p == 0u) {
+      lo |= 0x08080808u;
+      hi |= 0x08080808u;
     }
     spread[q] = make_uint2(lo, hi);
   }
@@ -230,7 +236,7 @@ template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
                                           const unsigned char *smem, const uint2 *spread,
-                                          uint32_t halfbits, float norm, float nrm0, float *ring) {
+                                          float norm, float nrm0, float *ring) {
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     if (TAIL && (unsigned)(8 * g) >= n) break;
@@ -238,8 +244,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
     const uint2 s1 = spread[1 * 256 + ((h[1] >> (8 * g)) & 255u)];
     const uint2 s2 = spread[2 * 256 + ((h[2] >> (8 * g)) & 255u)];
     const uint2 s3 = spread[3 * 256 + ((h[3] >> (8 * g)) & 255u)];
-    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x | halfbits,
-                            s0.y | s1.y | s2.y | s3.y | halfbits};
+    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x, s0.y | s1.y | s2.y | s3.y};
     constexpr int CPW = 32 / BITS;
     // emissions and transitions of the group, loaded ahead of the recurrence
     float2 e[8];
@@ -284,12 +289,12 @@ __device__ __forceinline__ void group_load_sp(GroupLds &G, const uint32_t (&h)[4
 // addresses need its spread entries.
 template <int BITS>
 __device__ __forceinline__ void group_load_et(GroupLds &G, const uint32_t (&cw)[BITS],
-                                              const float *tr, int g, uint32_t halfbits) {
+                                              const float *tr, int g) {
   constexpr int CPW = 32 / BITS;
 #pragma unroll
   for (int q = 0; q < 6; ++q) G.t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
-  const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x | halfbits,
-                          G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y | halfbits};
+  const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x,
+                          G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int s = 8 * g + k;
@@ -323,20 +328,20 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
                                                const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
                                                const float *tr, const uint32_t (&hn)[4],
                                                const uint32_t (&cn)[BITS], float *trn, float2 tpn,
-                                               const uint2 *spread, uint32_t halfbits, float norm,
+                                               const uint2 *spread, float norm,
                                                float nrm0, float *ring) {
   const unsigned lane = threadIdx.x & 63u;
-  group_load_et<BITS>(B, cw, tr, 1, halfbits);
+  group_load_et<BITS>(B, cw, tr, 1);
   group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
   group_load_sp(A, h, 2, spread);
   group_steps<MIXED, FAST>(st, A, 0, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(A, cw, tr, 2, halfbits);
+  group_load_et<BITS>(A, cw, tr, 2);
   group_steps<MIXED, FAST>(st, B, 1, 0, norm, nrm0, ring);
   group_load_sp(B, h, 3, spread);
   group_steps<MIXED, FAST>(st, B, 1, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(B, cw, tr, 3, halfbits);
+  group_load_et<BITS>(B, cw, tr, 3);
   group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
   // every lane writes (lanes 48-63 into the slot's unread tail), so that the
   // write, and with it the tran load, is not under a branch
@@ -344,7 +349,7 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
   group_load_sp(A, hn, 0, spread);
   group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(A, cn, trn, 0, halfbits);
+  group_load_et<BITS>(A, cn, trn, 0);
   group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
   group_load_sp(B, hn, 1, spread);
   group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
@@ -411,9 +416,10 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   const auto codes = gmem(codesp);
   const auto tran = gmem(tranp);
   constexpr int CPW = 32 / BITS;
-  const uint2 *spread = reinterpret_cast<const uint2 *>(smem + 16u * lut_entries);
   const unsigned lane = threadIdx.x & 63u;
-  const uint32_t halfbits = (lane & 1u) ? 0x08080808u : 0u;
+  // this lane's set of spread tables
+  const uint2 *spread =
+      reinterpret_cast<const uint2 *>(smem + 16u * lut_entries) + ((lane & 1u) ? 1024u : 0u);
   const unsigned ntr = 3u * M;
   PairState st;
   st.la = st.lb = 0.25f;
@@ -437,7 +443,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   GroupLds A, B;
   if (M >= 32u) {
     group_load_sp(A, X.h, 0, spread);
-    group_load_et<BITS>(A, X.cw, wtr, 0, halfbits);
+    group_load_et<BITS>(A, X.cw, wtr, 0);
     group_load_sp(B, X.h, 1, spread);
   }
   auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
@@ -451,9 +457,9 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     float *tr = wtr + kTranSlot * (ch & 1u), *trn = wtr + kTranSlot * ((ch + 1u) & 1u);
     if (n == 32)
       fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, cur.h, cur.cw, tr, nxt.h, nxt.cw, trn, nxt.tp,
-                                        spread, halfbits, norm, nrm0, ring);
+                                        spread, norm, nrm0, ring);
     else  // the window's last chunk, shorter than 32 sites
-      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, smem, spread, halfbits, norm,
+      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, smem, spread, norm,
                                          nrm0, ring);
     if (!MIXED) flush_ring(st, ring);
   };
