@@ -108,8 +108,8 @@ __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__re
 // A chain's forward pass runs on two adjacent lanes: the even lane carries
 // states 00 and 11, the odd lane 01 and 10.  The other lane's values arrive
 // through DPP quad permutations folded into the consuming add or multiply,
-// so every float operation of the serial recurrence happens once per lane,
-// in the reference's order, with the reference's operands.
+// so every float operation of the serial recurrence happens once per lane
+// (F32: in the reference's order, with the reference's operands).
 constexpr int kDppSwap = 0xB1;  // quad_perm [1,0,3,2]: the partner lane
 constexpr int kDppEven = 0xA0;  // quad_perm [0,0,2,2]: the pair's even lane
 constexpr int kDppOdd = 0xF5;   // quad_perm [1,1,3,3]: the pair's odd lane
@@ -404,7 +404,7 @@ __device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsign
 // pair m0, m1; rows as 32-bit words), computed by a lane pair; both lanes
 // return it.  Every lane of the wavefront must call it together (M and tran
 // wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
-// LDS buffer wtr, one chunk ahead.
+// LDS buffer wtr, during the chunk before it.
 template <int BITS, bool MIXED, bool FAST>
 __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p, const uint32_t *__restrict__ f1p,
                                const uint32_t *__restrict__ m0p, const uint32_t *__restrict__ m1p,
