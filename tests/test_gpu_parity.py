@@ -120,6 +120,27 @@ def test_batched_mixed_precision(orc, lanes):
     compare_run(orc, p, job, C, pen, seed, PREC_MIXED)
 
 
+@pytest.mark.parametrize("bits", [4, 8])
+@pytest.mark.parametrize("M", [1, 3, 4, 5, 31, 32, 33, 63, 64, 65, 97, 301])
+def test_mixed_precision_site_counts(orc, bits, M):
+    # the mixed-precision recurrence scales after every 4th site and reads a
+    # window's last chunk through clamped loads: site counts around the
+    # 4-site and 32-site boundaries, r = 0.3 so that every transition term
+    # counts, through the batched and the plain path
+    say(f"mixed precision bits={bits} M={M}")
+    p = helpers.synthetic(40, M, bits=bits, seed=3 * M + bits, training_size=4096)
+    r = np.float32(0.3)
+    p["tran"][1:] = (np.float32(1 - r) * np.float32(1 - r), r * np.float32(1 - r), r * r)
+    C, pen, seed = 10, 0.7, 17 + M
+    job = job_for(p, C, precision=PREC_MIXED, trace=True)
+    job.make_plan(3)
+    job.run_batched(gen=2, pen=pen, seed=seed)
+    compare_run(orc, p, job, C, pen, seed, PREC_MIXED, gen=2)
+    job2 = job_for(p, C, precision=PREC_MIXED, trace=True)
+    job2.run_plain(gen=2, pen=pen, seed=seed)
+    compare_run(orc, p, job2, C, pen, seed, PREC_MIXED, gen=2)
+
+
 def test_autotune_keeps_parity_and_inputs(orc):
     say("autotune")
     from glphase_amd.engine import Job
