@@ -154,3 +154,26 @@ def test_library_errors_without_gpu():
     assert b"windows" in L.glp_last_error()
     assert L.glp_emit_lut_bytes(8, 0) == 256 * 16 * 16
     assert L.glp_emit_lut_bytes(3, 0) == 0
+
+
+def test_plan_rejects_code_arrays_past_32_bit_offsets():
+    # the kernels index a window's packed codes with 32-bit word offsets: a
+    # window of 2^20 sites x 2^14 samples at 8-bit codes holds 2^32 words,
+    # which the descriptor check refuses before any device work (the
+    # pointers are never read)
+    from glphase_amd import _lib
+    L = _lib.lib()
+    fake = ctypes.c_void_p(4096)
+
+    def desc(M, N):
+        w = _lib.GlpWindow(M, (M + 63) // 64, fake, fake, fake, fake, fake)
+        d = _lib.GlpMhDesc()
+        d.num_windows, d.windows = 1, ctypes.pointer(w)
+        d.num_samples, d.num_haps, d.bits_per_code = N, 2 * N, 8
+        d.emit_lut, d.num_cycles, d.precision = fake, 4, 1
+        return d, w
+
+    out = ctypes.c_void_p()
+    d, w = desc(1 << 20, 1 << 14)  # w keeps the window alive
+    assert L.glp_plan_create(ctypes.byref(d), 0, ctypes.byref(out)) == 1
+    assert b"2^32" in L.glp_last_error()
