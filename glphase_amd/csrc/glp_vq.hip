@@ -21,7 +21,10 @@ namespace glp {
 
 // FindGLCode of (RR, Het): the first codeword of least
 // sqrt((c.RR - RR)^2 + (c.Het - Het)^2), the differences taken in float and
-// the rest in FP64, decided without most of the square roots:
+// the rest in FP64, decided without most of the square roots.  A float
+// difference squared is exact in FP64 (48 significant bits), so the sum of
+// squares is one FMA: fma(dr, dr, dh * dh) rounds once, like dr*dr + dh*dh.
+//  - the squares and the sum are thus the reference's values bit for bit;
 //  - sqrt is monotone, so codeword c can only win when its squared sum s is
 //    below the best one, bs;
 //  - when s < round(bs * (1 - 2^-50)), the exact roots differ by a factor
@@ -32,12 +35,12 @@ namespace glp {
 __device__ __forceinline__ unsigned vq_nearest(const float2 *cb, unsigned n, float RR, float Het) {
   constexpr double kShrink = 1.0 - 0x1p-50;
   double dr = (double)(cb[0].x - RR), dh = (double)(cb[0].y - Het);
-  double bs = dr * dr + dh * dh, thr = bs * kShrink;
+  double bs = fma(dr, dr, dh * dh), thr = bs * kShrink;
   unsigned best = 0;
   for (unsigned c = 1; c < n; ++c) {
     dr = (double)(cb[c].x - RR);
     dh = (double)(cb[c].y - Het);
-    const double s = dr * dr + dh * dh;
+    const double s = fma(dr, dr, dh * dh);
     bool win = s < thr;
     if (!win && s < bs) {
       // near tie, rare: the empty asm keeps the compiler from computing both
