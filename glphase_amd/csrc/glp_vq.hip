@@ -32,28 +32,59 @@ namespace glp {
 //    <= 2^-53 each) are strictly ordered: c wins;
 //  - only in the band between (a near tie) are the two rounded roots compared
 //    as FindGLCode compares them, so equal roots keep the earlier codeword.
+__device__ __forceinline__ double vq_dist2(float2 w, float RR, float Het) {
+  const double dr = (double)(w.x - RR), dh = (double)(w.y - Het);
+  return fma(dr, dr, dh * dh);
+}
+
+// One codeword of the search, near ties decided by the rounded roots.
+__device__ __forceinline__ void vq_step(const float2 *cb, unsigned c, float RR, float Het,
+                                        double &bs, double &thr, unsigned &best) {
+  constexpr double kShrink = 1.0 - 0x1p-50;
+  const double s = vq_dist2(cb[c], RR, Het);
+  bool win = s < thr;
+  if (!win && s < bs) {
+    // near tie, rare: the empty asm keeps the compiler from computing both
+    // square roots for every codeword and selecting afterwards
+    asm volatile("");
+    win = sqrt(s) < sqrt(bs);
+  }
+  if (win) {
+    bs = s;
+    thr = s * kShrink;
+    best = c;
+  }
+}
+
+// The codewords go in groups of 4 on a branch-free path that takes s < thr
+// as the decision and only notes near ties; a group in which a lane of the
+// wavefront met one is redone by vq_step from the state at its start.
 __device__ __forceinline__ unsigned vq_nearest(const float2 *cb, unsigned n, float RR, float Het) {
   constexpr double kShrink = 1.0 - 0x1p-50;
-  double dr = (double)(cb[0].x - RR), dh = (double)(cb[0].y - Het);
-  double bs = fma(dr, dr, dh * dh), thr = bs * kShrink;
-  unsigned best = 0;
-  for (unsigned c = 1; c < n; ++c) {
-    dr = (double)(cb[c].x - RR);
-    dh = (double)(cb[c].y - Het);
-    const double s = fma(dr, dr, dh * dh);
-    bool win = s < thr;
-    if (!win && s < bs) {
-      // near tie, rare: the empty asm keeps the compiler from computing both
-      // square roots for every codeword and selecting afterwards
-      asm volatile("");
-      win = sqrt(s) < sqrt(bs);
+  double bs = vq_dist2(cb[0], RR, Het), thr = bs * kShrink;
+  unsigned best = 0, c = 1;
+  for (; c + 4u <= n; c += 4u) {
+    const double bs0 = bs, thr0 = thr;
+    const unsigned best0 = best;
+    bool tie = false;
+#pragma unroll
+    for (unsigned k = 0; k < 4u; ++k) {
+      const double s = vq_dist2(cb[c + k], RR, Het);
+      const bool win = s < thr;
+      tie = tie || (!win && s < bs);
+      const double t = s * kShrink;
+      bs = win ? s : bs;
+      thr = win ? t : thr;
+      best = win ? c + k : best;
     }
-    if (win) {
-      bs = s;
-      thr = s * kShrink;
-      best = c;
+    if (__builtin_expect(__ballot(tie) != 0ull, 0)) {
+      bs = bs0;
+      thr = thr0;
+      best = best0;
+      for (unsigned k = 0; k < 4u; ++k) vq_step(cb, c + k, RR, Het, bs, thr, best);
     }
   }
+  for (; c < n; ++c) vq_step(cb, c, RR, Het, bs, thr, best);
   return best;
 }
 
