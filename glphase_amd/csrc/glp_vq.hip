@@ -64,22 +64,23 @@ __device__ __forceinline__ unsigned vq_nearest(const float2 *cb, unsigned n, flo
   double bs = vq_dist2(cb[0], RR, Het), thr = bs * kShrink;
   unsigned best = 0, c = 1;
   for (; c + 4u <= n; c += 4u) {
-    const double bs0 = bs, thr0 = thr;
+    const double bs0 = bs;
     const unsigned best0 = best;
     bool tie = false;
 #pragma unroll
     for (unsigned k = 0; k < 4u; ++k) {
-      const double s = vq_dist2(cb[c + k], RR, Het);
-      const bool win = s < thr;
+      // thr of the current best, formed here rather than kept: one multiply
+      // instead of a multiply and two selects
+      const double s = vq_dist2(cb[c + k], RR, Het), th = bs * kShrink;
+      const bool win = s < th;
       tie = tie || (!win && s < bs);
-      const double t = s * kShrink;
       bs = win ? s : bs;
-      thr = win ? t : thr;
       best = win ? c + k : best;
     }
+    thr = bs * kShrink;
     if (__builtin_expect(__ballot(tie) != 0ull, 0)) {
       bs = bs0;
-      thr = thr0;
+      thr = bs0 * kShrink;
       best = best0;
       for (unsigned k = 0; k < 4u; ++k) vq_step(cb, c + k, RR, Het, bs, thr, best);
     }
