@@ -166,7 +166,7 @@ SIGNATURES = [
       ctypes.POINTER(ctypes.c_char_p), c_u32p, ctypes.POINTER(ctypes.c_char_p),
       ctypes.POINTER(ctypes.c_char_p), c_u32p, ctypes.c_uint, ctypes.c_char_p]),
     ("glp_model_params", ctypes.c_int,
-     [ctypes.c_uint, ctypes.c_uint, c_f64p, ctypes.c_float, c_f32p, c_f32p, c_f32p]),
+     [ctypes.c_uint, ctypes.c_uint, c_f64p, ctypes.c_double, c_f32p, c_f32p, c_f32p]),
     ("glp_vq_train", ctypes.c_int,
      [c_f32p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint64, ctypes.c_size_t, c_f32p, c_u32p]),
     ("glp_vq_codes", ctypes.c_int, [c_f32p, ctypes.c_size_t, c_f32p, ctypes.c_uint, c_u8p]),

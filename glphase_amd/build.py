@@ -8,6 +8,7 @@ product and sum separately, as the reference CPU build does.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -29,12 +30,12 @@ FLAGS = [
 
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    objs = []
-    for s in srcs:
-        o = os.path.join(CSRC, "_obj_" + os.path.basename(s) + ".o")
-        cmd = [HIPCC, "-x", "hip"] + [f for f in FLAGS if f != "-shared"] + ["-c", s, "-o", o]
-        _run(cmd, verbose)
-        objs.append(o)
+    objs = [os.path.join(CSRC, "_obj_" + os.path.basename(s) + ".o") for s in srcs]
+    cmds = [[HIPCC, "-x", "hip"] + [f for f in FLAGS if f != "-shared"] + ["-c", s, "-o", o]
+            for s, o in zip(srcs, objs)]
+    # the sources compile independently: at most 8 compilers at once
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"], verbose)
     os.replace(tmp, LIB)

@@ -52,7 +52,7 @@ void mutation_matrix(float mu, float *pc) {
     }
 }
 
-void transition_matrix(const double *r_morgans, unsigned M, float inflation, float *tran) {
+void transition_matrix(const double *r_morgans, unsigned M, double inflation, float *tran) {
   tran[0] = tran[1] = tran[2] = 0;
   for (unsigned m = M - 1; m; m--) {
     float r = (float)r_morgans[m];
@@ -425,7 +425,7 @@ int set_error(int code, const char *msg);
 
 extern "C" {
 
-int glp_model_params(unsigned hn, unsigned num_sites, const double *r_morgans, float inflation,
+int glp_model_params(unsigned hn, unsigned num_sites, const double *r_morgans, double inflation,
                      float *tran_out, float *pc16_out, float *mu_out) {
   if (hn < 2 || num_sites == 0) return glp::set_error(GLP_ERR_ARG, "need hn >= 2 and num_sites > 0");
   const float mu = glp::mutation_rate(hn);

@@ -48,7 +48,7 @@ float mutation_rate(unsigned hn);
 void mutation_matrix(float mu, float *pc16);
 // tran[3m..3m+2] = ((1-r)^2, r(1-r), r^2) with r = min(r_morgans[m] *
 // inflation, 1) (insti.cpp:1277-1300); tran[0..2] = 0.
-void transition_matrix(const double *r_morgans, unsigned M, float inflation, float *tran);
+void transition_matrix(const double *r_morgans, unsigned M, double inflation, float *tran);
 
 // ---- GL vector quantisation: glVQ.hpp / glVQ.cpp ----------------------------
 class GLVQ {

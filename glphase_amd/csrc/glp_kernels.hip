@@ -135,13 +135,24 @@ __device__ __forceinline__ float dpp(float x) {
 //   (sites 3, 7, 11, ...) the pair scales its four probabilities by the
 //   power of two 2^k that brings their sum into [1, 2) (k >= 0, since the
 //   sum never exceeds 2), which rounds nothing; the lane adds up the k and
-//   the score is log(last sum) - (sum of k) ln 2 in FP64.
+//   the score is log(last sum) - (sum of k) ln 2 in FP64.  A site can lower
+//   the sum by as much as the smallest emission (mu^2 for a confident
+//   genotype that no parent carries: 2^-33 at 10,000 haplotypes), so four
+//   sites can leave the normal range.  The lane keeps the largest k; a pass
+//   whose sum fell below 2^-99 at a scaling (k >= 100) or at its end is
+//   redone with the scaling after every site (score_set), whose sums stay
+//   above the smallest emission.
 struct PairState {
   float la, lb;
   unsigned nres;  // MIXED: sum of the scale exponents so far
+  int kmax;       // MIXED: largest scale exponent so far
   float fscore;
   unsigned nq;
 };
+
+// MIXED: a scaling by 2^k with k >= kTinyK means the sum had fallen below
+// 2^-99 since the previous one
+constexpr int kTinyK = 100;
 
 constexpr double kLn2 = 0.6931471805599453;
 
@@ -179,6 +190,7 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
 // (l00 + l11) + (l01 + l10) in [1, 2) after it (frexp exponent e, k = 1 - e;
 // the sum is below 2, so k >= 0 and the scaling is exact).  Both lanes form
 // the same sum (addition commutes), so the pair scales together.
+template <bool TRACK>
 __device__ __forceinline__ void mixed_scale(PairState &st) {
   const float p = st.la + st.lb;
   const float s = dpp<kDppSwap>(p) + p;
@@ -186,6 +198,7 @@ __device__ __forceinline__ void mixed_scale(PairState &st) {
   st.la = __builtin_amdgcn_ldexpf(st.la, k);
   st.lb = __builtin_amdgcn_ldexpf(st.lb, k);
   st.nres += (unsigned)k;
+  if (TRACK) st.kmax = k > st.kmax ? k : st.kmax;
 }
 
 // The pair's l00 + l01 + l10 + l11 in the reference's order.
@@ -265,7 +278,8 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       if (TAIL && (unsigned)s >= n) break;
       pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
-      if (MIXED && (k & 3) == 3) mixed_scale(st);
+      // MIXED: every 4th site (FAST), or every site (the redo)
+      if (MIXED && (!FAST || (k & 3) == 3)) mixed_scale<FAST>(st);
     }
   }
 }
@@ -309,10 +323,12 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
                                             float norm, float nrm0, float *ring) {
   const float *tg = reinterpret_cast<const float *>(G.t4);
 #pragma unroll
-  for (int k = k0; k < k0 + 4; ++k)
+  for (int k = k0; k < k0 + 4; ++k) {
     pair_step<MIXED, FAST>(st, G.e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2],
                            (g == 0 && k == 0) ? nrm0 : norm, ring);
-  if (MIXED) mixed_scale(st);
+    if (MIXED && !FAST) mixed_scale<false>(st);  // the redo: every site
+  }
+  if (MIXED && FAST) mixed_scale<true>(st);
 }
 
 // A full 32-site chunk as a software pipeline over its 4 groups, in two
@@ -424,6 +440,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   PairState st;
   st.la = st.lb = 0.25f;
   st.nres = 0;
+  st.kmax = 0;
   st.fscore = 0.0f;
   st.nq = 0;
   const unsigned nch = (M + 31u) >> 5;
@@ -469,7 +486,11 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     chunk(ch + 1u, Y, X);
   }
   const float last = pair_sum(st);
-  if (MIXED) return log((double)last) - (double)st.nres * kLn2;
+  if (MIXED) {
+    // a sum below 2^-99 (or 0): NaN asks score_set for the redo
+    if (FAST && (st.kmax >= kTinyK || last < 0x1p-99f)) return __builtin_nan("");
+    return log((double)last) - (double)st.nres * kLn2;
+  }
   return (double)(st.fscore + cr_logf(last));
 }
 
@@ -495,9 +516,11 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
   const double v = forward_pass<BITS, MIXED, true>(f0, f1, m0, m1, W.packed + sample, N, W.tran,
                                                    W.num_sites, smem, lut_entries, wtr, ring,
                                                    norm);
-  // a subnormal (or zero) site sum made the short reciprocal NaN: the
+  // F32: a subnormal (or zero) site sum made the short reciprocal NaN: the
   // wavefront redoes its passes with the exact one (recip_normal is exact
-  // for normal sums, so a value that is not NaN is already the exact one)
+  // for normal sums, so a value that is not NaN is already the exact one).
+  // MIXED: a sum fell below 2^-99 between two scalings: the redo scales
+  // after every site.  Each lane keeps its first value unless it was NaN.
   if (__builtin_expect(__any(v != v), 0)) {
     const double x = forward_pass<BITS, MIXED, false>(f0, f1, m0, m1, W.packed + sample, N,
                                                       W.tran, W.num_sites, smem, lut_entries, wtr,

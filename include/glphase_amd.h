@@ -370,7 +370,7 @@ int glp_knn_build(const glp_knn_desc *desc, void *stream);
  * (r_morgans[m] between sites m-1 and m; geneticMap.cpp:105-111) times
  * `inflation`, capped at 1.  Any output pointer may be NULL. */
 int glp_model_params(unsigned hn, unsigned num_sites, const double *r_morgans,
-                     float inflation, float *tran_out, float *pc16_out,
+                     double inflation, float *tran_out, float *pc16_out,
                      float *mu_out);
 
 /* GLVQ::BuildCodeBook (glVQ.cpp:26-146) on n_gl GL trios: Lloyd iterations

@@ -51,7 +51,8 @@ def L():
         lib.orc_uniform_int.restype = ctypes.c_uint32
         lib.orc_unif_excl.restype = ctypes.c_uint32
         lib.orc_hmm_like_f32.restype = ctypes.c_float
-        for f in ("orc_hmm_like_mixed", "orc_hmm_like_f64", "orc_hmm_like_brute"):
+        for f in ("orc_hmm_like_mixed", "orc_hmm_like_mixed_pass", "orc_hmm_like_f64",
+                  "orc_hmm_like_brute"):
             getattr(lib, f).restype = ctypes.c_double
         lib.orc_vq_find.restype = ctypes.c_uint
         lib.orc_mu.restype = ctypes.c_float
@@ -63,7 +64,7 @@ def L():
         lib.orc_vq_codes.argtypes = [f32p, ctypes.c_size_t, f32p, ctypes.c_uint, u8p]
         lib.orc_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.c_uint32, u32p]
-        lib.orc_tran_from_morgans.argtypes = [f64p, ctypes.c_uint, ctypes.c_float, f32p]
+        lib.orc_tran_from_morgans.argtypes = [f64p, ctypes.c_uint, ctypes.c_double, f32p]
         lib.orc_hmm_work_weights.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
                                              ctypes.c_float, ctypes.c_uint, f32p, f32p]
         lib.orc_hmm_work_sample.argtypes = [f32p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint64,
@@ -147,6 +148,20 @@ def hmm_like_f32(haps, parents, emit, tran):
 
 def hmm_like_mixed(haps, parents, emit, tran):
     return _like(L().orc_hmm_like_mixed, haps, parents, emit, tran)
+
+
+def hmm_like_mixed_pass(haps, parents, emit, tran, every):
+    """One mixed-precision pass (scaled every 4th site or every site) and
+    whether a sum fell below 2^-99 in it."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    pr = np.ascontiguousarray(parents, np.uint32)
+    e = np.ascontiguousarray(emit, np.float32)
+    t = np.ascontiguousarray(tran, np.float32)
+    tiny = ctypes.c_int(0)
+    v = L().orc_hmm_like_mixed_pass(P(h, u64p), ctypes.c_uint(h.shape[1]), P(pr, u32p),
+                                    P(e, f32p), P(t, f32p), ctypes.c_uint(e.shape[0]),
+                                    ctypes.c_uint(every), ctypes.byref(tiny))
+    return v, bool(tiny.value)
 
 
 def hmm_like_f64(haps, parents, emit, tran):

@@ -195,16 +195,18 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
 /* The reference's FP32 recurrence (impute.cpp:354-366), with each state's
  * products l * t0 and l' * t2 fused with the (pair sum) * t1 term into two
  * FMAs, and without the per-site rescale test (impute.cpp:367-372).
- * Instead, after every 4th site (m = 3, 7,
- * 11, ...) the four probabilities are scaled by 2^k, where e is the frexp
- * exponent of (l00 + l11) + (l01 + l10) and k = 1 - e brings that sum into
- * [1, 2). The probabilities never sum to 2 or more, so k >= 0 and the
- * scaling rounds nothing. The score is log(last sum) - (sum of k) ln 2 in
- * FP64: the reference's score -= log(factor) sum (impute.cpp:367-374) with
- * every factor exact and the sum of logs carried as an integer. */
-double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
-                          const unsigned P[4], const float *emit,
-                          const float *tran, unsigned M) {
+ * Instead, after every 4th site (m = 3, 7, 11, ...; every = 4) or after
+ * every site including site 0 (every = 1), the four probabilities are
+ * scaled by 2^k, where e is the frexp exponent of (l00 + l11) + (l01 + l10)
+ * and k = 1 - e brings that sum into [1, 2). The probabilities never sum to
+ * 2 or more, so k >= 0 and the scaling rounds nothing. The score is
+ * log(last sum) - (sum of k) ln 2 in FP64: the reference's score -=
+ * log(factor) sum (impute.cpp:367-374) with every factor exact and the sum
+ * of logs carried as an integer.  *tiny is set when a scaling had k >= 100
+ * (the sum had fallen below 2^-99) or the last sum is below 2^-99. */
+double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                               const float *emit, const float *tran, unsigned M,
+                               unsigned every, int *tiny) {
   const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
                  *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
   const float *e = emit, *t = tran;
@@ -216,22 +218,24 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
   float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
         l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
   float b00, b01, b10, b11;
-  e += 4;
-  t += 3;
-  for (unsigned m = 1; m < M; m++, e += 4, t += 3) {
-    pe = (l00 + l11) * t[1];
-    po = (l01 + l10) * t[1];
-    b00 = fmaf(l11, t[2], fmaf(l00, t[0], po));
-    b01 = fmaf(l10, t[2], fmaf(l01, t[0], pe));
-    b10 = fmaf(l01, t[2], fmaf(l10, t[0], pe));
-    b11 = fmaf(l00, t[2], fmaf(l11, t[0], po));
-    l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
-    l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
-    l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
-    l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
-    if ((m & 3u) == 3u) {
+  for (unsigned m = 0; m < M; m++, e += 4, t += 3) {
+    if (m) {
+      pe = (l00 + l11) * t[1];
+      po = (l01 + l10) * t[1];
+      b00 = fmaf(l11, t[2], fmaf(l00, t[0], po));
+      b01 = fmaf(l10, t[2], fmaf(l01, t[0], pe));
+      b10 = fmaf(l01, t[2], fmaf(l10, t[0], pe));
+      b11 = fmaf(l00, t[2], fmaf(l11, t[0], po));
+      l00 = b00 * e[(hbit(f0, m) << 1) | hbit(m0, m)];
+      l01 = b01 * e[(hbit(f0, m) << 1) | hbit(m1, m)];
+      l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
+      l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
+    }
+    if (every == 1 || (m & 3u) == 3u) {
       (void)frexpf((l00 + l11) + (l01 + l10), &ex);
       const int k = 1 - ex;
+      if (k >= 100)
+        *tiny = 1;
       l00 = ldexpf(l00, k);
       l01 = ldexpf(l01, k);
       l10 = ldexpf(l10, k);
@@ -239,7 +243,23 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
       ksum += k;
     }
   }
-  return log((double)(l00 + l01 + l10 + l11)) - (double)ksum * ORC_LN2;
+  const float last = l00 + l01 + l10 + l11;
+  if (last < 0x1p-99f)
+    *tiny = 1;
+  return log((double)last) - (double)ksum * ORC_LN2;
+}
+
+/* Mixed precision: the pass scaled every 4th site; when its sum fell below
+ * 2^-99 (a run of sites whose emissions are all near mu^2) or it is NaN,
+ * the pass scaled after every site. */
+double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
+                          const unsigned P[4], const float *emit,
+                          const float *tran, unsigned M) {
+  int tiny = 0;
+  const double v = orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 4, &tiny);
+  if (!tiny && v == v)
+    return v;
+  return orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 1, &tiny);
 }
 
 /* FP64 forward oracle: the same model, normalised at every site. */
@@ -612,9 +632,11 @@ void orc_pc(float mu, float *pc) {
     }
 }
 
-/* insti.cpp:1277-1300 with r from a genetic map (Morgans); tran[0..2] unused */
+/* insti.cpp:1277-1300 with r from a genetic map (Morgans, converted to the
+ * float r), times the inflation factor (a double, insti.hpp:86), capped at
+ * 1; tran[0..2] unused */
 void orc_tran_from_morgans(const double *r_morgans, unsigned M,
-                           float inflation, float *tran) {
+                           double inflation, float *tran) {
   tran[0] = tran[1] = tran[2] = 0;
   for (unsigned m = M - 1; m; m--) {
     float r = (float)r_morgans[m];

@@ -67,6 +67,11 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M);
+/* one mixed-precision pass, scaled every 4th site (every = 4) or after every
+ * site (every = 1); *tiny is set when a sum fell below 2^-99 */
+double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                               const float *emit, const float *tran, unsigned M,
+                               unsigned every, int *tiny);
 double orc_hmm_like_f64(const uint64_t *haps, unsigned wn, const unsigned P[4],
                         const float *emit, const float *tran, unsigned M);
 double orc_hmm_like_brute(const uint64_t *haps, unsigned wn,
@@ -125,7 +130,7 @@ void orc_pack_nonvq(const float *gls, unsigned nsamp, unsigned M,
 float orc_mu(unsigned hn);
 void orc_pc(float mu, float *pc);
 void orc_tran_from_morgans(const double *r_morgans, unsigned M,
-                           float inflation, float *tran);
+                           double inflation, float *tran);
 void orc_tran_from_positions(const unsigned *pos, unsigned M, unsigned hn, double density,
                              double inflation, float *tran);
 

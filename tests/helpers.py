@@ -145,3 +145,41 @@ def count_subnormal_sums(p, samples, parents, emit_of):
                 l = [f32(x * sc) for x in l]
         hits += int(found)
     return hits
+
+
+def hom_alt_run_problem(hn, N=64, M=300, runs=((40, 12), (150, 5), (262, 30)), seed=0):
+    """A problem whose emissions reach the smallest value of the reference's
+    model: the mutation matrix of a panel of hn haplotypes (mu = 1/H_{hn-1},
+    then mu/(hn + mu), insti.cpp:1254-1270) and, over runs of sites (start,
+    length), GLs certain of the alternative homozygote (codeword (0, 0):
+    AA = 1) while every haplotype of the panel carries the reference allele.
+    There every parent state emits pc[0][3] = mu^2 (2^-33.2 at hn = 10,000,
+    2^-37.5 at 40,000), so four such sites take the state sum below 2^-126.
+    Elsewhere codes and haplotypes are random."""
+    rng = np.random.default_rng(seed)
+    bits = 4
+    H = 2 * N
+    wn = (M + 63) // 64
+    panel = rng.integers(0, np.iinfo(np.uint64).max, (H, wn), dtype=np.uint64, endpoint=True)
+    if M % 64:
+        panel[:, -1] &= np.uint64((1 << (M % 64)) - 1)
+    cb = np.zeros((16, 2), np.float32)
+    cb[0] = (0.0, 1.0)
+    cb[1] = (0.0, 0.0)  # AA = 1
+    cb[2] = (1.0, 0.0)
+    rr = rng.random(13).astype(np.float32)
+    cb[3:, 0] = rr
+    cb[3:, 1] = (rng.random(13).astype(np.float32) * (np.float32(1) - rr)).astype(np.float32)
+    codes = rng.integers(0, 16, size=(N, M)).astype(np.uint8)
+    bitmask = np.zeros(M, bool)
+    for a, n in runs:
+        codes[:, a:a + n] = 1
+        bitmask[a:a + n] = True
+    for m in np.nonzero(bitmask)[0]:
+        panel[:, m >> 6] &= ~np.uint64(1 << (int(m) & 63))
+    packed = host.pack_codes(codes, bits)
+    r = np.full(M, 2e-4)
+    r[0] = 0.0
+    tran, pc, mu = host.model_params(hn, r)
+    return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, codes=codes,
+                bits=bits, N=N, M=M, H=H, mu=mu)

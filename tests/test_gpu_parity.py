@@ -225,7 +225,6 @@ def test_klist_mode(orc):
     job.make_plan(8)
     job.run_batched(gen=0, pen=pen, seed=seed)
     compare_run(orc, p, job, C, pen, seed, PREC_F32, prop_mode=1, klist=kl)
-    assert np.isin(job.parents()[0], kl).all() or True
     for i in range(p["N"]):
         assert set(job.parents()[0][i]) <= set(kl[i])
 
@@ -381,6 +380,28 @@ def test_subnormal_site_sums_mh(orc, prec, form):
         job.make_plan(4)
         job.run_batched(gen=2, pen=pen, seed=seed)
     compare_run(orc, p, job, C, pen, seed, prec, gen=2)
+
+
+@pytest.mark.parametrize("hn", [10_000, 40_000])
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_mixed_smallest_emissions(orc, hn, form):
+    # runs of sites where every parent state emits mu^2 (a certain
+    # alternative homozygote against all-reference parents) at the mutation
+    # rate of hn haplotypes: a 4-site sum falls below 2^-126, the wavefront
+    # redoes its passes with the scaling after every site, and parents and
+    # log-likelihoods equal the oracle's; every log-likelihood is finite
+    say(f"mixed precision, smallest emissions hn={hn} form={form}")
+    p = helpers.hom_alt_run_problem(hn, seed=hn % 97)
+    C, pen, seed = 20, 1.0, 4242
+    job = job_for(p, C, precision=PREC_MIXED, trace=True)
+    if form == "plain":
+        job.run_plain(gen=1, pen=pen, seed=seed)
+    else:
+        job.make_plan(4)
+        job.run_batched(gen=1, pen=pen, seed=seed)
+    compare_run(orc, p, job, C, pen, seed, PREC_MIXED, gen=1)
+    tl, _ = job.trace()
+    assert np.isfinite(tl[0]).all()
 
 
 @pytest.mark.parametrize("M", [1, 3, 31, 33, 95, 97, 301])

@@ -147,3 +147,45 @@ def test_sample_emit_decodes_packed_codes(orc):
     for i in range(N):
         e = orc.sample_emit(packed, i, M, bits_, lut)
         assert (e == lut[codes[i]]).all()
+
+
+@pytest.mark.parametrize("hn", [10_000, 40_000])
+def test_mixed_smallest_emissions_match_fp64(orc, hn):
+    # runs of sites where every parent state emits mu^2 (a certain
+    # alternative homozygote against all-reference parents) take a 4-site
+    # sum below 2^-126 at these panel sizes (mu^2 = 2^-33.2 and 2^-37.5);
+    # the 4-site pass flags it and the mixed result comes from the pass
+    # scaled after every site, within 1e-7 of FP64
+    from tests import helpers
+    p = helpers.hom_alt_run_problem(hn)
+    lut = orc.emit_lut(p["codebook"], p["pc"])
+    mu2 = np.float32(p["mu"]) * np.float32(p["mu"])
+    assert lut[1, 0] == pytest.approx(mu2, rel=1e-6)  # pc[0][3] * AA
+    rng = np.random.default_rng(5)
+    worst_unguarded = 0.0
+    for _ in range(12):
+        i = int(rng.integers(p["N"]))
+        P = rng.choice([h for h in range(p["H"]) if h // 2 != i], 4).astype(np.uint32)
+        emit = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut)
+        f64 = orc.hmm_like_f64(p["panel"], P, emit, p["tran"])
+        v4, tiny4 = orc.hmm_like_mixed_pass(p["panel"], P, emit, p["tran"], 4)
+        v1, tiny1 = orc.hmm_like_mixed_pass(p["panel"], P, emit, p["tran"], 1)
+        assert tiny4 and not tiny1
+        got = orc.hmm_like_mixed(p["panel"], P, emit, p["tran"])
+        assert got == v1
+        assert abs(got - f64) <= 1e-7 * abs(f64)
+        worst_unguarded = max(worst_unguarded, abs(v4 - f64) / abs(f64) if np.isfinite(v4) else np.inf)
+    # without the guard the 4-site pass is off by more than the 1e-5 bound
+    assert worst_unguarded > 1e-5
+
+
+def test_mixed_guard_keeps_ordinary_passes(orc):
+    # a pass whose sums stay above 2^-99 is the 4-site pass itself
+    haps, emit, tran = problem(2000, seed=7)
+    P = np.array([1, 6, 3, 2], np.uint32)
+    v4, tiny = orc.hmm_like_mixed_pass(haps, P, emit, tran, 4)
+    assert not tiny
+    assert orc.hmm_like_mixed(haps, P, emit, tran) == v4
+    v1, _ = orc.hmm_like_mixed_pass(haps, P, emit, tran, 1)
+    f64 = orc.hmm_like_f64(haps, P, emit, tran)
+    assert abs(v1 - f64) <= 1e-7 * abs(f64)

@@ -166,22 +166,59 @@ def test_norm_value(orc):
 
 
 def test_model_params(orc):
-    # mu: 1/H_{hn-1} then mu/(hn+mu), in float as insti.cpp:1254-1270
-    for hn in (4, 200, 10000):
-        mu = 0.0
-        for i in range(1, hn):
-            mu = float(np.float32(mu + 1.0 / i))
-        mu = np.float32(np.float32(1) / np.float32(mu))
-        mu = np.float32(mu / np.float32(np.float32(hn) + mu))
-        assert orc.mu(hn) == mu
-    pc = orc.pc(np.float32(0.01))
-    assert pc[0, 0] == pc[3, 3] == np.float32(np.float32(0.99) * np.float32(0.99))
-    assert pc[0, 3] == pc[1, 2] == np.float32(np.float32(0.01) * np.float32(0.01))
-    assert pc[0, 1] == pc[2, 3] == pc[1, 0]
-    # rows of pc are the allele-flip probabilities: each row sums to 1
-    assert np.allclose(pc.sum(1), 1.0, atol=1e-6)
-    r = np.array([0, 1e-6, 0.3, 2.0])
-    t = orc.tran_from_morgans(r)
+    # mu = 1/H_{hn-1}, then mu/(hn + mu) (insti.cpp:1254-1270), against
+    # exact fractions worked by hand: hn = 4 gives H_3 = 11/6, 1/H = 6/11,
+    # (6/11)/(4 + 6/11) = 3/25 = 0.12; an off-by-one harmonic index (H_4 =
+    # 25/12 gives 12/(25*4 + 12) = 3/28) or a missing /(hn + mu) fails
+    from fractions import Fraction as F
+    for hn, want in ((2, F(1, 3)), (3, F(2, 11)), (4, F(3, 25)), (5, F(12, 137))):
+        assert abs(orc.mu(hn) - float(want)) <= 4e-7 * float(want), hn
+    # large hn: the FP64 harmonic sum (math.fsum) gives mu within the float
+    # accumulation error of hn terms
+    import math
+    for hn in (200, 2000, 10_000, 40_000):
+        Hs = math.fsum(1.0 / i for i in range(1, hn))
+        want = (1.0 / Hs) / (hn + 1.0 / Hs)
+        assert abs(orc.mu(hn) - want) <= 5e-4 * want, hn
+    # pc: (1-mu)^2 on the diagonal, mu^2 where both alleles flip (i ^ j == 3:
+    # (0,3), (1,2), (2,1), (3,0)), mu(1-mu) for one flip (insti.cpp:1307-1315)
+    mu = np.float32(0.01)
+    pc = orc.pc(mu)
+    m = 0.01
+    for i in range(4):
+        for j in range(4):
+            d = i ^ j
+            want = (1 - m) ** 2 if d == 0 else (m * m if d == 3 else m * (1 - m))
+            assert abs(pc[i, j] - want) <= 1e-6 * want, (i, j)
+    # tran = ((1-r)^2, r(1-r), r^2) with r = min(r_morgans * inflation, 1)
+    # (insti.cpp:1284-1299), against FP64 with inflation 1.5 and two capped
+    # sites; site 0 is unused and stays 0
+    r = np.array([0.25, 1e-6, 0.003, 0.3, 0.7, 2.0])
+    t = orc.tran_from_morgans(r, 1.5)
     assert (t[0] == 0).all()
-    assert t[3, 0] == 0 and t[3, 2] == 1  # r capped at 1 (insti.cpp:1293)
+    for k in (1, 2, 3):
+        rr = float(np.float32(r[k])) * 1.5
+        want = ((1 - rr) ** 2, rr * (1 - rr), rr * rr)
+        assert np.allclose(t[k], want, rtol=1e-6, atol=0), k
+    for k in (4, 5):  # 1.05 and 3.0 are capped at 1
+        assert tuple(t[k]) == (0.0, 0.0, 1.0), k
+    # inflation 1 keeps r: 0.7 is not capped
+    t1 = orc.tran_from_morgans(r, 1.0)
+    assert np.allclose(t1[4], (0.09, 0.21, 0.49), rtol=1e-6)
+    # doubly stochastic: t0 + 2 t1 + t2 = 1
     assert np.allclose(t[1:, 0] + 2 * t[1:, 1] + t[1:, 2], 1.0, atol=1e-6)
+
+
+def test_inflation_is_a_double(orc):
+    # the inflation factor is a double (insti.hpp:86): r is the float of
+    # (double)r * inflation, which for 1.1 differs from the float product
+    # (float)r * (float)1.1 at some sites
+    from glphase_amd import host
+    r = np.linspace(1e-4, 0.05, 4000)
+    t_orc = orc.tran_from_morgans(r, 1.1)
+    tran, _, _ = host.model_params(100, r, inflation=1.1)
+    assert np.array_equal(t_orc[1:], tran[1:])
+    want = np.array([np.float32(np.float64(np.float32(x)) * 1.1) for x in r[1:]])
+    assert np.array_equal(t_orc[1:, 2], want * want)
+    flt = np.array([np.float32(np.float32(x) * np.float32(1.1)) for x in r[1:]])
+    assert (flt != want).any()
