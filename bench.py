@@ -157,7 +157,7 @@ def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64):
     from glphase_amd import PREC_MIXED, PROP_KLIST, host
     from glphase_amd.engine import Job
     N, M = d["gls"].shape[0], d["gls"].shape[1]
-    tran, pc, _ = host.model_params(panel.shape[0], d["r_morgans"])
+    tran, pc, _ = host.model_params(2 * N, d["r_morgans"])  # hn = 2N (insti.cpp:1205)
     cb, _ = host.vq_train(d["gls"], 256, key=seed, training_size=VQ_TRAINING, device=True)
     packed = host.pack_codes(host.vq_codes(d["gls"], cb, device=True), 8)
     job = Job([dict(panel=panel, packed=packed, tran=tran)], N, panel.shape[0], cb, pc, bits=8,

@@ -42,6 +42,13 @@ class PhaseRun:
 
     het_aa: [M][N][2] GL pairs (host.read_stbin's drop_first layout).
     pos: [M] site positions (transitions without a genetic map) or tran: [M][3].
+
+    knn_k > 0 restricts candidates to K-lists built on the device from the
+    current panel (sample haplotypes plus ref_haps) at generation sa_burnin
+    and, with recluster_every, every n sampling generations.  This is an
+    extension: the reference builds its kNN sampler only from a scaffold at
+    the start of Insti::estimate and stops with "kNN sampler with no
+    scaffold is not implemented yet" without one (insti.cpp:1578-1586).
     """
 
     def __init__(self, het_aa, pos=None, tran=None, ref_haps=None, bits=8, cycles=None,
@@ -62,7 +69,9 @@ class PhaseRun:
         t0 = time.perf_counter()
         if tran is None:
             tran = host.tran_from_positions(pos, 2 * N)
-        _, pc, _ = host.model_params(self.H, np.zeros(M))
+        # mu from the sample haplotypes only: hn = 2N (insti.cpp:1205,
+        # 1256-1270); reference haplotypes only widen the candidate range
+        _, pc, _ = host.model_params(2 * N, np.zeros(M))
         trios = genotype_trios(g)
         cb, self.vq_iters = host.vq_train(trios, 1 << bits, key=seed, training_size=training_size,
                                           device=True)
