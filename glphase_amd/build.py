@@ -45,14 +45,19 @@ def build(verbose=False):
 
 
 def build_cpp_tests(verbose=False):
-    """The C++ drop-in test program tests/cpp/test_hmmlike (links the
-    library with an rpath to its directory)."""
-    src = os.path.join(ROOT, "tests", "cpp", "test_hmmlike.cpp")
-    out = os.path.join(ROOT, "tests", "cpp", "test_hmmlike")
-    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
-           src, "-o", out, "-L", HERE, "-lglphase_amd", "-Wl,-rpath," + HERE]
-    _run(cmd, verbose)
-    return out
+    """The C++ drop-in test programs tests/cpp/test_hmmlike (the host
+    classes) and tests/cpp/test_hmmlike_gsl (the reference-signature adapter
+    of include/glphase_amd_hmmlike.hpp), linked to the library with an rpath
+    to its directory."""
+    outs = []
+    for name in ("test_hmmlike", "test_hmmlike_gsl"):
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        out = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+               src, "-o", out, "-L", HERE, "-lglphase_amd", "-Wl,-rpath," + HERE]
+        _run(cmd, verbose)
+        outs.append(out)
+    return outs[0]
 
 
 def _run(cmd, verbose):

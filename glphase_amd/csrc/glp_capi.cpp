@@ -129,6 +129,7 @@ static void plan_graph_reset(glp_plan *p) {
 // issue slots and every pass slows down.
 // a wavefront holds 32 lane pairs, floor(32 / Q) chain slots
 constexpr unsigned kMaxPasses = 32;
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 
 static double waves_for(unsigned nchains, unsigned Q) {
   return std::ceil((double)nchains / (double)(32u / Q));
@@ -201,13 +202,6 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   a.bits = d->bits_per_code;
   a.lut_entries = 16u << d->bits_per_code;
   a.precision = d->precision;
-  // a wave walks every window once and serves each of its tasks in at most
-  // C + 1 rounds; bound the persistent loop well above that
-  {
-    const double bound = 2.0 * d->num_windows + 64.0 * (double)d->num_samples * d->num_windows *
-                                                    ((double)d->num_cycles + 2.0) + 1024.0;
-    a.max_rounds = bound > 4.0e9 ? 4000000000u : (unsigned)bound;
-  }
   return GLP_OK;
 }
 
@@ -277,7 +271,12 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
     delete p;
     return rc;
   }
-  if (p->bpc < 1) p->bpc = 1;
+  // a launch that cannot be resident is refused here rather than left to
+  // fail (or to run with a grid sized from a bad occupancy figure)
+  if (lds_bytes(a) > kLdsPerCu || p->bpc < 1) {
+    delete p;
+    return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
+  }
   p->nwin = desc->num_windows;
   p->wins_h.assign(desc->windows, desc->windows + p->nwin);
   p->order.resize(p->nwin);

@@ -29,7 +29,6 @@ struct KArgs {
   unsigned lut_entries;  // 16 * 2^bits
   unsigned bits;
   int precision;
-  unsigned max_rounds;   // persistent-loop bound of the batched kernel
 };
 
 int set_error(int code, const char *msg);

@@ -667,13 +667,17 @@ __global__ void __launch_bounds__(256)
   unsigned S[4] = {0, 0, 0, 0};
   double cur = 0.0;
   unsigned long long n_eval = 0, n_useful = 0;
-  // every task needs at most C + 1 rounds; a wave can never run more than
-  // max_rounds (set by the host from the task count) -- if it does, it flags
-  // the run as failed and exits instead of spinning
-  unsigned rounds = 0;
+  // Round budget.  In every round each active slot decides at least one
+  // proposal (the first one it scores from its current set is always
+  // valid), so a task takes at most C + 1 rounds, and a round with no
+  // active slot moves the wave to its next window.  The wave's rounds
+  // therefore never exceed (C + 1) per task it drew plus one per window
+  // and one to leave; a wave past that budget flags the run as failed and
+  // exits instead of spinning.
+  unsigned long long rounds = 0, budget = (unsigned long long)nwin + 2ull;
 
   for (;;) {
-    if (++rounds > a.max_rounds) {
+    if (++rounds > budget) {
       if (lane == 0) atomicOr(&ctrs[3], 1ull);
       break;
     }
@@ -686,7 +690,9 @@ __global__ void __launch_bounds__(256)
         if (lane == 0) base = atomicAdd(&wctr[bw], (unsigned)__popcll(wm));
         base = __builtin_amdgcn_readfirstlane(base);
         const unsigned rank = (unsigned)__popcll(wm & lanes_below);
-        unsigned t = (want && base + rank < a.N) ? base + rank : 0xFFFFFFFFu;
+        const bool took = want && base + rank < a.N;
+        budget += (unsigned long long)__popcll(__ballot(took)) * ((unsigned long long)a.C + 1ull);
+        unsigned t = took ? base + rank : 0xFFFFFFFFu;
         t = __shfl(t, sbase, 64);
         if (!active && t != 0xFFFFFFFFu) {
           i = t;

@@ -236,7 +236,7 @@ __device__ __forceinline__ void fwd_block(const WorkLane &L, const WorkBlock &cu
 // backward recursion.  Returns whether the last state is a proper one.
 template <bool FAST>
 __device__ __forceinline__ bool work_forward(const WorkLane &L) {
-  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
+  const unsigned M = L.M, N = L.N, i = L.i;
   const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
   float l = 0;
   WorkBlock x, y;

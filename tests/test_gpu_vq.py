@@ -115,6 +115,42 @@ def test_vq_assign_device_near_ties(orc):
                 assert want == 0  # the later codeword lost the tie
 
 
+def test_vq_assign_device_near_ties_in_groups(orc):
+    # the search takes codewords 1.. in branch-free groups of 4 (then a
+    # scalar tail) and redoes a group when a lane of the wavefront met a near
+    # tie in it: near-tie pairs placed inside a group (2/3, 9/10), across a
+    # group boundary (4/5, 8/9) and in the tail (13/14) of a 16-word book
+    # padded with far codewords, in both orders, with 1,024 points per launch
+    # so that lanes at the tie (the origin) share wavefronts with lanes that
+    # have none
+    say("vq assign: near ties inside and across 4-codeword groups")
+    import torch
+    from glphase_amd._lib import c_f32p, check, lib, ptr
+    eq, lt = _near_ties(6, seed=5)
+    rng = np.random.default_rng(9)
+    n = 1024
+    pts = rng.random((n, 2)).astype(np.float32)
+    pts[::37] = 0.0
+    dev = torch.from_numpy(pts).cuda()
+    out = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    far = np.array([[3.0 + k, 3.0 + 0.5 * k] for k in range(16)], np.float32)
+    ties = 0
+    for a, b1, b2 in eq + lt:
+        for i, j in ((2, 3), (9, 10), (4, 5), (8, 9), (13, 14)):
+            for first, second in (((a, 0), (b1, b2)), ((b1, b2), (a, 0))):
+                cb = far.copy()
+                cb[i], cb[j] = first, second
+                check(lib().glp_vq_assign_device(dev.data_ptr(), n, 0, ptr(cb, c_f32p), 16,
+                                                 out.data_ptr(), stream), "glp_vq_assign_device")
+                got = out.cpu().numpy()
+                want = np.array([orc.vq_find(cb, float(r), float(h)) for r, h in pts], np.uint8)
+                assert (got == want).all(), (i, j, cb[i].tolist(), cb[j].tolist())
+                assert set(got[::37]) <= {i, j}
+                ties += int(got[0] == i)
+    assert ties > 0
+
+
 def test_vq_full_benchmark_gls(orc):
     # the GL set of the MH update benchmark: 5,000 samples x 4,096 sites
     say("vq 5000 x 4096 GLs")
