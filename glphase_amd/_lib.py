@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libglphase_amd.so")
 GLP_OK = 0
 PROP_UNIFORM, PROP_KLIST, PROP_HOST = 0, 1, 2
 PREC_F32, PREC_MIXED = 0, 1
+GLP_NOALLOC_MAX_WINDOWS = 1024  # include/glphase_amd.h
 
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -124,6 +125,8 @@ SIGNATURES = [
      [ctypes.POINTER(GlpMhDesc), ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]),
     ("glp_plan_run", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p]),
+    ("glp_mh_run_noalloc", ctypes.c_int,
+     [ctypes.POINTER(GlpMhDesc), ctypes.c_uint, ctypes.c_void_p, c_u64p]),
     ("glp_plan_run_plain", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p]),
     ("glp_plan_stats", ctypes.c_int, [ctypes.c_void_p, c_u64p, c_u64p]),

@@ -307,6 +307,26 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   return GLP_OK;
 }
 
+int glp_mh_run_noalloc(const glp_mh_desc *desc, unsigned lanes_per_chain, void *stream,
+                       uint64_t *stats_out) {
+  if (desc && desc->num_windows > GLP_NOALLOC_MAX_WINDOWS)
+    return set_error(GLP_ERR_ARG, "glp_mh_run_noalloc: more than GLP_NOALLOC_MAX_WINDOWS windows");
+  KArgs a;
+  GLP_TRY(fill_args(desc, a));
+  if (lanes_per_chain > kMaxPasses)
+    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 (automatic) or 1..32");
+  int dev = 0, num_cu = 1;
+  GLP_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+  GLP_TRY(hip_check(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev),
+                    "device attribute"));
+  if (lds_bytes(a) > kLdsPerCu)
+    return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
+  const unsigned Q = lanes_per_chain ? lanes_per_chain
+                                     : auto_lanes(a.N * desc->num_windows, a.C, num_cu);
+  return run_noalloc(a, desc->windows, desc->num_windows, Q, num_cu,
+                     static_cast<hipStream_t>(stream), stats_out);
+}
+
 int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
   if (!p) return set_error(GLP_ERR_ARG, "null plan");
   KArgs a = p->a;

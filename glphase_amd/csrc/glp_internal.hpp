@@ -47,6 +47,10 @@ const void *batched_kernel_fn(unsigned bits, bool mixed);
 const void *reset_kernel_fn();
 int prepare_kernel_lds(const void *fn, size_t bytes);
 int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu);
+// One batched generation with every device buffer beyond the caller's in
+// the file-scope scratch array of glp_kernels.hip (glp_mh_run_noalloc).
+int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigned Q,
+                int num_cu, hipStream_t stream, uint64_t *stats_out);
 int launch_eval(const KArgs &a, const glp_window *d_wins,
                 const unsigned *d_samples, const unsigned *d_parents,
                 unsigned n, float *d_out, hipStream_t stream);

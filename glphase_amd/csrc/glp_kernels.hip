@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -948,6 +949,99 @@ int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samp
     hipLaunchKernelGGL((k_eval<4>), grid, block, smem, stream, d_wins, af, d_samples,
                        d_parents, n, d_out);
   return hip_check(hipGetLastError(), "k_eval launch");
+}
+
+// ---------------------------------------------------------------------------
+// the batched generation without allocations (glp_mh_run_noalloc)
+// ---------------------------------------------------------------------------
+// Every device buffer of a batched run beyond the caller's inputs and
+// outputs -- statistics, per-window task counters, the longest-first window
+// order, the lanes table and the window table -- sits in this file-scope
+// array, one copy per device, held until the process ends.  Host code uses
+// the per-device address from hipGetSymbolAddress (noalloc_scratch), never
+// the array's name.  Layout (bytes): [0, 32) statistics (4 x u64),
+// [256, ...) task counters, order, lanes (u32 each, kNoAllocMaxWindows
+// apiece), then the window table (16-byte aligned).
+constexpr unsigned kNoAllocMaxWindows = GLP_NOALLOC_MAX_WINDOWS;
+constexpr size_t kNoAllocWctr = 256;
+constexpr size_t kNoAllocOrder = kNoAllocWctr + 4 * (size_t)kNoAllocMaxWindows;
+constexpr size_t kNoAllocQtab = kNoAllocOrder + 4 * (size_t)kNoAllocMaxWindows;
+constexpr size_t kNoAllocWins = kNoAllocQtab + 4 * (size_t)kNoAllocMaxWindows;
+constexpr size_t kNoAllocBytes = kNoAllocWins + sizeof(glp_window) * (size_t)kNoAllocMaxWindows;
+static_assert(kNoAllocWins % 16 == 0, "window table alignment");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// order = the windows by site count, longest first, ties in index order (the
+// plan's stable sort); qtab = Q for every window.  One thread per window.
+__global__ void k_noalloc_order(const glp_window *__restrict__ wins, unsigned nwin, unsigned Q,
+                                unsigned *__restrict__ order, unsigned *__restrict__ qtab) {
+  const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwin) return;
+  const unsigned m = wins[w].num_sites;
+  unsigned rank = 0;
+  for (unsigned v = 0; v < nwin; ++v) {
+    const unsigned mv = wins[v].num_sites;
+    rank += (mv > m || (mv == m && v < w)) ? 1u : 0u;
+  }
+  order[rank] = w;
+  qtab[w] = Q;
+}
+
+// The scratch array's address on device `dev` (the current device), looked
+// up once per device.
+constexpr int kNoAllocMaxDevices = 64;
+
+static int noalloc_scratch(int dev, unsigned char **out) {
+  static std::mutex mu;
+  static unsigned char *addr[kNoAllocMaxDevices] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return set_error(GLP_ERR_ARG, "bad device");
+  if (!addr[dev]) {
+    void *p = nullptr;
+    GLP_TRY_K(hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address"));
+    addr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = addr[dev];
+  return GLP_OK;
+}
+
+int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigned Q,
+                int num_cu, hipStream_t stream, uint64_t *stats_out) {
+  if (nwin > kNoAllocMaxWindows)
+    return set_error(GLP_ERR_ARG, "glp_mh_run_noalloc: more than GLP_NOALLOC_MAX_WINDOWS windows");
+  int dev = 0;
+  GLP_TRY_K(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+  unsigned char *base = nullptr;
+  GLP_TRY_K(noalloc_scratch(dev, &base));
+  auto *ctrs = reinterpret_cast<unsigned long long *>(base);
+  auto *wctr = reinterpret_cast<unsigned *>(base + kNoAllocWctr);
+  auto *order = reinterpret_cast<unsigned *>(base + kNoAllocOrder);
+  auto *qtab = reinterpret_cast<unsigned *>(base + kNoAllocQtab);
+  auto *wins = reinterpret_cast<glp_window *>(base + kNoAllocWins);
+  const size_t smem = lds_bytes(a);
+  int bpc = 0;
+  GLP_TRY_K(batched_occupancy(a.bits, a.precision != kPrecF32, smem, &bpc));
+  if (bpc < 1) return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
+  // the plan's grid: one wave per floor(32 / Q) chains, 4 waves a block
+  const double waves = (double)nwin * std::ceil((double)a.N / (double)(32u / Q));
+  const unsigned blocks = (unsigned)std::ceil(waves / 4.0);
+  const unsigned grid = std::max(1u, std::min((unsigned)(bpc * num_cu), blocks));
+  GLP_TRY_K(hip_check(hipMemcpyAsync(wins, wins_h, sizeof(glp_window) * nwin,
+                                     hipMemcpyHostToDevice, stream), "window table copy"));
+  hipLaunchKernelGGL(k_noalloc_order, dim3((nwin + 255) / 256), dim3(256), 0, stream, wins, nwin,
+                     Q, order, qtab);
+  GLP_TRY_K(hip_check(hipGetLastError(), "k_noalloc_order launch"));
+  GLP_TRY_K(launch_batched(a, wins, order, qtab, nwin, wctr, ctrs, grid, stream));
+  if (stats_out) {
+    unsigned long long h[4];
+    GLP_TRY_K(hip_check(hipStreamSynchronize(stream), "synchronize"));
+    GLP_TRY_K(hip_check(hipMemcpy(h, ctrs, sizeof(h), hipMemcpyDeviceToHost), "stats copy"));
+    stats_out[0] = h[1];
+    stats_out[1] = h[2];
+    stats_out[2] = h[3];
+    if (h[3]) return set_error(GLP_ERR_STATE, "batched kernel hit its round bound");
+  }
+  return GLP_OK;
 }
 
 }  // namespace glp

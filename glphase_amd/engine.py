@@ -183,6 +183,17 @@ class Job:
         check(lib().glp_plan_graph_run_plain(self.plan, gen, pen, seed, self.stream_handle()),
               "glp_plan_graph_run_plain")
 
+    def run_noalloc(self, gen=0, pen=1.0, seed=0, lanes=0, stats=False):
+        """One batched generation through glp_mh_run_noalloc (no plan, no
+        device allocation); with stats=True waits and returns (passes
+        evaluated, useful passes, round-bound flag)."""
+        self.desc.gen, self.desc.pen, self.desc.seed = gen, pen, seed
+        out = np.zeros(3, np.uint64)
+        check(lib().glp_mh_run_noalloc(ctypes.byref(self.desc), lanes, self.stream_handle(),
+                                       ptr(out, c_u64p) if stats else None),
+              "glp_mh_run_noalloc")
+        return tuple(int(x) for x in out) if stats else None
+
     def run_plain_desc(self, gen=0, pen=1.0, seed=0):
         """The plain path through the stateless glp_plain_init / glp_plain_step."""
         self.desc.gen, self.desc.pen, self.desc.seed = gen, pen, seed

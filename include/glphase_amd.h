@@ -216,6 +216,26 @@ int glp_plan_graph_run(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
 int glp_plan_graph_run_plain(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                              void *stream);
 
+/* One batched generation that allocates nothing: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as glp_plan_create(desc, lanes_per_chain) +
+ * glp_plan_run(plan, desc->gen, desc->pen, desc->seed): the window table,
+ * window order, lanes table, task counters and statistics live in a
+ * 61,696-byte device array of the library (one per device, held until the
+ * process ends) instead of plan buffers.  At most GLP_NOALLOC_MAX_WINDOWS
+ * windows (larger descriptors are refused with GLP_ERR_ARG); the caller's
+ * window buffers, emission table and traces are used as given.
+ * Stream-ordered; with stats_out (host, 3 values: passes evaluated, useful
+ * passes, round-bound flag) the call waits for `stream` and reports the
+ * run's counters, returning GLP_ERR_STATE if the round bound was hit.
+ * Every call on a device shares that device's array and leaves the last
+ * call's contents in it: calls that overlap on one device (on different
+ * streams or threads) race. */
+#define GLP_NOALLOC_MAX_WINDOWS 1024
+int glp_mh_run_noalloc(const glp_mh_desc *desc, unsigned lanes_per_chain, void *stream,
+                       uint64_t *stats_out);
+
 /* Self-test of the device reciprocals of the rescale and resampling steps
  * (no counterpart in the reference): compares them with the IEEE division
  * 1.0f / x for every float x with bit pattern in [lo_bits, hi_bits) on the
