@@ -6,10 +6,13 @@ generation work, insti.cpp:1668-1705).  Site-evals (SE) per step =
 N * (C + 1) * M: one forward-recurrence step of one chain at one site.
 
 Default workload: 5,000 samples x 4,096 sites, 64 proposals per sample,
-synthetic founder-mosaic haplotypes and Poisson(2)-depth GLs, 8-bit VQ codes,
-batched persistent kernel with FP32 recurrence and FP64 log accumulation.
-The plain path (one launch per MH iteration) runs on the same inputs, and the
-CPU oracle (C, -O3) is timed on a bounded sample of the same chains.
+synthetic founder-mosaic haplotypes and Poisson(2)-depth GLs, 8-bit VQ codes
+(codebook trained on every GL), batched persistent kernel with FP32
+recurrence and FP64 log accumulation.  The plain path (one launch per MH
+iteration) runs on the same inputs; one generation is replayed on the host
+against an FP64 forward pass (log-likelihood error, accept decisions); the
+other benchmark workloads run beside it, and the CPU oracle (C, -O3) is
+timed on a bounded sample of each.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -41,15 +44,153 @@ def make_workload(N, M, C, depth, seed):
     panel = synth.pack_haps(hb)
     tran, pc, mu = host.model_params(2 * N, d["r_morgans"])
     t1 = time.perf_counter()
-    cb, iters = host.vq_train(d["gls"], 256, key=seed, training_size=VQ_TRAINING)
-    vq_host_s = time.perf_counter() - t1
-    codes = host.vq_codes(d["gls"], cb)
+    # every GL trains the codebook, as GLVQ does (glVQ.cpp:53-73); the
+    # nearest-codeword search of each Lloyd iteration runs on the device
+    cb, iters = host.vq_train(d["gls"], 256, key=seed, training_size=0, device=True)
+    vq_train_s = time.perf_counter() - t1
+    codes = host.vq_codes(d["gls"], cb, device=True)
     packed = host.pack_codes(codes, 8)
     log(f"workload N={N} M={M} C={C}: generated in {time.time() - t0:.1f}s (VQ {iters} iterations)")
     return dict(panel=panel, packed=packed, tran=tran, codebook=cb, pc=pc, N=N, M=M,
                 H=panel.shape[0], het_aa=np.ascontiguousarray(d["gls"][:, :, 1:3]),
-                gls=d["gls"], codes=codes, vq_iters=iters, vq_host_s=vq_host_s, seed=seed,
+                gls=d["gls"], codes=codes, vq_iters=iters, vq_train_s=vq_train_s, seed=seed,
                 truth=d["truth"])
+
+
+# ---------------------------------------------------------------------------
+# accuracy of a timed generation against FP64 (numpy; no oracle involved)
+# ---------------------------------------------------------------------------
+_PHILOX_M = (np.uint64(0xD2511F53), np.uint64(0xCD9E8D57))
+_PHILOX_W = (np.uint32(0x9E3779B9), np.uint32(0xBB67AE85))
+
+
+def philox_blocks(k, sample, window, gen, seed):
+    """Philox4x32-10 of counters (k, sample, window, gen) under key seed
+    (Salmon et al., SC'11), the engine's draw layout; arrays broadcast."""
+    c = [np.asarray(x, np.uint32) for x in np.broadcast_arrays(k, sample, window, gen)]
+    k0, k1 = np.uint32(seed & 0xFFFFFFFF), np.uint32(seed >> 32)
+    m32 = np.uint64(0xFFFFFFFF)
+    for r in range(10):
+        if r:
+            k0 = np.uint32((int(k0) + int(_PHILOX_W[0])) & 0xFFFFFFFF)
+            k1 = np.uint32((int(k1) + int(_PHILOX_W[1])) & 0xFFFFFFFF)
+        p0 = _PHILOX_M[0] * c[0].astype(np.uint64)
+        p1 = _PHILOX_M[1] * c[2].astype(np.uint64)
+        hi0, lo0 = (p0 >> np.uint64(32)).astype(np.uint32), (p0 & m32).astype(np.uint32)
+        hi1, lo1 = (p1 >> np.uint64(32)).astype(np.uint32), (p1 & m32).astype(np.uint32)
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+    return c
+
+
+def unif_excl(x, nhaps, sample):
+    r = ((x.astype(np.uint64) * np.uint64(nhaps - 2)) >> np.uint64(32)).astype(np.int64)
+    return r + 2 * (r >= 2 * np.asarray(sample, np.int64))
+
+
+def emission_table(codebook, pc):
+    """FillEmit per code in float32 with the engine's operation order."""
+    g0, g1 = codebook[:, 0].astype(np.float32), codebook[:, 1].astype(np.float32)
+    g2 = np.maximum(np.float32(1) - g0 - g1, np.float32(0))
+    pc = np.asarray(pc, np.float32)
+    return np.stack([((pc[i, 0] * g0 + pc[i, 1] * g1) + pc[i, 2] * g1) + pc[i, 3] * g2
+                     for i in range(4)], 1).astype(np.float32)
+
+
+def fp64_forward(panel, codes, E, tran, samples, parents):
+    """Log-likelihoods of (sample, 4 parents) rows in FP64, normalised at
+    every site: the same model as the engine's passes."""
+    n, M = len(samples), codes.shape[1]
+    bits = np.unpackbits(np.ascontiguousarray(panel).view(np.uint8).reshape(panel.shape[0], -1),
+                         axis=1, bitorder="little")[:, :M]
+    # site-major copies, so that each site reads contiguous rows
+    h = [np.ascontiguousarray(bits[parents[:, q]].T).astype(np.int64) for q in range(4)]
+    cs = np.ascontiguousarray(codes[samples].T)
+    E64 = E.astype(np.float64).reshape(-1)
+    l = np.zeros((n, 4))
+    score = np.zeros(n)
+    t = tran.astype(np.float64)
+    for m in range(M):
+        f0, f1, m0, m1 = (x[m] for x in h)
+        base = 4 * cs[m].astype(np.int64)
+        e00, e01 = E64[base + 2 * f0 + m0], E64[base + 2 * f0 + m1]
+        e10, e11 = E64[base + 2 * f1 + m0], E64[base + 2 * f1 + m1]
+        if m == 0:
+            b = np.full((n, 4), 0.25)
+        else:
+            t0, t1, t2 = t[m]
+            b = np.stack([l[:, 0] * t0 + (l[:, 1] + l[:, 2]) * t1 + l[:, 3] * t2,
+                          l[:, 1] * t0 + (l[:, 0] + l[:, 3]) * t1 + l[:, 2] * t2,
+                          l[:, 2] * t0 + (l[:, 0] + l[:, 3]) * t1 + l[:, 1] * t2,
+                          l[:, 3] * t0 + (l[:, 1] + l[:, 2]) * t1 + l[:, 0] * t2], 1)
+        l = b * np.stack([e00, e01, e10, e11], 1)
+        s = l.sum(1)
+        score += np.log(s)
+        l /= s[:, None]
+    return score
+
+
+def accuracy_vs_fp64(w, C, pen, seed, lanes, gen=0, n_chains=160, tol=1e-5):
+    """Runs generation `gen` of the timed workload once more with traces and
+    replays n_chains chains on the host (their Philox draws, the device's
+    accept decisions): every proposal's log-likelihood against FP64, and the
+    accept decisions FP64 values would take, outside a band of relative
+    width tol around the accept threshold."""
+    import torch  # noqa: F401
+    from glphase_amd import PREC_MIXED
+    from glphase_amd.engine import Job
+    N, H = w["N"], w["H"]
+    job = Job([dict(panel=w["panel"], packed=w["packed"], tran=w["tran"])], N, H, w["codebook"],
+              w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED, trace=True)
+    job.make_plan(lanes)
+    job.run_batched(gen=gen, pen=pen, seed=seed)
+    tl, ta = job.trace()
+    job.close()
+    rng = np.random.default_rng(1)
+    S = np.sort(rng.choice(N, n_chains, replace=False))
+    init = philox_blocks(0xFFFFFFFF, S, 0, gen, seed)
+    P = np.stack([unif_excl(init[q], H, S) for q in range(4)], 1)
+    kk = np.arange(C)[None, :]
+    d = philox_blocks(kk, S[:, None], 0, gen, seed)
+    rp = (d[0] & 3).astype(np.int64)
+    cand = unif_excl(d[1], H, S[:, None])
+    u = d[2].astype(np.float64) * 2.0 ** -32
+    rows_s, rows_p = [S.copy()], [P.copy()]
+    cur = P.copy()
+    for c in range(C):
+        prop = cur.copy()
+        prop[np.arange(len(S)), rp[:, c]] = cand[:, c]
+        rows_s.append(S.copy())
+        rows_p.append(prop)
+        acc = ta[0, S, c].astype(bool)
+        cur = np.where(acc[:, None], prop, cur)
+    samples = np.concatenate(rows_s)
+    parents = np.concatenate(rows_p)
+    E = emission_table(w["codebook"], w["pc"])
+    f64 = fp64_forward(w["panel"], w["codes"], E, w["tran"], samples, parents)
+    f64 = f64.reshape(C + 1, len(S)).T  # [chain][init + C]
+    dev = tl[0, S]
+    rel = np.abs(dev - f64) / np.abs(f64)
+    # accept decisions along the device's path, decided with FP64 values
+    cur64 = f64[:, 0].copy()
+    agree = band = total = 0
+    for c in range(C):
+        p64 = f64[:, c + 1]
+        with np.errstate(over="ignore"):
+            d64 = (p64 >= cur64) | (u[:, c] < np.exp((p64 - cur64) * pen))
+        # accept <=> p > cur + log(u) / pen: decisions whose FP64 value lies
+        # within the two values' tolerance of that threshold are in the band
+        thr = cur64 + np.log(np.maximum(u[:, c], 1e-300)) / pen if pen > 0 else cur64
+        inside = np.abs(p64 - thr) <= 2 * tol * np.abs(cur64)
+        gpu = ta[0, S, c].astype(bool)
+        band += int(inside.sum())
+        total += int((~inside).sum())
+        agree += int(((d64 == gpu) & ~inside).sum())
+        cur64 = np.where(gpu, p64, cur64)
+    return {"chains": int(len(S)), "proposals": int(len(S) * C), "generation": gen,
+            "max_rel_err": float(rel.max()), "p99_rel_err": float(np.percentile(rel, 99)),
+            "median_rel_err": float(np.median(rel)), "tolerance": tol,
+            "accept_agreement": agree / max(total, 1), "decisions_outside_band": total,
+            "decisions_inside_band": band}
 
 
 def time_steps(torch, fn, steps, warmup, stream_events=True):
@@ -111,7 +252,44 @@ def cpu_baseline(w, C, pen, seed, budget_s=12.0, precision=1):
     }
 
 
-def aux_knn(torch, steps, seed=11):
+def oracle_rates(windows, cb, pc, N, C, pen, seed, precision, prop_mode=0, klist=None,
+                 budget_s=3.0):
+    """CPU baseline of one workload: the oracle (C restatement of the
+    reference's Impute::hmm_like + Insti::solve, -O3) on this host.  One core
+    over chains of the first window in order (the reference's serial loop),
+    then every core over whole generations of every window (OpenMP over
+    samples), each for about budget_s / 2 seconds."""
+    from oracle import bind as orc
+    lut = orc.emit_lut(cb, pc)
+    kw = dict(precision=precision, prop_mode=prop_mode)
+    if klist is not None:
+        kw["klist"] = klist
+    mhs = [orc.MH(w["panel"], w["packed"], 8, lut, w["tran"], N) for w in windows]
+    se_gen = sum(N * (C + 1) * int(w["tran"].shape[0]) for w in windows)
+    se0 = (C + 1) * int(windows[0]["tran"].shape[0])
+    n1, t1 = 0, 0.0
+    while n1 < N and t1 < budget_s / 2:
+        b = min(N, n1 + 8)
+        t0 = time.perf_counter()
+        mhs[0].run(C, pen, seed, s0=n1, s1=b, **kw)
+        t1 += time.perf_counter() - t0
+        n1 = b
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = min(cores, os.cpu_count() or 1)
+    gens, ta = 0, 0.0
+    while gens == 0 or ta < budget_s / 2:
+        t0 = time.perf_counter()
+        for k, mh in enumerate(mhs):
+            mh.run(C, pen, seed, gen=gens, window_id=k, nthreads=cores, **kw)
+        ta += time.perf_counter() - t0
+        gens += 1
+    return {"cpu_1core": se0 * n1 / t1, "cpu_allcore": se_gen * gens / ta, "cores": cores,
+            "kind": "port",
+            "sample": f"1 core: {n1} chains ({t1:.1f}s); {cores} cores: {gens} generation(s) "
+                      f"of {N} samples x {len(windows)} window(s) ({ta:.1f}s)"}
+
+
+def aux_knn(torch, steps, seed=11, cpu=True):
     """kNN candidate lists (KNN::ClusterHaps) of the reference-panel workload:
     10,000 samples plus 20,000 panel haplotypes x 2,048 sites, K = 200, panel
     resident on the device.  Cost unit: one 64-bit word compared between a
@@ -139,7 +317,7 @@ def aux_knn(torch, steps, seed=11):
     same = bool((out[:ns].cpu().numpy().view(np.uint32) == want).all())
     log(f"kNN: {ms:.3f} ms/build, {words / (ms / 1e3):.3e} word-compares/s; "
         f"oracle {ns} samples in {cpu_s:.2f}s; first {ns} rows identical: {same}")
-    mh = aux_reference_panel_mh(torch, d, panel, out, steps, seed)
+    mh = aux_reference_panel_mh(torch, d, panel, out, steps, seed, cpu=cpu)
     return {"ms_per_build": ms, "unit": "word-compares/s", "value": words / (ms / 1e3),
             "mh_klist": mh,
             "config": {"samples": N, "panel_haplotypes": R, "sites": M, "K": K},
@@ -149,7 +327,7 @@ def aux_knn(torch, steps, seed=11):
                              "sample": f"oracle kNN lists of {ns} samples ({cpu_s:.2f}s)"}}
 
 
-def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64):
+def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64, cpu=True):
     """The MH update of the reference-panel workload: 10,000 samples whose
     candidates are their K = 200 nearest haplotypes of panel + samples
     (lists from the kNN build), 64 proposals each, mixed precision,
@@ -174,16 +352,29 @@ def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64):
     secs, _ = time_steps(torch, step, steps, 1)
     ms = secs / steps * 1e3
     se = N * (C + 1) * M
+
+    def pstep():
+        job.run_plain(gen=gen[0], pen=1.0, seed=seed)
+        gen[0] += 1
+
+    psecs, _ = time_steps(torch, pstep, 1, 0)
+    pms = psecs * 1e3
     log(f"reference-panel MH (K-lists): {ms:.3f} ms/step, {se / (ms / 1e3):.3e} SE/s, "
-        f"lanes/chain {lanes}")
+        f"lanes/chain {lanes}; plain {pms:.3f} ms/step")
     job.close()
-    return {"ms_per_step": ms, "value": se / (ms / 1e3), "unit": "site-evals/s",
-            "lanes_per_chain": lanes,
-            "config": {"samples": N, "panel_haplotypes": panel.shape[0] - 2 * N, "sites": M,
-                       "proposals": C, "K": int(klist.shape[1])}}
+    out = {"ms_per_step": ms, "value": se / (ms / 1e3), "unit": "site-evals/s",
+           "lanes_per_chain": lanes,
+           "plain": {"ms_per_step": pms, "value": se / (pms / 1e3)},
+           "config": {"samples": N, "panel_haplotypes": panel.shape[0] - 2 * N, "sites": M,
+                      "proposals": C, "K": int(klist.shape[1]), "precision": "mixed"}}
+    if cpu:
+        out["cpu"] = oracle_rates([dict(panel=panel, packed=packed, tran=tran)], cb, pc, N, C,
+                                  1.0, seed, PREC_MIXED, prop_mode=PROP_KLIST,
+                                  klist=klist.cpu().numpy().view(np.uint32))
+    return out
 
 
-def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32):
+def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32, cpu=True):
     """Uneven windows: 2,500 samples over 32 windows whose site counts are
     lognormal (median 1,024, clipped to 256..4,096), 32 proposals per sample,
     FP32 forward and FP32 log accumulation (the reference's arithmetic); one
@@ -231,6 +422,8 @@ def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32):
         log(f"batching {name}: {ms:.3f} ms/step, {se / (ms / 1e3):.3e} SE/s")
     out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
     job.close()
+    if cpu:
+        out["cpu"] = oracle_rates(wins, cb, pc, N, C, 1.0, seed, PREC_F32)
     return out
 
 
@@ -262,31 +455,43 @@ def aux_whole_run(torch, w, C, seed=7, burnin=56, sampling=200):
     return out
 
 
-def aux_plain_small(torch, steps, seed=31, N=1000, M=1024, C=16):
-    """The plain path at 1,000 samples x 1,024 sites: one launch per MH
-    iteration, one proposal per sample per launch (C launches after the
-    initial one), FP32 arithmetic.  Per-launch time and site-evals/s."""
+def aux_small(torch, steps, N, M, C, seed, cpu=True):
+    """A one-window workload in the reference's FP32 arithmetic: the plain
+    path (one launch per MH iteration, one proposal per sample per launch)
+    and the batched kernel (autotuned depth) on the same inputs, and the CPU
+    oracle.  Per-launch time of the plain path and site-evals/s of both."""
     from glphase_amd import PREC_F32
-    p = make_workload(N, M, C, 4.0, seed)
     from glphase_amd.engine import Job
-    job = Job([dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])], N, p["H"],
-              p["codebook"], p["pc"], bits=8, num_cycles=C, precision=PREC_F32)
-    job.make_plan(1)
-    gen = [0]
-
-    def step():
-        job.run_plain(gen=gen[0], pen=1.0, seed=seed)
-        gen[0] += 1
-
-    secs, _ = time_steps(torch, step, steps, 1)
-    ms = secs / steps * 1e3
+    p = make_workload(N, M, C, 4.0, seed)
+    win = dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])
+    job = Job([win], N, p["H"], p["codebook"], p["pc"], bits=8, num_cycles=C, precision=PREC_F32)
+    job.make_plan(0)
+    lanes = int(job.autotune(gen=0, pen=1.0, seed=seed)[0])
     se = N * (C + 1) * M
-    log(f"plain 1000 x 1024: {ms / (C + 1):.4f} ms/launch, {se / (ms / 1e3):.3e} SE/s")
-    job.close()
-    return {"ms_per_launch": ms / (C + 1), "launches_per_step": C + 1, "value": se / (ms / 1e3),
-            "unit": "site-evals/s",
-            "config": {"samples": N, "sites": M, "proposals_per_launch": 1, "precision": "fp32"}}
+    res = {}
+    for name, fn in (("plain", job.run_plain), ("batched", job.run_batched)):
+        gen = [0]
 
+        def step(fn=fn):
+            fn(gen=gen[0], pen=1.0, seed=seed)
+            gen[0] += 1
+
+        secs, _ = time_steps(torch, step, steps, 1)
+        res[name] = secs / steps * 1e3
+    job.close()
+    ms = res["plain"]
+    log(f"{N} x {M} x {C}: plain {ms / (C + 1):.4f} ms/launch, {se / (ms / 1e3):.3e} SE/s; "
+        f"batched {res['batched']:.3f} ms/step ({lanes} lanes/chain)")
+    out = {"ms_per_launch": ms / (C + 1), "launches_per_step": C + 1, "value": se / (ms / 1e3),
+           "unit": "site-evals/s",
+           "plain_roofline_frac": FLOP_PER_SE * se / (ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS,
+           "batched": {"ms_per_step": res["batched"], "value": se / (res["batched"] / 1e3),
+                       "lanes_per_chain": lanes},
+           "config": {"samples": N, "sites": M, "proposals": C, "proposals_per_launch": 1,
+                      "precision": "fp32"}}
+    if cpu:
+        out["cpu"] = oracle_rates([win], p["codebook"], p["pc"], N, C, 1.0, seed, PREC_F32)
+    return out
 
 def aux_vq(torch, w, steps):
     """GL codebook training (GLVQ) and coding of the MH workload's GLs with
@@ -295,10 +500,15 @@ def aux_vq(torch, w, steps):
     from glphase_amd import host
     from glphase_amd._lib import c_f32p, check, lib, ptr
     gls, cb = w["gls"], w["codebook"]
+    # host and device-assisted training give the same codebook (checked on a
+    # subsample of VQ_TRAINING GLs: the host search over every GL is slow)
     t0 = time.perf_counter()
     cbd, itd = host.vq_train(gls, 256, key=w["seed"], training_size=VQ_TRAINING, device=True)
     dev_s = time.perf_counter() - t0
-    same_cb = bool(cbd.tobytes() == cb.tobytes() and itd == w["vq_iters"])
+    t0 = time.perf_counter()
+    cbh, ith = host.vq_train(gls, 256, key=w["seed"], training_size=VQ_TRAINING)
+    host_s = time.perf_counter() - t0
+    same_cb = bool(cbd.tobytes() == cbh.tobytes() and itd == ith)
     flat = np.ascontiguousarray(gls.reshape(-1, 3))
     n = flat.shape[0]
     gd = torch.from_numpy(flat).to("cuda:0")
@@ -311,18 +521,21 @@ def aux_vq(torch, w, steps):
 
     secs, _ = time_steps(torch, code, steps, 1)
     ms = secs / steps * 1e3
-    same_codes = bool((out.cpu().numpy() == w["codes"].reshape(-1)).all())
     t0 = time.perf_counter()
-    host.vq_codes(gls, cb)
+    host_codes = host.vq_codes(gls, cb)
     host_codes_s = time.perf_counter() - t0
-    log(f"VQ: training on {VQ_TRAINING} GLs {w['vq_host_s']:.2f}s host / {dev_s:.2f}s device "
+    same_codes = bool((out.cpu().numpy() == host_codes.reshape(-1)).all())
+    log(f"VQ: training on all {n} GLs {w['vq_train_s']:.2f}s ({w['vq_iters']} iterations, "
+        f"device search); on {VQ_TRAINING} GLs {host_s:.2f}s host / {dev_s:.2f}s device "
         f"(identical: {same_cb}); codes of {n} GLs {ms:.3f} ms device / "
         f"{host_codes_s * 1e3:.1f} ms host (identical: {same_codes})")
     return {"codes_ms": ms, "unit": "distances/s", "value": n * cb.shape[0] / (ms / 1e3),
             "gls": n, "codewords": int(cb.shape[0]), "codes_identical": same_codes,
             "host_codes_ms": host_codes_s * 1e3,
-            "train": {"training_gls": VQ_TRAINING, "iterations": itd,
-                      "device_s": dev_s, "host_s": w["vq_host_s"], "identical": same_cb}}
+            "train": {"training_gls": n, "iterations": w["vq_iters"], "device_s": w["vq_train_s"],
+                      "subsample_check": {"training_gls": VQ_TRAINING, "iterations": itd,
+                                          "device_s": dev_s, "host_s": host_s,
+                                          "identical": same_cb}}}
 
 
 def recorded_traffic(workload):
@@ -499,11 +712,13 @@ def main():
         "data": "synthetic founder-mosaic haplotypes and GLs",
         "config": {"workload": f"{N} samples x {M} sites, {C} proposals/sample, 8-bit VQ GLs, "
                                "batched persistent kernel (mixed precision)",
-                   "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes},
+                   "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes,
+                   "vq_training_gls": N * M, "vq_iterations": w["vq_iters"]},
         "mh_iterations_per_s": 1e3 / ms,
         "proposals_per_s": N * C / (ms / 1e3),
         "plain": {"value": pvalue, "ms_per_step": pms, "launches_per_step": C + 1,
-                  "batched_over_plain": value / pvalue, "graph_ms_per_step": pgms},
+                  "batched_over_plain": value / pvalue, "graph_ms_per_step": pgms,
+                  "roofline_frac": FLOP_PER_SE * se_step / (pms / 1e3) / 1e12 / PEAK_FP32_TFLOPS},
         "graph": {"batched_ms_per_step": gms, "plain_ms_per_step": pgms},
         "hmm_work": {"ms_per_step": wms, "generation_ms": ms + wms, "full_generation_ms": fms,
                      "note": "Impute::hmm_work for all samples after the MH update"},
@@ -513,15 +728,55 @@ def main():
                      "flop_per_site_eval": FLOP_PER_SE,
                      "hbm_gbps": None if traffic is None else traffic / (ms / 1e3) / 1e9},
     }
+    out["accuracy"] = accuracy_vs_fp64(w, C, pen, seed, lanes)
+    a = out["accuracy"]
+    log(f"accuracy vs FP64 ({a['chains']} chains, {a['proposals']} proposals of generation "
+        f"{a['generation']}): max {a['max_rel_err']:.2e}, p99 {a['p99_rel_err']:.2e}; accept "
+        f"agreement {a['accept_agreement']:.6f} outside the band ({a['decisions_inside_band']} inside)")
+    cpu = not args.no_cpu
     if not args.no_aux:
         out["vq"] = aux_vq(torch, w, args.steps)
-        out["knn"] = aux_knn(torch, args.steps)
-        out["batching"] = aux_batching(torch, args.steps)
-        out["plain_small"] = aux_plain_small(torch, args.steps)
+        out["knn"] = aux_knn(torch, args.steps, cpu=cpu)
+        out["batching"] = aux_batching(torch, args.steps, cpu=cpu)
+        out["plain_small"] = aux_small(torch, args.steps, 1000, 1024, 16, 31, cpu=cpu)
+        out["small_run"] = aux_small(torch, args.steps, 100, 1024, 200, 41, cpu=cpu)
         out["whole_run"] = aux_whole_run(torch, w, C)
-    if not args.no_cpu:
+    if cpu:
         out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
+    if not args.no_aux:
+        out["configs"] = config_table(out)
     print(json.dumps(out), flush=True)
+
+
+def config_table(out):
+    """The five benchmark workloads side by side: CPU oracle on one core and
+    on every core, the GPU plain path and the batched form (site-evals/s),
+    and the batched form's (config 2: the plain path's) fraction of the FP32
+    VALU peak at 29 FLOP per site-eval."""
+    def frac(v):
+        return None if v is None else FLOP_PER_SE * v / 1e12 / PEAK_FP32_TFLOPS
+
+    def row(cpu, plain, batched, primary):
+        return {"cpu_1core": cpu and cpu["cpu_1core"], "cpu_allcore": cpu and cpu["cpu_allcore"],
+                "gpu_plain": plain, "gpu_batched": batched,
+                "roofline_frac": frac(batched if primary == "batched" else plain)}
+
+    c1, c2, c3 = out["small_run"], out["plain_small"], out["batching"]
+    c5 = out["knn"]["mh_klist"]
+    cb = out.get("cpu_baseline")
+    return {
+        "1": dict(row(c1.get("cpu"), c1["value"], c1["batched"]["value"], "batched"),
+                  workload="100 x 1024, C = 200, fp32"),
+        "2": dict(row(c2.get("cpu"), c2["value"], c2["batched"]["value"], "plain"),
+                  workload="1000 x 1024, C = 16, fp32, plain path"),
+        "3": dict(row(c3.get("cpu"), c3["plain"]["value"], c3["batched"]["value"], "batched"),
+                  workload="2500 x 32 windows (lognormal sites), C = 32, fp32"),
+        "4": dict(row(cb and {"cpu_1core": cb["single_core"]["value"], "cpu_allcore": cb["value"]},
+                      out["plain"]["value"], out["value"], "batched"),
+                  workload="5000 x 4096, C = 64, mixed precision"),
+        "5": dict(row(c5.get("cpu"), c5["plain"]["value"], c5["value"], "batched"),
+                  workload="10000 + 20000 panel x 2048, K = 200, C = 64, mixed precision"),
+    }
 
 
 if __name__ == "__main__":
