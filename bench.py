@@ -525,6 +525,22 @@ def aux_vq(torch, w, steps):
     host_codes = host.vq_codes(gls, cb)
     host_codes_s = time.perf_counter() - t0
     same_codes = bool((out.cpu().numpy() == host_codes.reshape(-1)).all())
+    # GLPack::GetPackedGLs of those codes on the device ([M/4][N] words)
+    N, M = gls.shape[0], gls.shape[1]
+    packed = torch.empty(((M + 3) // 4, N), dtype=torch.int32, device="cuda:0")
+
+    def pack():
+        check(lib().glp_pack_codes_device(out.data_ptr(), N, M, 8, 0, N, packed.data_ptr(),
+                                          stream), "glp_pack_codes_device")
+
+    psecs, _ = time_steps(torch, pack, steps, 1)
+    pack_ms = psecs / steps * 1e3
+    t0 = time.perf_counter()
+    host_packed = host.pack_codes(host_codes, 8)
+    host_pack_s = time.perf_counter() - t0
+    same_packed = bool((packed.cpu().numpy().view(np.uint32) == host_packed).all())
+    log(f"GL packing: {pack_ms:.3f} ms device / {host_pack_s * 1e3:.1f} ms host "
+        f"(identical: {same_packed}), {2 * n / (pack_ms / 1e3) / 1e9:.0f} GB/s")
     log(f"VQ: training on all {n} GLs {w['vq_train_s']:.2f}s ({w['vq_iters']} iterations, "
         f"device search); on {VQ_TRAINING} GLs {host_s:.2f}s host / {dev_s:.2f}s device "
         f"(identical: {same_cb}); codes of {n} GLs {ms:.3f} ms device / "
@@ -532,6 +548,8 @@ def aux_vq(torch, w, steps):
     return {"codes_ms": ms, "unit": "distances/s", "value": n * cb.shape[0] / (ms / 1e3),
             "gls": n, "codewords": int(cb.shape[0]), "codes_identical": same_codes,
             "host_codes_ms": host_codes_s * 1e3,
+            "pack": {"ms": pack_ms, "host_ms": host_pack_s * 1e3, "identical": same_packed,
+                     "bytes_per_s": 2 * n / (pack_ms / 1e3)},
             "train": {"training_gls": n, "iterations": w["vq_iters"], "device_s": w["vq_train_s"],
                       "subsample_check": {"training_gls": VQ_TRAINING, "iterations": itd,
                                           "device_s": dev_s, "host_s": host_s,

@@ -182,6 +182,9 @@ SIGNATURES = [
       ctypes.c_void_p]),
     ("glp_pack_codes", ctypes.c_int,
      [c_u8p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_u32p]),
+    ("glp_pack_codes_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+      ctypes.c_void_p, ctypes.c_void_p]),
     ("glp_pack_nonvq", ctypes.c_int,
      [c_f32p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_u32p]),
 ]

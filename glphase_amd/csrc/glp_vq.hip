@@ -115,6 +115,75 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// GLPack::GetPackedGLs / GLIdxs2uint32_t (glPack.cpp:32-192) on the device:
+// the codes [n_samples][num_sites] (one byte each) of samples [s0, s0 + ns)
+// become words [ceil(num_sites / per)][ns], per = 32 / bits codes a word,
+// code j of a word at bit 32 - bits * (j + 1) (MSB first); a site past the
+// last one takes the code of GL 0 (glPack.cpp:66-70).  A workgroup packs a
+// tile of 64 samples x 256 sites: it reads the tile along the sample rows
+// (a wavefront covers 64 consecutive bytes of one row) into LDS, then each
+// lane packs one sample's words and a wavefront's stores cover 64
+// consecutive samples of one word row.
+constexpr unsigned kPackSites = 256, kPackSamples = 64;
+
+// VEC: rows start on 16-byte boundaries (codes 16-byte aligned, num_sites a
+// multiple of 16): the tile is read in 16-byte pieces, 16 lanes a row.
+template <bool VEC>
+__global__ void __launch_bounds__(256)
+    k_pack_codes(const uint8_t *__restrict__ codes, unsigned num_sites, unsigned bits, unsigned s0,
+                 unsigned ns, uint32_t *__restrict__ out) {
+  // rows of 65 dwords: the pack phase's lanes (one row each) read distinct banks
+  __shared__ __align__(16) uint32_t tile[kPackSamples][kPackSites / 4 + 1];
+  uint8_t *tb = reinterpret_cast<uint8_t *>(&tile[0][0]);
+  constexpr unsigned kRowBytes = 4 * (kPackSites / 4 + 1);
+  const unsigned site0 = blockIdx.x * kPackSites, samp0 = blockIdx.y * kPackSamples;
+  const unsigned t = threadIdx.x;
+  const uint8_t pad = codes[0];
+  if (VEC) {
+    // piece q of the tile: row q / 16, bytes 16 * (q % 16) ..; the row's
+    // pieces past num_sites (all or nothing: num_sites % 16 == 0) take pad
+    const uint32_t pad4 = 0x01010101u * pad;
+    for (unsigned q = t; q < kPackSamples * (kPackSites / 16); q += 256) {
+      const unsigned k = q >> 4, site = site0 + 16 * (q & 15u), samp = samp0 + k;
+      uint4 v = make_uint4(pad4, pad4, pad4, pad4);
+      if (samp < ns && site < num_sites)
+        v = *reinterpret_cast<const uint4 *>(codes + (size_t)(s0 + samp) * num_sites + site);
+      uint32_t *d = &tile[k][4 * (q & 15u)];
+      d[0] = v.x;
+      d[1] = v.y;
+      d[2] = v.z;
+      d[3] = v.w;
+    }
+  } else {
+    const unsigned site = site0 + t;
+    for (unsigned k = 0; k < kPackSamples; ++k) {
+      const unsigned samp = samp0 + k;
+      uint8_t c = pad;
+      if (samp < ns && site < num_sites) c = codes[(size_t)(s0 + samp) * num_sites + site];
+      tb[k * kRowBytes + t] = c;
+    }
+  }
+  __syncthreads();
+  const unsigned per = 32u / bits, words = kPackSites / per;
+  const unsigned nwords = (num_sites + per - 1u) / per;
+  const unsigned lane = t & 63u, samp = samp0 + lane;
+  if (samp >= ns) return;
+  for (unsigned w = t >> 6; w < words; w += 4u) {
+    const unsigned gw = site0 / per + w;
+    if (gw >= nwords) break;
+    // the word's per codes are per / 4 dwords of the row (byte j of the
+    // run is code j), placed MSB first
+    uint32_t word = 0;
+    for (unsigned q = 0; q < per / 4u; ++q) {
+      const uint32_t d = tile[lane][w * (per / 4u) + q];
+#pragma unroll
+      for (unsigned b = 0; b < 4u; ++b)
+        word |= ((d >> (8u * b)) & 0xFFu) << (32u - bits * (4u * q + b + 1u));
+    }
+    out[(size_t)gw * ns + samp] = word;
+  }
+}
+
 int launch_vq_assign(const float *pts_dev, size_t n, bool raw, const float *cb_dev,
                      unsigned ncode, uint8_t *out_dev, hipStream_t s) {
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
@@ -230,4 +299,23 @@ extern "C" int glp_vq_codes_device(const float *gls, size_t n_gl, const float *c
   } catch (const HipFailure &) {
     return GLP_ERR_HIP;  // the message is already set
   }
+}
+
+extern "C" int glp_pack_codes_device(const uint8_t *codes_dev, unsigned n_samples,
+                                     unsigned num_sites, unsigned bits, unsigned s0, unsigned s1,
+                                     uint32_t *out_dev, void *stream) {
+  if (!codes_dev || !out_dev || num_sites == 0 || s0 >= s1 || s1 > n_samples)
+    return set_error(GLP_ERR_ARG, "bad argument");
+  if (bits != 1 && bits != 2 && bits != 4 && bits != 8)
+    return set_error(GLP_ERR_ARG, "bits must divide 8");
+  const unsigned ns = s1 - s0;
+  const dim3 grid((num_sites + kPackSites - 1) / kPackSites, (ns + kPackSamples - 1) / kPackSamples);
+  const bool vec = (reinterpret_cast<uintptr_t>(codes_dev) & 15u) == 0 && num_sites % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_pack_codes<true>, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
+                       codes_dev, num_sites, bits, s0, ns, out_dev);
+  else
+    hipLaunchKernelGGL(k_pack_codes<false>, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
+                       codes_dev, num_sites, bits, s0, ns, out_dev);
+  return hip_check(hipGetLastError(), "k_pack_codes launch");
 }

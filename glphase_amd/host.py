@@ -45,12 +45,22 @@ def vq_codes(gls, codebook, device=False):
     return out.reshape(shape)
 
 
-def pack_codes(codes, bits=8, s0=0, s1=None):
-    """codes [N][M] uint8 -> packed words [ceil(M*bits/32)][s1-s0] uint32."""
+def pack_codes(codes, bits=8, s0=0, s1=None, device=False):
+    """codes [N][M] uint8 -> packed words [ceil(M*bits/32)][s1-s0] uint32.
+    device=True packs on the current GPU (glp_pack_codes_device); the words
+    are the same."""
     c = np.ascontiguousarray(codes, dtype=np.uint8)
     N, M = c.shape
     s1 = N if s1 is None else s1
     per = 32 // bits
+    if device:
+        import torch
+        cd = torch.from_numpy(c).to("cuda")
+        od = torch.empty(((M + per - 1) // per, s1 - s0), dtype=torch.int32, device="cuda")
+        check(lib().glp_pack_codes_device(cd.data_ptr(), N, M, bits, s0, s1, od.data_ptr(),
+                                          torch.cuda.current_stream().cuda_stream),
+              "glp_pack_codes_device")
+        return od.cpu().numpy().view(np.uint32)
     out = np.zeros(((M + per - 1) // per, s1 - s0), np.uint32)
     check(lib().glp_pack_codes(ptr(c, c_u8p), N, M, bits, s0, s1, ptr(out, c_u32p)), "glp_pack_codes")
     return out

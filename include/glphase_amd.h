@@ -82,7 +82,10 @@ int glp_set_tran(glp_ctx *ctx, const float *tran, size_t n);
 int glp_set_mutation_mat(glp_ctx *ctx, const float *pc16);
 
 /* HMMLikeCUDA::CopyCodeBookToDevice (hmmLike.h:29-30, hmmLike.cu:369-383).
- * pairs = n_codes (RR, Het) float pairs; n_codes = 2^bits, bits in 1,2,4,8. */
+ * pairs = n_codes (RR, Het) float pairs; n_codes = 2^bits with bits 4 (the
+ * reference GPU path's BITSPERCODE, globals.h:23) or 8.  The packing helpers
+ * below also take 1- and 2-bit codes (glPack.cpp:26-27); the MH kernels do
+ * not. */
 int glp_set_codebook(glp_ctx *ctx, const float *pairs, unsigned n_codes);
 
 /* HMMLikeCUDA::CopyPackedGLsToDevice (hmmLike.h:28, hmmLike.cu:357-366). */
@@ -429,6 +432,13 @@ int glp_vq_assign_device(const float *points_dev, size_t n, int raw, const float
 int glp_pack_codes(const uint8_t *codes, unsigned n_samples,
                    unsigned num_sites, unsigned bits, unsigned s0, unsigned s1,
                    uint32_t *out);
+
+/* glp_pack_codes on device buffers (codes_dev [n_samples][num_sites] bytes,
+ * out_dev [ceil(num_sites*bits/32)][s1-s0] words), bits 1, 2, 4 or 8, the
+ * same words; enqueued on `stream`. */
+int glp_pack_codes_device(const uint8_t *codes_dev, unsigned n_samples,
+                          unsigned num_sites, unsigned bits, unsigned s0, unsigned s1,
+                          uint32_t *out_dev, void *stream);
 
 /* The same without VQ: two 4-bit halves floor((GL - eps) * 16) per trio
  * (glPack.cpp:221-243). */

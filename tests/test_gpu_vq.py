@@ -151,6 +151,22 @@ def test_vq_assign_device_near_ties_in_groups(orc):
     assert ties > 0
 
 
+@pytest.mark.parametrize("bits", [1, 2, 4, 8])
+@pytest.mark.parametrize("N,M,s0,s1", [(1, 1, 0, 1), (70, 300, 0, 70), (130, 1000, 7, 100),
+                                       (5000, 4096, 0, 5000)])
+def test_pack_codes_device_matches_oracle(orc, bits, N, M, s0, s1):
+    # GLPack::GetPackedGLs on the device, word for word: MSB-first codes,
+    # site counts that leave a partial last word (padded with the code of
+    # GL 0, glPack.cpp:66-70), sample ranges, 1/2/4/8-bit codes
+    say(f"pack codes on the device bits={bits} N={N} M={M} [{s0},{s1})")
+    rng = np.random.default_rng(N + M + bits)
+    codes = rng.integers(0, 1 << bits, (N, M)).astype(np.uint8)
+    got = host.pack_codes(codes, bits, s0, s1, device=True)
+    want = orc.pack_codes(codes, bits, s0, s1)
+    assert got.shape == want.shape and (got == want).all()
+    assert (got == host.pack_codes(codes, bits, s0, s1)).all()
+
+
 def test_vq_full_benchmark_gls(orc):
     # the GL set of the MH update benchmark: 5,000 samples x 4,096 sites
     say("vq 5000 x 4096 GLs")
