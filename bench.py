@@ -406,7 +406,8 @@ def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32, cpu=True):
     out = {"config": {"samples": N, "windows": W, "sites_total": int(sizes.sum()),
                       "sites_min": int(sizes.min()), "sites_median": int(np.median(sizes)),
                       "sites_max": int(sizes.max()), "proposals": C, "precision": "fp32"},
-           "lanes_per_window": tuned, "unit": "site-evals/s"}
+           "lanes_per_window": tuned, "waves_per_workgroup": job.waves(),
+           "unit": "site-evals/s"}
     for name, fn in (("batched", job.run_batched), ("batched_graph", job.run_batched_graph),
                      ("plain", job.run_plain)):
         gen = [0]
@@ -629,7 +630,8 @@ def main():
         t0 = time.time()
         tuned = job.autotune(gen=0, pen=pen, seed=seed)
         lanes, grid = job.lanes()
-        log(f"autotune: lanes/chain {list(tuned)} in {time.time() - t0:.2f}s")
+        log(f"autotune: lanes/chain {list(tuned)}, {job.waves()} wavefronts per workgroup "
+            f"in {time.time() - t0:.2f}s")
     se_step = N * (C + 1) * M
     gen = [0]
 
@@ -731,6 +733,7 @@ def main():
         "config": {"workload": f"{N} samples x {M} sites, {C} proposals/sample, 8-bit VQ GLs, "
                                "batched persistent kernel (mixed precision)",
                    "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes,
+                   "waves_per_workgroup": job.waves(),
                    "vq_training_gls": N * M, "vq_iterations": w["vq_iters"]},
         "mh_iterations_per_s": 1e3 / ms,
         "proposals_per_s": N * C / (ms / 1e3),

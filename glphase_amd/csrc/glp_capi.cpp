@@ -99,7 +99,9 @@ using namespace glp;
 // batched plan
 // ---------------------------------------------------------------------------
 struct glp_plan {
-  int dev = 0, num_cu = 1, bpc = 1;
+  int dev = 0, num_cu = 1, bpc = 1;  // bpc: resident workgroups per CU at wpb
+  unsigned wpb = 4;                  // wavefronts per workgroup: 4 or 8
+  int bpc_of[2] = {0, 0};            // resident workgroups per CU at 4 and 8 (0: does not fit)
   KArgs a{};
   unsigned nwin = 0, grid = 1;
   std::vector<glp_window> wins_h;
@@ -146,8 +148,20 @@ static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu) {
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
   double waves = 0;
   for (unsigned w : wins) waves += waves_for(p->a.N, p->q[w]);
-  const unsigned blocks = (unsigned)std::ceil(waves / 4.0);
+  const unsigned blocks = (unsigned)std::ceil(waves / (double)p->wpb);
   return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
+}
+
+// Switches the plan to workgroups of `wpb` wavefronts (4 or 8); the
+// occupancy at that size was measured at creation.
+static int plan_use_waves(glp_plan *p, unsigned wpb) {
+  if (wpb != 4 && wpb != 8) return set_error(GLP_ERR_ARG, "waves per workgroup must be 4 or 8");
+  const int bpc = p->bpc_of[wpb == 8 ? 1 : 0];
+  if (bpc < 1) return set_error(GLP_ERR_ARG, "workgroups of this many wavefronts do not fit a CU (LDS)");
+  if (wpb != p->wpb) plan_graph_reset(p);  // the graph's node holds the kernel and block size
+  p->wpb = wpb;
+  p->bpc = bpc;
+  return GLP_OK;
 }
 
 static int plan_upload_q(glp_plan *p) {
@@ -266,14 +280,20 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   if (rc == GLP_OK)
     rc = hip_check(hipDeviceGetAttribute(&p->num_cu, hipDeviceAttributeMultiprocessorCount, p->dev),
                    "device attribute");
-  if (rc == GLP_OK) rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, lds_bytes(a), &p->bpc);
+  // occupancy at both workgroup sizes; a size whose LDS exceeds the CU's is
+  // not queried and cannot be chosen
+  for (int k = 0; k < 2 && rc == GLP_OK; ++k) {
+    const unsigned w = k ? 8u : 4u;
+    if (lds_bytes(a, w) <= kLdsPerCu)
+      rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, w, lds_bytes(a, w), &p->bpc_of[k]);
+  }
   if (rc != GLP_OK) {
     delete p;
     return rc;
   }
   // a launch that cannot be resident is refused here rather than left to
   // fail (or to run with a grid sized from a bad occupancy figure)
-  if (lds_bytes(a) > kLdsPerCu || p->bpc < 1) {
+  if (plan_use_waves(p, 4) != GLP_OK) {
     delete p;
     return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
   }
@@ -319,7 +339,7 @@ int glp_mh_run_noalloc(const glp_mh_desc *desc, unsigned lanes_per_chain, void *
   GLP_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
   GLP_TRY(hip_check(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev),
                     "device attribute"));
-  if (lds_bytes(a) > kLdsPerCu)
+  if (lds_bytes(a, 4) > kLdsPerCu)
     return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
   const unsigned Q = lanes_per_chain ? lanes_per_chain
                                      : auto_lanes(a.N * desc->num_windows, a.C, num_cu);
@@ -335,7 +355,8 @@ int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stre
   a.seed = seed;
   return launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
                         p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
-                        p->ctrs.as<unsigned long long>(), p->grid, static_cast<hipStream_t>(stream));
+                        p->ctrs.as<unsigned long long>(), p->grid, p->wpb,
+                        static_cast<hipStream_t>(stream));
 }
 
 int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
@@ -405,7 +426,7 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
       GLP_TRY(hip_check(hipEventRecord(e0, s), "event record"));
       GLP_TRY(launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
                              p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
-                             p->ctrs.as<unsigned long long>(), grid, s));
+                             p->ctrs.as<unsigned long long>(), grid, p->wpb, s));
       GLP_TRY(hip_check(hipEventRecord(e1, s), "event record"));
       GLP_TRY(hip_check(hipEventSynchronize(e1), "event sync"));
       float t = 0;
@@ -419,16 +440,24 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
   static const unsigned kTrials[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 32};
   float best_ms = 1e30f;
   std::vector<unsigned> best_q = p->q;
-  for (unsigned Q : kTrials) {
-    if (rc != GLP_OK || Q > a.C + 1) break;
-    std::fill(p->q.begin(), p->q.end(), Q);
-    float ms = 0;
-    rc = trial(&ms);
-    if (rc == GLP_OK && ms < best_ms) {
-      best_ms = ms;
-      best_q = p->q;
+  unsigned best_w = p->wpb;
+  // one depth for every window, at each workgroup size that fits
+  for (unsigned w : {4u, 8u}) {
+    if (rc != GLP_OK || p->bpc_of[w == 8 ? 1 : 0] < 1) continue;
+    rc = plan_use_waves(p, w);
+    for (unsigned Q : kTrials) {
+      if (rc != GLP_OK || Q > a.C + 1) break;
+      std::fill(p->q.begin(), p->q.end(), Q);
+      float ms = 0;
+      rc = trial(&ms);
+      if (rc == GLP_OK && ms < best_ms) {
+        best_ms = ms;
+        best_q = p->q;
+        best_w = w;
+      }
     }
   }
+  if (rc == GLP_OK) rc = plan_use_waves(p, best_w);
   if (buckets.size() > 1) {
     for (auto &b : buckets) {
       for (unsigned Q : kTrials) {
@@ -515,11 +544,12 @@ struct PlainArgs {
   }
 };
 
-hipKernelNodeParams node_params(const void *fn, dim3 grid, unsigned shmem, void **args) {
+hipKernelNodeParams node_params(const void *fn, dim3 grid, unsigned shmem, void **args,
+                                unsigned threads = 256) {
   hipKernelNodeParams k{};
   k.func = const_cast<void *>(fn);
   k.gridDim = grid;
-  k.blockDim = dim3(256);
+  k.blockDim = dim3(threads);
   k.sharedMemBytes = shmem;
   k.kernelParams = args;
   k.extra = nullptr;
@@ -531,11 +561,11 @@ static int plan_graph_build(glp_plan *p, int kind) {
   plan_graph_reset(p);
   const KArgs &a = p->a;
   const bool mixed = a.precision != GLP_PREC_F32;
-  const unsigned shmem = (unsigned)lds_bytes(a);
+  const unsigned shmem = (unsigned)lds_bytes(a, kind == 1 ? p->wpb : 4u);
   GLP_TRY(hip_check(hipGraphCreate(&p->graph, 0), "hipGraphCreate"));
   int rc = GLP_OK;
   if (kind == 1) {
-    const void *fn = batched_kernel_fn(a.bits, mixed);
+    const void *fn = batched_kernel_fn(a.bits, mixed, p->wpb);
     rc = prepare_kernel_lds(fn, shmem);
     unsigned long long *ctrs = p->ctrs.as<unsigned long long>();
     unsigned *wctr = p->wctr.as<unsigned>();
@@ -547,7 +577,7 @@ static int plan_graph_build(glp_plan *p, int kind) {
     BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
                    p->nwin, wctr, ctrs, a, {}};
     ba.bind();
-    hipKernelNodeParams bk = node_params(fn, dim3(p->grid), shmem, ba.ptrs);
+    hipKernelNodeParams bk = node_params(fn, dim3(p->grid), shmem, ba.ptrs, 64 * p->wpb);
     if (rc == GLP_OK) rc = hip_check(hipGraphAddKernelNode(&n1, p->graph, &n0, 1, &bk), "graph node");
     if (rc == GLP_OK) p->nodes = {n0, n1};
   } else {
@@ -584,12 +614,13 @@ static int plan_graph_run(glp_plan *p, int kind, unsigned gen, float pen, uint64
   a.pen = pen;
   a.seed = seed;
   const bool mixed = a.precision != GLP_PREC_F32;
-  const unsigned shmem = (unsigned)lds_bytes(a);
+  const unsigned shmem = (unsigned)lds_bytes(a, kind == 1 ? p->wpb : 4u);
   if (kind == 1) {
     BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
                    p->nwin, p->wctr.as<unsigned>(), p->ctrs.as<unsigned long long>(), a, {}};
     ba.bind();
-    hipKernelNodeParams bk = node_params(batched_kernel_fn(a.bits, mixed), dim3(p->grid), shmem, ba.ptrs);
+    hipKernelNodeParams bk = node_params(batched_kernel_fn(a.bits, mixed, p->wpb), dim3(p->grid),
+                                         shmem, ba.ptrs, 64 * p->wpb);
     GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[1], &bk), "graph update"));
   } else {
     const void *fn = plain_kernel_fn(a.bits, mixed);
@@ -613,6 +644,19 @@ int glp_plan_graph_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void
 int glp_plan_graph_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stream) {
   if (!p) return set_error(GLP_ERR_ARG, "null plan");
   return plan_graph_run(p, 2, gen, pen, seed, static_cast<hipStream_t>(stream));
+}
+
+int glp_plan_set_waves(glp_plan *p, unsigned waves_per_workgroup) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  GLP_TRY(plan_use_waves(p, waves_per_workgroup));
+  p->grid = plan_grid(p, p->order);
+  return GLP_OK;
+}
+
+int glp_plan_waves(glp_plan *p, unsigned *waves_per_workgroup) {
+  if (!p || !waves_per_workgroup) return set_error(GLP_ERR_ARG, "null argument");
+  *waves_per_workgroup = p->wpb;
+  return GLP_OK;
 }
 
 int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {

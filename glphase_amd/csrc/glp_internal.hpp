@@ -36,17 +36,20 @@ int hip_check(hipError_t err, const char *what);
 
 int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
                  unsigned step, hipStream_t stream);
-size_t lds_bytes(const KArgs &a);
+// LDS bytes of a workgroup of `waves` wavefronts (the plain path and the
+// single evaluations use 4).
+size_t lds_bytes(const KArgs &a, unsigned waves);
 int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
                    const unsigned *d_qtab, unsigned nwin, unsigned *d_wctr,
-                   unsigned long long *d_ctrs, unsigned grid, hipStream_t stream);
+                   unsigned long long *d_ctrs, unsigned grid, unsigned waves,
+                   hipStream_t stream);
 // Kernel entry points for HIP graph nodes, and the dynamic-LDS attribute they
 // need before any launch.
 const void *plain_kernel_fn(unsigned bits, bool mixed);
-const void *batched_kernel_fn(unsigned bits, bool mixed);
+const void *batched_kernel_fn(unsigned bits, bool mixed, unsigned waves);
 const void *reset_kernel_fn();
 int prepare_kernel_lds(const void *fn, size_t bytes);
-int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu);
+int batched_occupancy(unsigned bits, bool mixed, unsigned waves, size_t smem, int *blocks_per_cu);
 // One batched generation with every device buffer beyond the caller's in
 // the file-scope scratch array of glp_kernels.hip (glp_mh_run_noalloc).
 int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigned Q,

@@ -644,8 +644,12 @@ __global__ void __launch_bounds__(256)
 // slot; the walk stops at the first proposal for which that fails, and the
 // next round re-scores from there.  The result equals the serial loop of
 // insti.cpp:1505-1539 draw for draw; speculation only spends lane time.
-template <int BITS, bool MIXED>
-__global__ void __launch_bounds__(256)
+// WAVES wavefronts per workgroup share one copy of the emission and spread
+// tables: 4 (one workgroup per CU) or 8 (two wavefronts per SIMD where the
+// LDS of the F32 ring or of fewer tables allows it; the compiler then keeps
+// the kernel within 256 VGPRs).
+template <int BITS, bool MIXED, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
     k_batched(const glp_window *__restrict__ wins, const unsigned *__restrict__ order,
               const unsigned *__restrict__ qtab, unsigned nwin, unsigned *__restrict__ wctr,
               unsigned long long *__restrict__ ctrs, KArgs a) {
@@ -859,8 +863,7 @@ __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *b
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t lds_bytes(const KArgs &a) {
-  const size_t waves = 4;  // 256-thread blocks
+size_t lds_bytes(const KArgs &a, unsigned waves) {
   size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * kTranSlot * 4;
   if (a.precision == kPrecF32) b += waves * kRingSlots * 64 * 4;
   return b;
@@ -879,7 +882,7 @@ static int allow_lds(const void *fn, size_t bytes) {
 int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
                  unsigned step, hipStream_t stream) {
   const dim3 block(256), grid((2 * a.N + 255) / 256, nwin);
-  const size_t smem = lds_bytes(a);
+  const size_t smem = lds_bytes(a, 4);
   const bool mixed_ = a.precision != kPrecF32;
   const void *fn = a.bits == 8 ? (mixed_ ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>)
                                : (mixed_ ? (const void *)&k_plain<4, true> : (const void *)&k_plain<4, false>);
@@ -908,31 +911,37 @@ const void *reset_kernel_fn() { return (const void *)&k_reset_ctrs; }
 
 int prepare_kernel_lds(const void *fn, size_t bytes) { return allow_lds(fn, bytes); }
 
-const void *batched_kernel_fn(unsigned bits, bool mixed);
+template <int W>
+static const void *batched_kernel_w(unsigned bits, bool mixed) {
+  if (bits == 8)
+    return mixed ? (const void *)&k_batched<8, true, W> : (const void *)&k_batched<8, false, W>;
+  return mixed ? (const void *)&k_batched<4, true, W> : (const void *)&k_batched<4, false, W>;
+}
 
-static const void *batched_kernel(unsigned bits, bool mixed) {
-  if (bits == 8) return mixed ? (const void *)&k_batched<8, true> : (const void *)&k_batched<8, false>;
-  return mixed ? (const void *)&k_batched<4, true> : (const void *)&k_batched<4, false>;
+const void *batched_kernel_fn(unsigned bits, bool mixed, unsigned waves) {
+  return waves == 8 ? batched_kernel_w<8>(bits, mixed) : batched_kernel_w<4>(bits, mixed);
 }
 
 int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
                    const unsigned *d_qtab, unsigned nwin, unsigned *d_wctr,
-                   unsigned long long *d_ctrs, unsigned grid, hipStream_t stream) {
+                   unsigned long long *d_ctrs, unsigned grid, unsigned waves,
+                   hipStream_t stream) {
+  if (waves != 4 && waves != 8) return set_error(GLP_ERR_ARG, "waves per workgroup must be 4 or 8");
   hipLaunchKernelGGL(k_reset_ctrs, dim3(1), dim3(256), 0, stream, d_ctrs, d_wctr, nwin);
-  const void *fn = batched_kernel(a.bits, a.precision != kPrecF32);
+  const void *fn = batched_kernel_fn(a.bits, a.precision != kPrecF32, waves);
   void *args[] = {(void *)&d_wins, (void *)&d_order, (void *)&d_qtab, (void *)&nwin,
                   (void *)&d_wctr, (void *)&d_ctrs, (void *)&a};
-  hipError_t err = hipLaunchKernel((void *)fn, dim3(grid), dim3(256), args, lds_bytes(a), stream);
+  hipError_t err = hipLaunchKernel((void *)fn, dim3(grid), dim3(64 * waves), args,
+                                   lds_bytes(a, waves), stream);
   return hip_check(err, "k_batched launch");
 }
 
-const void *batched_kernel_fn(unsigned bits, bool mixed) { return batched_kernel(bits, mixed); }
-
-int batched_occupancy(unsigned bits, bool mixed, size_t smem, int *blocks_per_cu) {
-  const void *fn = batched_kernel(bits, mixed);
+int batched_occupancy(unsigned bits, bool mixed, unsigned waves, size_t smem, int *blocks_per_cu) {
+  const void *fn = batched_kernel_fn(bits, mixed, waves);
   GLP_TRY_K(allow_lds(fn, smem));
-  return hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 256, smem),
-                   "occupancy query");
+  return hip_check(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 64 * (int)waves, smem),
+      "occupancy query");
 }
 
 int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samples,
@@ -940,7 +949,7 @@ int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samp
   const dim3 block(256), grid((2 * n + 255) / 256);
   KArgs af = a;  // k_eval accumulates in F32 (Impute::hmm_like returns a float)
   af.precision = kPrecF32;
-  const size_t smem = lds_bytes(af);
+  const size_t smem = lds_bytes(af, 4);
   GLP_TRY_K(allow_lds(a.bits == 8 ? (const void *)&k_eval<8> : (const void *)&k_eval<4>, smem));
   if (a.bits == 8)
     hipLaunchKernelGGL((k_eval<8>), grid, block, smem, stream, d_wins, af, d_samples,
@@ -1018,9 +1027,9 @@ int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigne
   auto *order = reinterpret_cast<unsigned *>(base + kNoAllocOrder);
   auto *qtab = reinterpret_cast<unsigned *>(base + kNoAllocQtab);
   auto *wins = reinterpret_cast<glp_window *>(base + kNoAllocWins);
-  const size_t smem = lds_bytes(a);
+  const size_t smem = lds_bytes(a, 4);
   int bpc = 0;
-  GLP_TRY_K(batched_occupancy(a.bits, a.precision != kPrecF32, smem, &bpc));
+  GLP_TRY_K(batched_occupancy(a.bits, a.precision != kPrecF32, 4, smem, &bpc));
   if (bpc < 1) return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
   // the plan's grid: one wave per floor(32 / Q) chains, 4 waves a block
   const double waves = (double)nwin * std::ceil((double)a.N / (double)(32u / Q));
@@ -1031,7 +1040,7 @@ int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigne
   hipLaunchKernelGGL(k_noalloc_order, dim3((nwin + 255) / 256), dim3(256), 0, stream, wins, nwin,
                      Q, order, qtab);
   GLP_TRY_K(hip_check(hipGetLastError(), "k_noalloc_order launch"));
-  GLP_TRY_K(launch_batched(a, wins, order, qtab, nwin, wctr, ctrs, grid, stream));
+  GLP_TRY_K(launch_batched(a, wins, order, qtab, nwin, wctr, ctrs, grid, 4, stream));
   if (stats_out) {
     unsigned long long h[4];
     GLP_TRY_K(hip_check(hipStreamSynchronize(stream), "synchronize"));

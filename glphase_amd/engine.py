@@ -134,6 +134,15 @@ class Job:
         q = np.ascontiguousarray(np.broadcast_to(np.asarray(lanes, np.uint32), (self.W,)))
         check(lib().glp_plan_set_lanes(self.plan, ptr(q, c_u32p)), "glp_plan_set_lanes")
 
+    def set_waves(self, waves):
+        """Wavefronts per workgroup of the batched kernel (4 or 8)."""
+        check(lib().glp_plan_set_waves(self.plan, waves), "glp_plan_set_waves")
+
+    def waves(self):
+        w = np.zeros(1, np.uint32)
+        check(lib().glp_plan_waves(self.plan, ptr(w, c_u32p)), "glp_plan_waves")
+        return int(w[0])
+
     def lanes(self):
         q, g = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().glp_plan_lanes(self.plan, ctypes.byref(q), ctypes.byref(g)), "glp_plan_lanes")

@@ -193,11 +193,19 @@ int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
 /* Sets passes per chain round per window (1..32). */
 int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
+/* Wavefronts per workgroup of the batched kernel: 4 (the default; one
+ * workgroup per CU with the 8-bit emission table) or 8 (the workgroup's 8
+ * wavefronts share one copy of the emission and spread tables, two per
+ * SIMD; refused with GLP_ERR_ARG where its LDS does not fit a CU).  The
+ * autotuner tries both. */
+int glp_plan_set_waves(glp_plan *plan, unsigned waves_per_workgroup);
+int glp_plan_waves(glp_plan *plan, unsigned *waves_per_workgroup);
 /* Autotuner: times whole generations of the plan (every window, as a real
  * run shares the device) with HIP events on `stream`: first one
  * lanes-per-chain for every window from 1..8, 10, 12, 16, 32 (at most the
- * proposals + 1), then, with several window-size buckets (ceil(log2 sites)),
- * each bucket's value in turn with the others held; keeps the fastest.
+ * proposals + 1) at 4 and at 8 wavefronts per workgroup, then, with several
+ * window-size buckets (ceil(log2 sites)), each bucket's value in turn with
+ * the others held and the workgroup size fixed; keeps the fastest.
  * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
  * trial.  lanes_out (optional) receives the choice per window.
  * Synchronises `stream`. */
