@@ -77,6 +77,9 @@ def L():
                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, u64p, f64p]
         lib.orc_knn.argtypes = [u64p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                 ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int, u32p]
+        lib.orc_knn_rows.argtypes = [u64p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                     ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     u32p, ctypes.c_uint, u32p, ctypes.c_int]
         lib.orc_hmm_work.argtypes = [u64p, ctypes.c_uint, u32p, f32p, f32p, f32p, f32p,
                                      ctypes.c_float, ctypes.c_uint, ctypes.c_uint64,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p]
@@ -374,4 +377,15 @@ def knn(haps, M, nsamp, K, lb=0.0, ub=1.0, using_maf=False):
     out = np.zeros((nsamp, kout), np.uint32)
     L().orc_knn(P(h, u64p), h.shape[1], h.shape[0], M, nsamp, K, lb, ub, int(using_maf),
                 P(out, u32p))
+    return out
+
+
+def knn_rows(haps, M, rows, K, lb=0.0, ub=1.0, using_maf=False, nthreads=8):
+    """The lists of the given samples only, over nthreads threads."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    r = np.ascontiguousarray(rows, np.uint32)
+    kout = L().orc_knn_k_out(K, h.shape[0])
+    out = np.zeros((r.shape[0], kout), np.uint32)
+    L().orc_knn_rows(P(h, u64p), h.shape[1], h.shape[0], M, K, lb, ub, int(using_maf),
+                     P(r, u32p), r.shape[0], P(out, u32p), nthreads)
     return out

@@ -940,6 +940,44 @@ void orc_knn_mask(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M,
   }
 }
 
+/* the list of sample u (haplotypes 2u, 2u + 1) into o[kout]; da, db hold
+ * nhaps entries each */
+static void knn_one(const uint64_t *haps, unsigned wn, unsigned nhaps, const uint64_t *mask,
+                    unsigned u, unsigned kh, orc_dist *da, orc_dist *db, unsigned *o) {
+  const uint64_t *a = haps + (size_t)(2 * u) * wn, *b = a + wn;
+  unsigned n = 0;
+  for (unsigned h = 0; h < nhaps; ++h) {
+    if ((h >> 1) == u) continue;
+    const uint64_t *c = haps + (size_t)h * wn;
+    unsigned x = 0, y = 0;
+    for (unsigned w = 0; w < wn; ++w) {
+      x += (unsigned)__builtin_popcountll((a[w] ^ c[w]) & mask[w]);
+      y += (unsigned)__builtin_popcountll((b[w] ^ c[w]) & mask[w]);
+    }
+    da[n].d = x;
+    da[n].idx = h;
+    db[n].d = y;
+    db[n].idx = h;
+    ++n;
+  }
+  qsort(da, n, sizeof(orc_dist), orc_dist_cmp);
+  qsort(db, n, sizeof(orc_dist), orc_dist_cmp);
+  for (unsigned j = 0; j < kh; ++j) o[2 * j] = da[j].idx;
+  unsigned hb = 0;
+  for (unsigned j = 0; j < kh; ++j) {
+    for (;;) {
+      const unsigned cand = db[hb++].idx;
+      int seen = 0;
+      for (unsigned t = 0; t < kh && !seen; ++t) seen = o[2 * t] == cand;
+      for (unsigned t = 0; t < j && !seen; ++t) seen = o[2 * t + 1] == cand;
+      if (!seen) {
+        o[2 * j + 1] = cand;
+        break;
+      }
+    }
+  }
+}
+
 void orc_knn(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned nsamp,
              unsigned K, double lb, double ub, int using_maf, unsigned *out) {
   uint64_t *mask = (uint64_t *)malloc(sizeof(uint64_t) * wn);
@@ -947,42 +985,35 @@ void orc_knn(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsi
   const unsigned kout = orc_knn_k_out(K, nhaps), kh = kout / 2;
   orc_dist *da = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
   orc_dist *db = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
-  for (unsigned u = 0; u < nsamp; ++u) {
-    const uint64_t *a = haps + (size_t)(2 * u) * wn, *b = a + wn;
-    unsigned n = 0;
-    for (unsigned h = 0; h < nhaps; ++h) {
-      if ((h >> 1) == u) continue;
-      const uint64_t *c = haps + (size_t)h * wn;
-      unsigned x = 0, y = 0;
-      for (unsigned w = 0; w < wn; ++w) {
-        x += (unsigned)__builtin_popcountll((a[w] ^ c[w]) & mask[w]);
-        y += (unsigned)__builtin_popcountll((b[w] ^ c[w]) & mask[w]);
-      }
-      da[n].d = x;
-      da[n].idx = h;
-      db[n].d = y;
-      db[n].idx = h;
-      ++n;
-    }
-    qsort(da, n, sizeof(orc_dist), orc_dist_cmp);
-    qsort(db, n, sizeof(orc_dist), orc_dist_cmp);
-    unsigned *o = out + (size_t)u * kout;
-    for (unsigned j = 0; j < kh; ++j) o[2 * j] = da[j].idx;
-    unsigned hb = 0;
-    for (unsigned j = 0; j < kh; ++j) {
-      for (;;) {
-        const unsigned cand = db[hb++].idx;
-        int seen = 0;
-        for (unsigned t = 0; t < kh && !seen; ++t) seen = o[2 * t] == cand;
-        for (unsigned t = 0; t < j && !seen; ++t) seen = o[2 * t + 1] == cand;
-        if (!seen) {
-          o[2 * j + 1] = cand;
-          break;
-        }
-      }
-    }
-  }
+  for (unsigned u = 0; u < nsamp; ++u)
+    knn_one(haps, wn, nhaps, mask, u, kh, da, db, out + (size_t)u * kout);
   free(da);
   free(db);
+  free(mask);
+}
+
+/* orc_knn for the samples rows[0..nrows) only (out[r] = the list of sample
+ * rows[r]), over nthreads OpenMP threads */
+void orc_knn_rows(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned K,
+                  double lb, double ub, int using_maf, const unsigned *rows, unsigned nrows,
+                  unsigned *out, int nthreads) {
+  uint64_t *mask = (uint64_t *)malloc(sizeof(uint64_t) * wn);
+  orc_knn_mask(haps, wn, nhaps, M, lb, ub, using_maf, mask, NULL);
+  const unsigned kout = orc_knn_k_out(K, nhaps), kh = kout / 2;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    orc_dist *da = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
+    orc_dist *db = (orc_dist *)malloc(sizeof(orc_dist) * nhaps);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 8)
+#endif
+    for (long r = 0; r < (long)nrows; ++r)
+      knn_one(haps, wn, nhaps, mask, rows[r], kh, da, db, out + (size_t)r * kout);
+    free(da);
+    free(db);
+  }
+  (void)nthreads;
   free(mask);
 }

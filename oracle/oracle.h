@@ -160,6 +160,9 @@ void orc_knn_mask(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M,
                   double ub, int using_maf, uint64_t *mask, double *af);
 void orc_knn(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned nsamp,
              unsigned K, double lb, double ub, int using_maf, unsigned *out);
+void orc_knn_rows(const uint64_t *haps, unsigned wn, unsigned nhaps, unsigned M, unsigned K,
+                  double lb, double ub, int using_maf, const unsigned *rows, unsigned nrows,
+                  unsigned *out, int nthreads);
 
 #ifdef __cplusplus
 }

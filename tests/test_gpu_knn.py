@@ -68,6 +68,9 @@ def test_knn_reference_panel_size(orc):
     assert got.shape == (10000, 200)
     want = orc.knn(p["panel"], 2048, 40, 200)
     assert (got[:40] == want).all()
+    # every row, through the row-parallel restatement of the same lists
+    rows = np.arange(10000, dtype=np.uint32)
+    assert (got == orc.knn_rows(p["panel"], 2048, rows, 200, nthreads=16)).all()
 
 
 def test_mh_on_gpu_built_lists(orc):

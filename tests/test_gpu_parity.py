@@ -303,8 +303,8 @@ def test_planted_parents_gpu(orc):
 
 def test_large_window_sampled_parity(orc):
     # N = 5000 x M = 4096, 64 proposals, mixed precision: bit-exact parents
-    # and 1e-12 log-likelihoods for a random subset of chains, and 1e-5
-    # relative agreement of every returned log-likelihood with FP64
+    # and 1e-12 log-likelihoods for every chain, and 1e-5 relative agreement
+    # of sampled returned log-likelihoods with FP64
     say("large window")
     p = helpers.synthetic(5000, 4096, seed=4, depth=2.0, training_size=1 << 16)
     C, pen, seed = 64, 1.0, 31337
@@ -313,11 +313,12 @@ def test_large_window_sampled_parity(orc):
     job.run_batched(gen=0, pen=pen, seed=seed)
     par, like = job.parents()[0], job.like()[0]
     mh, lut = oracle_for(orc, p)
+    # every chain against the oracle (OpenMP over samples)
+    po, co = mh.run(C, pen, seed, precision=PREC_MIXED, nthreads=8)
+    assert (par == po).all()
+    assert (np.abs(like - co) <= 1e-12 * np.abs(co)).all()
     rng = np.random.default_rng(0)
     for i in sorted(rng.choice(p["N"], 24, replace=False)):
-        po, co = mh.run(C, pen, seed, precision=PREC_MIXED, s0=int(i), s1=int(i) + 1)
-        assert (par[i] == po[i]).all(), i
-        assert abs(like[i] - co[i]) <= 1e-12 * abs(co[i])
         emit = orc.sample_emit(p["packed"], int(i), p["M"], 8, lut)
         f64 = orc.hmm_like_f64(p["panel"], par[i], emit, p["tran"])
         assert abs(like[i] - f64) <= 1e-5 * abs(f64)
