@@ -96,6 +96,7 @@ class GlpMhDesc(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("trace_like", ctypes.c_void_p),
         ("trace_acc", ctypes.c_void_p),
+        ("kickstart_from_ref", ctypes.c_int),
     ]
 
 

@@ -182,6 +182,10 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   if (d->prop_mode == GLP_PROP_HOST && d->num_cycles > 0 && !d->extra)
     return set_error(GLP_ERR_ARG, "host proposal mode needs extra");
   if (d->prop_mode < 0 || d->prop_mode > 2) return set_error(GLP_ERR_ARG, "bad prop_mode");
+  // main.cpp:430-433: kickstarting needs a reference panel
+  if (d->kickstart_from_ref && d->prop_mode != GLP_PROP_HOST &&
+      d->num_haps <= 2 * d->num_samples)
+    return set_error(GLP_ERR_ARG, "kickstart_from_ref needs reference haplotypes beyond 2N");
   if (d->precision < 0 || d->precision > 1)
     return set_error(GLP_ERR_ARG, "precision must be GLP_PREC_F32 or GLP_PREC_MIXED");
   if (d->k0 + (uint64_t)d->num_cycles >= 0x80000000ull)
@@ -216,6 +220,7 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   a.bits = d->bits_per_code;
   a.lut_entries = 16u << d->bits_per_code;
   a.precision = d->precision;
+  a.kick = d->kickstart_from_ref ? 1 : 0;
   return GLP_OK;
 }
 

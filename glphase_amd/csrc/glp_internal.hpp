@@ -29,6 +29,7 @@ struct KArgs {
   unsigned lut_entries;  // 16 * 2^bits
   unsigned bits;
   int precision;
+  int kick;  // proposal 0 draws from the reference haplotypes [2N, H)
 };
 
 int set_error(int code, const char *msg);

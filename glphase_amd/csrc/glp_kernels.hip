@@ -572,6 +572,25 @@ __device__ __forceinline__ void proposal_draws(const KArgs &a, unsigned w,
     cand = a.klist[((size_t)w * a.N + i) * a.K + uniform_int(x.y, a.K)];
   else
     cand = unif_excl(x.y, a.H, i);
+  // kickstart (insti.cpp:1510-1513): the generation's first proposal from
+  // the reference haplotypes only; Sampler::SampleHap(I, true)'s rejection
+  // loop (sampler.cpp:12-27) is uniform over them
+  if (a.kick && a.k0 + c == 0u && a.prop_mode != kPropHost) {
+    const unsigned ref0 = 2u * a.N;
+    if (a.prop_mode == kPropKList) {
+      const unsigned *L = a.klist + ((size_t)w * a.N + i) * a.K;
+      unsigned nref = 0;
+      for (unsigned k = 0; k < a.K; ++k) nref += L[k] >= ref0 ? 1u : 0u;
+      if (nref) {
+        unsigned pick = uniform_int(x.y, nref), k = 0;
+        for (;; ++k)
+          if (L[k] >= ref0 && pick-- == 0u) break;
+        cand = L[k];
+      }
+    } else {
+      cand = ref0 + uniform_int(x.y, a.H - ref0);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

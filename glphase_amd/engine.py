@@ -37,7 +37,7 @@ class Job:
 
     def __init__(self, windows, n_samples, n_haps, codebook, pc, bits=8, num_cycles=1,
                  prop_mode=_lib.PROP_UNIFORM, precision=_lib.PREC_F32, klist=None, extra=None,
-                 init_parents=None, trace=False, k0=0, device="cuda:0"):
+                 init_parents=None, trace=False, k0=0, kickstart=False, device="cuda:0"):
         torch = _torch()
         self.torch = torch
         self.device = torch.device(device)
@@ -97,6 +97,7 @@ class Job:
             self.extra_d = _dev(torch, np.asarray(extra, np.uint32), self.device)
             d.extra = self.extra_d.data_ptr()
         d.precision = precision
+        d.kickstart_from_ref = int(kickstart)
         self.trace_like = self.trace_acc = None
         if trace:
             self.trace_like = torch.full((self.W, n_samples, num_cycles + 1), float("nan"),

@@ -43,6 +43,9 @@ class PhaseRun:
     het_aa: [M][N][2] GL pairs (host.read_stbin's drop_first layout).
     pos: [M] site positions (transitions without a genetic map) or tran: [M][3].
 
+    kickstart=True draws the first proposal of every generation from ref_haps
+    only (Insti::s_bKickStartFromRef, insti.cpp:1510-1513; it needs them).
+
     knn_k > 0 restricts candidates to K-lists built on the device from the
     current panel (sample haplotypes plus ref_haps) at generation sa_burnin
     and, with recluster_every, every n sampling generations.  This is an
@@ -53,7 +56,8 @@ class PhaseRun:
 
     def __init__(self, het_aa, pos=None, tran=None, ref_haps=None, bits=8, cycles=None,
                  burnin=56, sampling=200, sa_burnin=28, seed=1, knn_k=0, recluster_every=0,
-                 training_size=0, precision=PREC_MIXED, lanes=0, device="cuda:0"):
+                 training_size=0, precision=PREC_MIXED, lanes=0, kickstart=False,
+                 device="cuda:0"):
         torch = _torch()
         self.torch = torch
         g = np.ascontiguousarray(het_aa, np.float32)
@@ -84,7 +88,7 @@ class PhaseRun:
                        num_cycles=self.C, precision=precision,
                        prop_mode=PROP_KLIST if knn_k else PROP_UNIFORM,
                        klist=np.zeros((1, N, max(2, self._kout())), np.uint32) if knn_k else None,
-                       device=device)
+                       kickstart=kickstart, device=device)
         self.panel_d = self.job.keep[0]
         self.het_aa_d = _dev(torch, g, self.job.device)
         # initial haplotypes (insti.cpp:1316-1365) straight into the device panel

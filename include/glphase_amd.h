@@ -150,6 +150,14 @@ typedef struct {
   int precision;           /* GLP_PREC_* */
   double *trace_like;      /* optional device [num_windows][N][C+1] */
   uint8_t *trace_acc;      /* optional device [num_windows][N][C] */
+  /* Insti::s_bKickStartFromRef (insti.cpp:1510-1513, main.cpp:273,430-433):
+   * nonzero draws the candidate of proposal 0 of every generation (Philox
+   * counter k0 + c == 0) from the reference haplotypes [2 * num_samples,
+   * num_haps) only -- uniformly, or uniformly over the K-list's entries in
+   * that range (GLP_PROP_KLIST; a list with none falls back to the usual
+   * draw, where the reference's rejection loop would not end).  Needs
+   * num_haps > 2 * num_samples; ignored under GLP_PROP_HOST. */
+  int kickstart_from_ref;
 } glp_mh_desc;
 
 /* Bytes of the device emission table for codes of `bits` bits. */

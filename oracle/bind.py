@@ -29,7 +29,8 @@ class OrcMhArgs(ctypes.Structure):
     _fields_ = [("ncycles", ctypes.c_uint), ("pen", ctypes.c_float), ("seed", ctypes.c_uint64),
                 ("window_id", ctypes.c_uint32), ("gen", ctypes.c_uint32), ("k0", ctypes.c_uint32),
                 ("prop_mode", ctypes.c_int), ("precision", ctypes.c_int), ("klist", u32p),
-                ("K", ctypes.c_uint), ("init_parents", u32p), ("extra", u32p)]
+                ("K", ctypes.c_uint), ("init_parents", u32p), ("extra", u32p),
+                ("kickstart", ctypes.c_int)]
 
 
 def build():
@@ -189,12 +190,13 @@ class MH:
 
     def run(self, ncycles, pen, seed, window_id=0, gen=0, k0=0, prop_mode=0, precision=0,
             klist=None, init_parents=None, extra=None, s0=0, s1=None, trace=False,
-            nthreads=0):
+            nthreads=0, kickstart=False):
         s1 = self.nsamp if s1 is None else s1
         a = OrcMhArgs()
         a.ncycles, a.pen, a.seed = ncycles, pen, seed
         a.window_id, a.gen, a.k0 = window_id, gen, k0
         a.prop_mode, a.precision = prop_mode, precision
+        a.kickstart = int(kickstart)
         keep = []
         if klist is not None:
             kl = np.ascontiguousarray(klist, np.uint32)

@@ -372,6 +372,31 @@ static void mh_one(const orc_window *w, const orc_mh_args *a, unsigned i,
       cand = a->klist[(size_t)i * a->K + orc_uniform_int(x[1], a->K)];
     else
       cand = orc_unif_excl(x[1], w->nhaps, i);
+    /* insti.cpp:1510-1513: with kickstarting, the first proposal of the
+     * generation comes from the reference haplotypes [2 nsamp, nhaps) only;
+     * Sampler::SampleHap(I, true) (sampler.cpp:12-27) rejects draws below
+     * 2 nsamp, which leaves the draw uniform over the reference haplotypes
+     * (UnifSampler) or over the K-list's entries among them (KNN; a list
+     * with none keeps the usual draw: the reference would loop forever) */
+    if (a->kickstart && a->k0 + n == 0 && a->prop_mode != ORC_PROP_HOST) {
+      const unsigned ref0 = 2 * w->nsamp;
+      if (a->prop_mode == ORC_PROP_KLIST) {
+        const unsigned *L = a->klist + (size_t)i * a->K;
+        unsigned nref = 0;
+        for (unsigned k = 0; k < a->K; ++k)
+          nref += L[k] >= ref0;
+        if (nref) {
+          unsigned pick = orc_uniform_int(x[1], nref);
+          for (unsigned k = 0; k < a->K; ++k)
+            if (L[k] >= ref0 && pick-- == 0) {
+              cand = L[k];
+              break;
+            }
+        }
+      } else {
+        cand = ref0 + orc_uniform_int(x[1], w->nhaps - ref0);
+      }
+    }
     P[rp] = cand;
     double prop = like_of(w, P, emit, a->precision);
     int acc = prop >= curr || orc_u01(x[2]) < exp((prop - curr) * pen);

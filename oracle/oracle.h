@@ -101,6 +101,7 @@ typedef struct {
   unsigned K;
   const unsigned *init_parents; /* [nsamp][4] for ORC_PROP_HOST (else null) */
   const unsigned *extra;      /* [ncycles][nsamp] for ORC_PROP_HOST */
+  int kickstart;              /* proposal k0 + n == 0 from the reference haps only */
 } orc_mh_args;
 
 /* Runs samples [s0, s1) sequentially.  out_parents [nsamp][4];

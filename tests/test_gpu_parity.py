@@ -130,6 +130,36 @@ def test_batched_eight_wave_workgroups(orc, prec, lanes):
     assert useful == p["N"] * (C + 1)
 
 
+@pytest.mark.parametrize("mode", ["uniform", "klist"])
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_kickstart_from_reference(orc, mode, form):
+    # insti.cpp:1510-1513: proposal 0 of the generation from the reference
+    # haplotypes only, uniform over them or over a K-list's entries among
+    # them (lists without any keep the usual draw)
+    say(f"kickstart mode={mode} form={form}")
+    from glphase_amd.engine import Job
+    N, R, M, C, K = 120, 30, 300, 10, 12
+    p = helpers.synthetic(N, M, seed=31, n_ref=R, training_size=4096)
+    kw, okw = {}, {}
+    if mode == "klist":
+        rng = np.random.default_rng(4)
+        kl = np.stack([rng.choice([h for h in range(p["H"]) if h // 2 != i], K, replace=False)
+                       for i in range(N)]).astype(np.uint32)
+        for i in range(0, N, 3):  # every third list without reference entries
+            kl[i] = rng.choice([h for h in range(2 * N) if h // 2 != i], K, replace=False)
+        kw = dict(prop_mode=PROP_KLIST, klist=kl[None])
+        okw = dict(prop_mode=1, klist=kl)
+    win = dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])
+    job = Job([win], N, p["H"], p["codebook"], p["pc"], bits=8, num_cycles=C,
+              precision=PREC_MIXED, trace=True, kickstart=True, **kw)
+    if form == "plain":
+        job.run_plain(gen=3, pen=0.5, seed=66)
+    else:
+        job.make_plan(4)
+        job.run_batched(gen=3, pen=0.5, seed=66)
+    compare_run(orc, p, job, C, 0.5, 66, PREC_MIXED, gen=3, kickstart=True, **okw)
+
+
 @pytest.mark.parametrize("lanes", [4, 6, 16])
 def test_batched_mixed_precision(orc, lanes):
     say(f"batched mixed lanes={lanes}")

@@ -156,6 +156,24 @@ def test_library_errors_without_gpu():
     assert L.glp_emit_lut_bytes(3, 0) == 0
 
 
+def test_kickstart_needs_reference_haplotypes():
+    # main.cpp:430-433: kickstarting without a reference panel is an error,
+    # refused by the descriptor check before any device work
+    from glphase_amd import _lib
+    L = _lib.lib()
+    fake = ctypes.c_void_p(4096)
+    w = _lib.GlpWindow(64, 1, fake, fake, fake, fake, fake)
+    d = _lib.GlpMhDesc()
+    d.num_windows, d.windows = 1, ctypes.pointer(w)
+    d.num_samples, d.num_haps, d.bits_per_code = 10, 20, 8
+    d.emit_lut, d.num_cycles, d.precision = fake, 4, 1
+    d.kickstart_from_ref = 1
+    out = ctypes.c_void_p()
+    assert L.glp_plan_create(ctypes.byref(d), 0, ctypes.byref(out)) == 1
+    assert b"kickstart" in L.glp_last_error()
+    assert L.glp_plain_init(ctypes.byref(d), None) == 1
+
+
 def test_plan_rejects_code_arrays_past_32_bit_offsets():
     # the kernels index a window's packed codes with 32-bit word offsets: a
     # window of 2^20 sites x 2^14 samples at 8-bit codes holds 2^32 words,
@@ -177,3 +195,40 @@ def test_plan_rejects_code_arrays_past_32_bit_offsets():
     d, w = desc(1 << 20, 1 << 14)  # w keeps the window alive
     assert L.glp_plan_create(ctypes.byref(d), 0, ctypes.byref(out)) == 1
     assert b"2^32" in L.glp_last_error()
+
+
+def test_kickstart_draws_from_reference_haplotypes(orc):
+    # insti.cpp:1510-1513: with kickstarting, proposal 0 of a generation is
+    # drawn from the reference haplotypes [2N, H) only.  With pen = 0 every
+    # proposal is accepted (u < exp(0) = 1), so after one proposal every
+    # chain holds a reference haplotype; without kickstarting only the
+    # chains whose draws happen to reach the 10 reference haplotypes do
+    from tests import helpers
+    N, R = 100, 10
+    p = helpers.synthetic(N, 64, seed=2, n_ref=R, training_size=4096)
+    lut = orc.emit_lut(p["codebook"], p["pc"])
+    mh = orc.MH(p["panel"], p["packed"], 8, lut, p["tran"], N)
+    par_k, _ = mh.run(1, 0.0, 77, kickstart=True)
+    par_n, _ = mh.run(1, 0.0, 77)
+    assert (par_k >= 2 * N).any(axis=1).all()
+    assert (par_n >= 2 * N).any(axis=1).mean() < 0.5
+    # only proposal 0: with k0 = 1 (the generation's second proposal) the
+    # flag changes nothing
+    a, _ = mh.run(1, 0.0, 77, k0=1, kickstart=True)
+    b, _ = mh.run(1, 0.0, 77, k0=1)
+    assert (a == b).all()
+    # K-lists: proposal 0 takes a list entry among the reference haplotypes;
+    # a list with none keeps the usual draw
+    rng = np.random.default_rng(3)
+    K = 8
+    kl = np.stack([rng.choice([h for h in range(2 * N + R) if h // 2 != i], K, replace=False)
+                   for i in range(N)]).astype(np.uint32)
+    kl[::2, :] = np.minimum(kl[::2, :], 2 * N - 1)  # even samples: no reference entries
+    for i in range(0, N, 2):
+        kl[i] = rng.choice([h for h in range(2 * N) if h // 2 != i], K, replace=False)
+    pk, _ = mh.run(1, 0.0, 77, prop_mode=1, klist=kl, kickstart=True)
+    pn, _ = mh.run(1, 0.0, 77, prop_mode=1, klist=kl)
+    odd = np.arange(1, N, 2)
+    has_ref = (kl[odd] >= 2 * N).any(axis=1)
+    assert (pk[odd][has_ref] >= 2 * N).any(axis=1).all()
+    assert (pk[0::2] == pn[0::2]).all()
