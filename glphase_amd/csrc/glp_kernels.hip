@@ -648,16 +648,19 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Wavefronts walk the windows in `order` (longest first).  While on a window
 // a wavefront keeps its data (sites, tran, panel, codes) wave-uniform and
-// holds floor(32/Q) chain slots of Q lane pairs (Q = qtab[window], 1..32;
-// the 32 mod Q pairs past the last slot stay idle); a lane pair scores one
-// parent set per round.
-// Idle slots take the window's next samples from its task counter; when the
-// window runs dry and every slot is idle the wavefront moves on.
+// holds floor(32/Q) home slots (Q = qtab[window], 1..32).  A home slot owns
+// one (window, sample) chain at a time; the chain's state is replicated over
+// the slot's 2Q home lanes.  Idle home slots take the window's next samples
+// from its task counter; when the window runs dry and every slot is idle the
+// wavefront moves on.
 //
-// A slot owns one (window, sample) task.  In a round the slot's pairs score
-// the next Q undecided proposals (the first pair of a fresh task scores its
-// initial parents), all from the slot's current parent set S.  Then every
-// lane of the slot replays the serial MH walk over those Q values in
+// Each round the wavefront's 32 lane pairs are shared among its A active
+// chains: in home-slot order, chain c gets a contiguous run of
+// floor(32/A) pairs, one more for the first 32 mod A chains, so the chains
+// left after their neighbours finish speculate deeper.  A chain's pairs
+// score its next undecided proposals (the first pair of a fresh chain scores
+// its initial parents), all from the chain's current parent set S.  Then
+// the chain's home lanes replay the serial MH walk over those values in
 // proposal order.  A value is the serial chain's value exactly when every
 // slot of S changed in this walk so far equals the proposal's own replace
 // slot; the walk stops at the first proposal for which that fails, and the
@@ -674,30 +677,31 @@ __global__ void __launch_bounds__(64 * WAVES)
               unsigned long long *__restrict__ ctrs, KArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
-  const unsigned lane = threadIdx.x & 63u;
+  const unsigned lane = threadIdx.x & 63u, pair = lane >> 1;
   const bool even = (lane & 1u) == 0u;
   const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double pen = (double)a.pen;
 
-  // wave-uniform window state; j = this pair's index in its slot, sbase =
-  // the slot's first lane
+  // wave-uniform window state; sbase = this lane's home slot's first lane,
+  // home = the lane belongs to a home slot (pairs past the last whole slot
+  // only score)
   int wo = -1;  // position in `order`
-  unsigned bw = 0, Q = 1, j = 0, sbase = lane & ~1u;
-  bool inslot = true;  // false for the pairs past the last whole slot (32 % Q of them)
+  unsigned bw = 0, Q = 1, sbase = lane & ~1u;
+  bool home = false;
   glp_window W{};
-  // slot state, replicated over the slot's lanes
+  // home-slot chain state, replicated over the slot's home lanes
   bool active = false, fresh = false;
   unsigned i = 0, kp = 0;
   unsigned S[4] = {0, 0, 0, 0};
   double cur = 0.0;
   unsigned long long n_eval = 0, n_useful = 0;
-  // Round budget.  In every round each active slot decides at least one
+  // Round budget.  In every round each active chain decides at least one
   // proposal (the first one it scores from its current set is always
-  // valid), so a task takes at most C + 1 rounds, and a round with no
-  // active slot moves the wave to its next window.  The wave's rounds
-  // therefore never exceed (C + 1) per task it drew plus one per window
-  // and one to leave; a wave past that budget flags the run as failed and
-  // exits instead of spinning.
+  // valid, and every chain holds at least one pair), so a task takes at
+  // most C + 1 rounds, and a round with no active chain moves the wave to
+  // its next window.  The wave's rounds therefore never exceed (C + 1) per
+  // task it drew plus one per window and one to leave; a wave past that
+  // budget flags the run as failed and exits instead of spinning.
   unsigned long long rounds = 0, budget = (unsigned long long)nwin + 2ull;
 
   for (;;) {
@@ -705,9 +709,9 @@ __global__ void __launch_bounds__(64 * WAVES)
       if (lane == 0) atomicOr(&ctrs[3], 1ull);
       break;
     }
-    // 1. idle slots take the next samples of the current window
+    // 1. idle home slots take the next samples of the current window
     if (wo >= 0) {
-      const bool want = inslot && lane == sbase && !active;
+      const bool want = home && lane == sbase && !active;
       const uint64_t wm = __ballot(want);
       if (wm) {
         unsigned base = 0;
@@ -718,7 +722,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         budget += (unsigned long long)__popcll(__ballot(took)) * ((unsigned long long)a.C + 1ull);
         unsigned t = took ? base + rank : 0xFFFFFFFFu;
         t = __shfl(t, sbase, 64);
-        if (!active && t != 0xFFFFFFFFu) {
+        if (home && !active && t != 0xFFFFFFFFu) {
           i = t;
           initial_parents(a, W, bw, i, S);
           active = true;
@@ -727,64 +731,89 @@ __global__ void __launch_bounds__(64 * WAVES)
         }
       }
     }
-    if (!__any(active)) {
+    // the active chains, one bit at each active home slot's first lane
+    const uint64_t am = __ballot(home && lane == sbase && active);
+    if (am == 0ull) {
       // 2. the window is done for this wavefront: move to the next one
       if (++wo >= (int)nwin) break;
       bw = __builtin_amdgcn_readfirstlane(order[wo]);
       Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
-      {
-        const unsigned pair = lane >> 1, slot = pair / Q;
-        j = pair - slot * Q;
-        sbase = 2u * Q * slot;
-        inslot = slot < 32u / Q;
-      }
+      const unsigned slot = pair / Q;
+      sbase = 2u * Q * slot;
+      home = slot < 32u / Q;
       W = wins[bw];
       continue;
     }
 
-    // 3. this lane's job: initial set or one speculative proposal
+    // 3. the pairs of the round: chain rank c (in home-slot order) holds the
+    //    contiguous pairs [start_c, start_c + q_c), q_c = base + [c < extra]
+    const unsigned A = (unsigned)__popcll(am);
+    const unsigned base = 32u / A, extra = 32u % A, big = extra * (base + 1u);
+    unsigned c, j;  // this pair's chain rank and place in the chain's run
+    if (pair < big) {
+      c = pair / (base + 1u);
+      j = pair - c * (base + 1u);
+    } else {
+      c = extra + (pair - big) / base;
+      j = pair - big - (c - extra) * base;
+    }
+    uint64_t m = am;  // the c-th set bit of am is the chain's home lane
+    for (unsigned k = 0; k < c; ++k) m &= m - 1ull;
+    const unsigned hl = (unsigned)__builtin_ctzll(m);
+    const unsigned ci = __shfl(i, hl, 64), ckp = __shfl(kp, hl, 64);
+    const bool cfresh = __shfl(fresh ? 1 : 0, hl, 64) != 0;
+    unsigned P[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) P[q] = __shfl(S[q], hl, 64);
+
+    // 4. this pair's job: the chain's initial set or one speculative proposal
     bool work = false;
     unsigned rp = 4, cand = 0;
     uint32_t ub = 0;
-    unsigned P[4] = {S[0], S[1], S[2], S[3]};
-    if (active) {
-      const int job = fresh ? (int)j - 1 : (int)j;
+    {
+      const int job = cfresh ? (int)j - 1 : (int)j;
       if (job < 0) {
         work = true;
-      } else if (kp + (unsigned)job < a.C) {
+      } else if (ckp + (unsigned)job < a.C) {
         work = true;
-        proposal_draws(a, bw, i, kp + (unsigned)job, rp, cand, ub);
+        proposal_draws(a, bw, ci, ckp + (unsigned)job, rp, cand, ub);
         P[0] = rp == 0 ? cand : P[0];
         P[1] = rp == 1 ? cand : P[1];
         P[2] = rp == 2 ? cand : P[2];
         P[3] = rp == 3 ? cand : P[3];
       }
     }
-    // the whole wavefront runs the pass; lanes without work score their
-    // slot's current set and the value is ignored
-    double v = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+    // the whole wavefront runs the pass; pairs without work score their
+    // chain's current set and the value is ignored
+    double v = score_set<BITS, MIXED>(W, P, ci, a.N, smem, a.lut_entries, a.norm);
     if (work && even) ++n_eval;
 
-    // 4. serial MH walk over the slot's values (every lane of the slot
-    //    replays it, so the slot state stays replicated)
-    const double v0 = __shfl(v, sbase, 64);
+    // 5. serial MH walk over each active chain's values, replayed by every
+    //    lane of its home slot (so the chain state stays replicated)
+    const bool mine = home && active;
+    const uint64_t below_base = sbase == 0u ? 0ull : (~0ull >> (64u - sbase));
+    const unsigned myc = (unsigned)__popcll(am & below_base);  // this home slot's rank
+    const unsigned qn = myc < extra ? base + 1u : base;
+    const unsigned start = myc < extra ? myc * (base + 1u) : big + (myc - extra) * base;
+    const unsigned qmax = base + (extra ? 1u : 0u);
     const int jstart = fresh ? 1 : 0;
-    if (active && fresh) {
+    const double v0 = __shfl(v, (2u * start) & 63u, 64);
+    if (mine && fresh) {
       cur = v0;
       if (lane == sbase && a.trace_like) a.trace_like[((size_t)bw * a.N + i) * (a.C + 1)] = v0;
     }
     unsigned changed = 0, decided = 0;
-    bool stop = !active;
-    for (int jj = 0; jj < (int)Q; ++jj) {
-      // pairs outside a whole slot are never active; keep their reads in the wave
-      const unsigned src = (sbase + 2u * (unsigned)jj) & 63u;
+    bool stop = !mine;
+    for (unsigned jj = 0; jj < qmax; ++jj) {
+      // the reads run on every lane; only a chain's own run is used
+      const unsigned src = (2u * (start + jj)) & 63u;
       const double vj = __shfl(v, src, 64);
       const unsigned rpj = __shfl(rp, src, 64);
       const unsigned cj = __shfl(cand, src, 64);
       const uint32_t uj = __shfl(ub, src, 64);
-      if (!stop && jj >= jstart) {
-        const unsigned kk = kp + (unsigned)(jj - jstart);
-        if (kk >= a.C || (changed & ~(1u << rpj)) != 0u) {
+      if (!stop && (int)jj >= jstart) {
+        const unsigned kk = kp + (jj - (unsigned)jstart);
+        if (jj >= qn || kk >= a.C || (changed & ~(1u << rpj)) != 0u) {
           stop = true;
         } else {
           const bool acc = mh_accept(vj, cur, pen, uj);
@@ -797,7 +826,7 @@ __global__ void __launch_bounds__(64 * WAVES)
             changed |= 1u << rpj;
           }
           ++decided;
-          if (j == (unsigned)jj && even) {
+          if (lane == sbase) {
             const size_t row = (size_t)bw * a.N + i;
             if (a.trace_like) a.trace_like[row * (a.C + 1) + 1 + kk] = vj;
             if (a.trace_acc) a.trace_acc[row * a.C + kk] = acc ? 1 : 0;
@@ -805,7 +834,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         }
       }
     }
-    if (active) {
+    if (mine) {
       kp += decided;
       fresh = false;
       if (lane == sbase) n_useful += decided + (jstart ? 1u : 0u);
