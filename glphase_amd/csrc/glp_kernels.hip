@@ -132,28 +132,33 @@ __device__ __forceinline__ float dpp(float x) {
 //   ring and their correctly rounded logs are subtracted, oldest first, at
 //   the end of each 32-site chunk -- the same float sum in the same order.
 //   MIXED: the FP32 recurrence, with l * t0 and l' * t2 fused into the
-//   partner term by two FMAs per state, forms no site sums.  After every 4th site
-//   (sites 3, 7, 11, ...) the pair scales its four probabilities by the
-//   power of two 2^k that brings their sum into [1, 2) (k >= 0, since the
-//   sum never exceeds 2), which rounds nothing; the lane adds up the k and
-//   the score is log(last sum) - (sum of k) ln 2 in FP64.  A site can lower
-//   the sum by as much as the smallest emission (mu^2 for a confident
-//   genotype that no parent carries: 2^-33 at 10,000 haplotypes), so four
-//   sites can leave the normal range.  The lane keeps the largest k; a pass
-//   whose sum fell below 2^-99 at a scaling (k >= 100) or at its end is
-//   redone with the scaling after every site (score_set), whose sums stay
-//   above the smallest emission.
+//   partner term by two FMAs per state, forms no site sums.  The pass
+//   starts from l = 2^98 (= 1/4 * 2^100) and after every 8th site (sites
+//   7, 15, 23, ...) the pair scales its four probabilities by the power of
+//   two 2^k that brings their sum into [2^100, 2^101), which rounds nothing;
+//   the lane adds up the k (starting from 100) and the score is
+//   log(last sum) - (sum of k) ln 2 in FP64.  A site lowers the sum by at
+//   most the smallest emission (mu^2 for a confident genotype that no
+//   parent carries: 2^-33 at 10,000 haplotypes, 2^-13 for one mismatch), so
+//   eight sites from 2^100 stay far above the normal range unless nearly
+//   every one of them is a double mismatch.  The lane keeps the largest k;
+//   a pass whose sum fell below 2^-99 at a scaling (k >= 200) or at its end,
+//   or whose last sum is not finite, is redone with the scaling after every
+//   site (score_set), whose sums stay above 2^100 times the smallest
+//   emission.
 struct PairState {
   float la, lb;
-  unsigned nres;  // MIXED: sum of the scale exponents so far
-  int kmax;       // MIXED: largest scale exponent so far
+  int nres;  // MIXED: sum of the scale exponents so far
+  int kmax;  // MIXED: largest scale exponent so far
   float fscore;
   unsigned nq;
 };
 
-// MIXED: a scaling by 2^k with k >= kTinyK means the sum had fallen below
-// 2^-99 since the previous one
-constexpr int kTinyK = 100;
+// MIXED: the scalings bring the pair's sum into [2^kScaleT, 2^(kScaleT+1));
+// one by 2^k with k >= kTinyK means the sum had fallen below 2^-99 since
+// the previous one
+constexpr int kScaleT = 100;
+constexpr int kTinyK = 200;
 
 constexpr double kLn2 = 0.6931471805599453;
 
@@ -187,18 +192,19 @@ __device__ __forceinline__ void pair_step(PairState &st, const float2 e, float t
   st.nq += r ? 1u : 0u;
 }
 
-// MIXED, after every 4th site: scale the pair's probabilities by 2^k with
-// (l00 + l11) + (l01 + l10) in [1, 2) after it (frexp exponent e, k = 1 - e;
-// the sum is below 2, so k >= 0 and the scaling is exact).  Both lanes form
-// the same sum (addition commutes), so the pair scales together.
+// MIXED, after every 8th site: scale the pair's probabilities by 2^k with
+// (l00 + l11) + (l01 + l10) in [2^100, 2^101) after it (frexp exponent e,
+// k = 101 - e; the scaling is exact while the result stays normal, which
+// the guard checks).  Both lanes form the same sum (addition commutes), so
+// the pair scales together.
 template <bool TRACK>
 __device__ __forceinline__ void mixed_scale(PairState &st) {
   const float p = st.la + st.lb;
   const float s = dpp<kDppSwap>(p) + p;
-  const int k = 1 - __builtin_amdgcn_frexp_expf(s);
+  const int k = (kScaleT + 1) - __builtin_amdgcn_frexp_expf(s);
   st.la = __builtin_amdgcn_ldexpf(st.la, k);
   st.lb = __builtin_amdgcn_ldexpf(st.lb, k);
-  st.nres += (unsigned)k;
+  st.nres += k;
   if (TRACK) st.kmax = k > st.kmax ? k : st.kmax;
 }
 
@@ -279,8 +285,8 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       if (TAIL && (unsigned)s >= n) break;
       pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
-      // MIXED: every 4th site (FAST), or every site (the redo)
-      if (MIXED && (!FAST || (k & 3) == 3)) mixed_scale<FAST>(st);
+      // MIXED: every 8th site (FAST), or every site (the redo)
+      if (MIXED && (!FAST || k == 7)) mixed_scale<FAST>(st);
     }
   }
 }
@@ -318,7 +324,7 @@ __device__ __forceinline__ void group_load_et(GroupLds &G, const uint32_t (&cw)[
   }
 }
 
-// Sites k0 .. k0 + 3 of group g, from its loaded inputs.
+// Sites k0 .. k0 + 3 of group g (k0 = 0 or 4), from its loaded inputs.
 template <bool MIXED, bool FAST>
 __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, int g, int k0,
                                             float norm, float nrm0, float *ring) {
@@ -329,7 +335,7 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
                            (g == 0 && k == 0) ? nrm0 : norm, ring);
     if (MIXED && !FAST) mixed_scale<false>(st);  // the redo: every site
   }
-  if (MIXED && FAST) mixed_scale<true>(st);
+  if (MIXED && FAST && k0 == 4) mixed_scale<true>(st);  // after sites 7, 15, ...
 }
 
 // A full 32-site chunk as a software pipeline over its 4 groups, in two
@@ -439,8 +445,9 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
       reinterpret_cast<const uint2 *>(smem + 16u * lut_entries) + ((lane & 1u) ? 1024u : 0u);
   const unsigned ntr = 3u * M;
   PairState st;
-  st.la = st.lb = 0.25f;
-  st.nres = 0;
+  // MIXED starts scaled by 2^kScaleT
+  st.la = st.lb = MIXED ? 0x1p98f : 0.25f;
+  st.nres = MIXED ? kScaleT : 0;
   st.kmax = 0;
   st.fscore = 0.0f;
   st.nq = 0;
@@ -451,8 +458,8 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   // the other buffer while this one is used, with no register copies
   ChunkIn<BITS> X, Y;
   load_chunk<BITS>(X, 0, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
-  // site 0's row becomes (1, 0, 0), so that from l = 1/4 the generic step
-  // gives l_xy = 1/4 * e_xy exactly
+  // site 0's row becomes (1, 0, 0), so that from l = 1/4 (F32) or 2^98
+  // (MIXED) the generic step gives l_xy = l * e_xy exactly
   if (lane == 0u) X.tp = make_float2(1.0f, 0.0f);
   if (lane == 1u) X.tp.x = 0.0f;
   // chunk ch's tran sits in LDS slot ch & 1 of the wave's two; a full
@@ -488,8 +495,10 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   }
   const float last = pair_sum(st);
   if (MIXED) {
-    // a sum below 2^-99 (or 0): NaN asks score_set for the redo
-    if (FAST && (st.kmax >= kTinyK || last < 0x1p-99f)) return __builtin_nan("");
+    // a sum below 2^-99 (or 0), or one that is not finite: NaN asks
+    // score_set for the redo
+    if (FAST && (st.kmax >= kTinyK || !(last >= 0x1p-99f && last <= 0x1.fffffep127f)))
+      return __builtin_nan("");
     return log((double)last) - (double)st.nres * kLn2;
   }
   return (double)(st.fscore + cr_logf(last));
@@ -520,8 +529,8 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
   // F32: a subnormal (or zero) site sum made the short reciprocal NaN: the
   // wavefront redoes its passes with the exact one (recip_normal is exact
   // for normal sums, so a value that is not NaN is already the exact one).
-  // MIXED: a sum fell below 2^-99 between two scalings: the redo scales
-  // after every site.  Each lane keeps its first value unless it was NaN.
+  // MIXED: a sum fell below 2^-99 between two scalings, or the last one is
+  // not finite: the redo scales after every site.  Each lane keeps its first value unless it was NaN.
   if (__builtin_expect(__any(v != v), 0)) {
     const double x = forward_pass<BITS, MIXED, false>(f0, f1, m0, m1, W.packed + sample, N,
                                                       W.tran, W.num_sites, smem, lut_entries, wtr,

@@ -48,7 +48,7 @@ enum { GLP_PROP_UNIFORM = 0, GLP_PROP_KLIST = 1, GLP_PROP_HOST = 2 };
 /* Arithmetic of the forward pass (glp_mh_desc.precision).  GLP_PREC_F32: the reference's FP32
  * recurrence and FP32 log accumulation (bit-identical to Impute::hmm_like);
  * GLP_PREC_MIXED: the same FP32 recurrence, kept in range by exact power-of-two
- * scalings after every 4th site instead of the per-site 1/sum rescale, FP64
+ * scalings after every 8th site instead of the per-site 1/sum rescale, FP64
  * log-likelihood and FP64 MH ratio.  An FP16 emission table was measured at up to 1.3e-4 relative error
  * against the FP64 forward pass (bound 1e-5) and is not provided. */
 enum { GLP_PREC_F32 = 0, GLP_PREC_MIXED = 1 };

@@ -155,8 +155,8 @@ def hmm_like_mixed(haps, parents, emit, tran):
 
 
 def hmm_like_mixed_pass(haps, parents, emit, tran, every):
-    """One mixed-precision pass (scaled every 4th site or every site) and
-    whether a sum fell below 2^-99 in it."""
+    """One mixed-precision pass (scaled every 8th site or every site) and
+    whether a sum fell below 2^-99 in it, or its last sum is not finite."""
     h = np.ascontiguousarray(haps, np.uint64)
     pr = np.ascontiguousarray(parents, np.uint32)
     e = np.ascontiguousarray(emit, np.float32)

@@ -194,29 +194,30 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
 
 /* The reference's FP32 recurrence (impute.cpp:354-366), with each state's
  * products l * t0 and l' * t2 fused with the (pair sum) * t1 term into two
- * FMAs, and without the per-site rescale test (impute.cpp:367-372).
- * Instead, after every 4th site (m = 3, 7, 11, ...; every = 4) or after
- * every site including site 0 (every = 1), the four probabilities are
- * scaled by 2^k, where e is the frexp exponent of (l00 + l11) + (l01 + l10)
- * and k = 1 - e brings that sum into [1, 2). The probabilities never sum to
- * 2 or more, so k >= 0 and the scaling rounds nothing. The score is
- * log(last sum) - (sum of k) ln 2 in FP64: the reference's score -=
- * log(factor) sum (impute.cpp:367-374) with every factor exact and the sum
- * of logs carried as an integer.  *tiny is set when a scaling had k >= 100
- * (the sum had fallen below 2^-99) or the last sum is below 2^-99. */
+ * FMAs, and without the per-site rescale test (impute.cpp:367-372). The
+ * pass starts from 1/4 * 2^100 and, after every 8th site (m = 7, 15, 23,
+ * ...; every = 8) or after every site including site 0 (every = 1), the
+ * four probabilities are scaled by 2^k, where e is the frexp exponent of
+ * (l00 + l11) + (l01 + l10) and k = 101 - e brings that sum into
+ * [2^100, 2^101); the scaling rounds nothing while the values stay normal.
+ * The score is log(last sum) - (sum of k, from 100) ln 2 in FP64: the
+ * reference's score -= log(factor) sum (impute.cpp:367-374) with every
+ * factor exact and the sum of logs carried as an integer.  *tiny is set
+ * when a scaling had k >= 200 (the sum had fallen below 2^-99) or the last
+ * sum is below 2^-99 or not finite. */
 double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
                                const float *emit, const float *tran, unsigned M,
                                unsigned every, int *tiny) {
   const uint64_t *f0 = haps + (size_t)P[0] * wn, *f1 = haps + (size_t)P[1] * wn,
                  *m0 = haps + (size_t)P[2] * wn, *m1 = haps + (size_t)P[3] * wn;
   const float *e = emit, *t = tran;
-  long ksum = 0;
+  long ksum = 100;
   int ex;
   float pe, po;
-  float l00 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
-        l01 = 0.25f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
-  float l10 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
-        l11 = 0.25f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
+  float l00 = 0x1p98f * e[(hbit(f0, 0) << 1) | hbit(m0, 0)],
+        l01 = 0x1p98f * e[(hbit(f0, 0) << 1) | hbit(m1, 0)];
+  float l10 = 0x1p98f * e[(hbit(f1, 0) << 1) | hbit(m0, 0)],
+        l11 = 0x1p98f * e[(hbit(f1, 0) << 1) | hbit(m1, 0)];
   float b00, b01, b10, b11;
   for (unsigned m = 0; m < M; m++, e += 4, t += 3) {
     if (m) {
@@ -231,10 +232,10 @@ double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned
       l10 = b10 * e[(hbit(f1, m) << 1) | hbit(m0, m)];
       l11 = b11 * e[(hbit(f1, m) << 1) | hbit(m1, m)];
     }
-    if (every == 1 || (m & 3u) == 3u) {
+    if (every == 1 || (m & 7u) == 7u) {
       (void)frexpf((l00 + l11) + (l01 + l10), &ex);
-      const int k = 1 - ex;
-      if (k >= 100)
+      const int k = 101 - ex;
+      if (k >= 200)
         *tiny = 1;
       l00 = ldexpf(l00, k);
       l01 = ldexpf(l01, k);
@@ -244,19 +245,19 @@ double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned
     }
   }
   const float last = l00 + l01 + l10 + l11;
-  if (last < 0x1p-99f)
+  if (!(last >= 0x1p-99f && last <= FLT_MAX))
     *tiny = 1;
   return log((double)last) - (double)ksum * ORC_LN2;
 }
 
-/* Mixed precision: the pass scaled every 4th site; when its sum fell below
- * 2^-99 (a run of sites whose emissions are all near mu^2) or it is NaN,
- * the pass scaled after every site. */
+/* Mixed precision: the pass scaled every 8th site; when its sum fell below
+ * 2^-99 (a run of sites whose emissions are all near mu^2), its last sum is
+ * not finite, or it is NaN, the pass scaled after every site. */
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M) {
   int tiny = 0;
-  const double v = orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 4, &tiny);
+  const double v = orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 8, &tiny);
   if (!tiny && v == v)
     return v;
   return orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 1, &tiny);

@@ -67,8 +67,9 @@ float orc_hmm_like_f32(const uint64_t *haps, unsigned wn, const unsigned P[4],
 double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
                           const unsigned P[4], const float *emit,
                           const float *tran, unsigned M);
-/* one mixed-precision pass, scaled every 4th site (every = 4) or after every
- * site (every = 1); *tiny is set when a sum fell below 2^-99 */
+/* one mixed-precision pass, scaled every 8th site (every = 8) or after every
+ * site (every = 1); *tiny is set when a sum fell below 2^-99 or the last sum
+ * is not finite */
 double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
                                const float *emit, const float *tran, unsigned M,
                                unsigned every, int *tiny);

@@ -154,7 +154,8 @@ def hom_alt_run_problem(hn, N=64, M=300, runs=((40, 12), (150, 5), (262, 30)), s
     length), GLs certain of the alternative homozygote (codeword (0, 0):
     AA = 1) while every haplotype of the panel carries the reference allele.
     There every parent state emits pc[0][3] = mu^2 (2^-33.2 at hn = 10,000,
-    2^-37.5 at 40,000), so four such sites take the state sum below 2^-126.
+    2^-37.5 at 40,000), so eight such sites take the state sum down by more
+    than 2^-199 (from 2^100 below 2^-99).
     Elsewhere codes and haplotypes are random."""
     rng = np.random.default_rng(seed)
     bits = 4

@@ -172,11 +172,11 @@ def test_batched_mixed_precision(orc, lanes):
 
 
 @pytest.mark.parametrize("bits", [4, 8])
-@pytest.mark.parametrize("M", [1, 3, 4, 5, 31, 32, 33, 63, 64, 65, 97, 301])
+@pytest.mark.parametrize("M", [1, 3, 4, 5, 7, 8, 9, 31, 32, 33, 39, 40, 41, 63, 64, 65, 97, 301])
 def test_mixed_precision_site_counts(orc, bits, M):
-    # the mixed-precision recurrence scales after every 4th site and reads a
+    # the mixed-precision recurrence scales after every 8th site and reads a
     # window's last chunk through clamped loads: site counts around the
-    # 4-site and 32-site boundaries, r = 0.3 so that every transition term
+    # 8-site and 32-site boundaries, r = 0.3 so that every transition term
     # counts, through the batched and the plain path
     say(f"mixed precision bits={bits} M={M}")
     p = helpers.synthetic(40, M, bits=bits, seed=3 * M + bits, training_size=4096)
@@ -439,9 +439,10 @@ def test_subnormal_site_sums_mh(orc, prec, form):
 def test_mixed_smallest_emissions(orc, hn, form):
     # runs of sites where every parent state emits mu^2 (a certain
     # alternative homozygote against all-reference parents) at the mutation
-    # rate of hn haplotypes: a 4-site sum falls below 2^-126, the wavefront
-    # redoes its passes with the scaling after every site, and parents and
-    # log-likelihoods equal the oracle's; every log-likelihood is finite
+    # rate of hn haplotypes: an 8-site run takes the sum from 2^100 below
+    # 2^-99, the wavefront redoes its passes with the scaling after every
+    # site, and parents and log-likelihoods equal the oracle's; every
+    # log-likelihood is finite
     say(f"mixed precision, smallest emissions hn={hn} form={form}")
     p = helpers.hom_alt_run_problem(hn, seed=hn % 97)
     C, pen, seed = 20, 1.0, 4242

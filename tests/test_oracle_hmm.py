@@ -152,10 +152,10 @@ def test_sample_emit_decodes_packed_codes(orc):
 @pytest.mark.parametrize("hn", [10_000, 40_000])
 def test_mixed_smallest_emissions_match_fp64(orc, hn):
     # runs of sites where every parent state emits mu^2 (a certain
-    # alternative homozygote against all-reference parents) take a 4-site
-    # sum below 2^-126 at these panel sizes (mu^2 = 2^-33.2 and 2^-37.5);
-    # the 4-site pass flags it and the mixed result comes from the pass
-    # scaled after every site, within 1e-7 of FP64
+    # alternative homozygote against all-reference parents) take the sum
+    # from 2^100 below 2^-99 within 8 sites at these panel sizes (mu^2 =
+    # 2^-33.2 and 2^-37.5); the 8-site pass flags it and the mixed result
+    # comes from the pass scaled after every site, within 1e-7 of FP64
     from tests import helpers
     p = helpers.hom_alt_run_problem(hn)
     lut = orc.emit_lut(p["codebook"], p["pc"])
@@ -168,24 +168,24 @@ def test_mixed_smallest_emissions_match_fp64(orc, hn):
         P = rng.choice([h for h in range(p["H"]) if h // 2 != i], 4).astype(np.uint32)
         emit = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut)
         f64 = orc.hmm_like_f64(p["panel"], P, emit, p["tran"])
-        v4, tiny4 = orc.hmm_like_mixed_pass(p["panel"], P, emit, p["tran"], 4)
+        v8, tiny8 = orc.hmm_like_mixed_pass(p["panel"], P, emit, p["tran"], 8)
         v1, tiny1 = orc.hmm_like_mixed_pass(p["panel"], P, emit, p["tran"], 1)
-        assert tiny4 and not tiny1
+        assert tiny8 and not tiny1
         got = orc.hmm_like_mixed(p["panel"], P, emit, p["tran"])
         assert got == v1
         assert abs(got - f64) <= 1e-7 * abs(f64)
-        worst_unguarded = max(worst_unguarded, abs(v4 - f64) / abs(f64) if np.isfinite(v4) else np.inf)
-    # without the guard the 4-site pass is off by more than the 1e-5 bound
+        worst_unguarded = max(worst_unguarded, abs(v8 - f64) / abs(f64) if np.isfinite(v8) else np.inf)
+    # without the guard the 8-site pass is off by more than the 1e-5 bound
     assert worst_unguarded > 1e-5
 
 
 def test_mixed_guard_keeps_ordinary_passes(orc):
-    # a pass whose sums stay above 2^-99 is the 4-site pass itself
+    # a pass whose sums stay above 2^-99 is the 8-site pass itself
     haps, emit, tran = problem(2000, seed=7)
     P = np.array([1, 6, 3, 2], np.uint32)
-    v4, tiny = orc.hmm_like_mixed_pass(haps, P, emit, tran, 4)
+    v8, tiny = orc.hmm_like_mixed_pass(haps, P, emit, tran, 8)
     assert not tiny
-    assert orc.hmm_like_mixed(haps, P, emit, tran) == v4
+    assert orc.hmm_like_mixed(haps, P, emit, tran) == v8
     v1, _ = orc.hmm_like_mixed_pass(haps, P, emit, tran, 1)
     f64 = orc.hmm_like_f64(haps, P, emit, tran)
     assert abs(v1 - f64) <= 1e-7 * abs(f64)
