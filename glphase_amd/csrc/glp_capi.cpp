@@ -197,10 +197,13 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
       return set_error(GLP_ERR_ARG, "words_per_hap must be ceil(num_sites / 64)");
     if (!W.panel || !W.packed || !W.tran || !W.parents || !W.like)
       return set_error(GLP_ERR_ARG, "window with a null pointer");
-    // the kernels index a window's packed codes with 32-bit word offsets
+    // the kernels read a window's panel and packed codes through buffer
+    // descriptors with 32-bit byte offsets
     const uint64_t code_words = ((uint64_t)W.num_sites * d->bits_per_code + 31) / 32;
-    if (code_words * d->num_samples >= (1ull << 32))
-      return set_error(GLP_ERR_ARG, "a window's packed codes exceed 2^32 words");
+    if (code_words * d->num_samples * 4 >= (1ull << 32))
+      return set_error(GLP_ERR_ARG, "a window's packed codes reach 4 GiB");
+    if ((uint64_t)d->num_haps * W.words_per_hap * 8 >= (1ull << 32))
+      return set_error(GLP_ERR_ARG, "a window's panel reaches 4 GiB");
   }
   a.N = d->num_samples;
   a.H = d->num_haps;
@@ -723,6 +726,9 @@ static int ctx_window(glp_ctx *c, const uint64_t *hap_panel, size_t n_panel_word
     return set_error(GLP_ERR_ARG, "packed GLs have the wrong size for num_sites x num_samples");
   const unsigned wn = (num_sites + 63) / 64;
   if (n_panel_words % wn != 0) return set_error(GLP_ERR_ARG, "panel size not a multiple of words per hap");
+  // the kernels read the panel and codes with 32-bit byte offsets
+  if ((uint64_t)n_panel_words * 8 >= (1ull << 32) || (uint64_t)expect * 4 >= (1ull << 32))
+    return set_error(GLP_ERR_ARG, "panel or packed GLs reach 4 GiB");
   GLP_TRY(ensure_lut(c));
   if (c->codebook.size() != 2 * ((size_t)1 << c->bits))
     return set_error(GLP_ERR_STATE, "codebook size does not match bits_per_code of the packed GLs");

@@ -378,21 +378,49 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
   group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
-// Floats o and o + 1 of tran (n floats in all).  Both loads always run, from
-// addresses clamped to the last float, so that no load sits under a branch
-// (when n is odd the last float stands alone: its partner is a copy of it).
-// Values past the end are never used in arithmetic: the sites they belong
-// to are not stepped.
-__device__ __forceinline__ float2 tran_pair(gptr_t<float> tran, unsigned o, unsigned n) {
-  const float x = tran[o < n ? o : n - 1u];
-  const float y = tran[o + 1u < n ? o + 1u : n - 1u];
-  return make_float2(x, y);
+// The global inputs of a pass are read through raw buffer loads: the
+// buffer descriptor (base, size) sits in scalar registers and each load
+// adds a per-lane byte offset (a register fixed for the pass) to a scalar
+// one (the chunk's), so no load needs a 64-bit address computed per lane.
+// Raw buffers (stride 0) bounds-check the per-lane offset plus the
+// instruction offset against the size, dword by dword: a dword at or past
+// it reads 0 and touches no memory.  The panel and code buffers are
+// declared 4 GB - 1 long and every offset into them stays inside the array
+// (the chunk and word indices are clamped; the host refuses arrays of 4 GB
+// or more); the tran buffer is declared its true size and relies on the
+// check.
+typedef __amdgpu_buffer_rsrc_t brsrc_t;
+constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit data format
+
+__device__ __forceinline__ brsrc_t buf_rsrc(const void *p, uint32_t bytes) {
+  // the base must be wave-uniform: it is, but make it an SGPR for certain
+  const uint64_t a = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  void *u = (void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(u, 0, (int)bytes, kBufDword3);
 }
+// soff is wave-uniform in every call; readfirstlane states it, so that the
+// compiler keeps it in an SGPR rather than looping over its distinct values
+__device__ __forceinline__ uint32_t buf_u32(brsrc_t r, uint32_t voff, uint32_t soff) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+      r, (int)voff, (int)__builtin_amdgcn_readfirstlane(soff), 0);
+}
+
+// Where one pass reads: the window's panel, codes and tran, the 4 parents'
+// row offsets and the sample's code column (bytes, per lane).
+struct PassSrc {
+  brsrc_t panel, codes, tran;
+  uint32_t row[4];
+  uint32_t col;     // 4 * sample
+  uint32_t stride;  // 4 * number of samples: bytes between a sample's code words
+};
 
 // The global inputs of chunk c: the 4 parents' allele words, the code
 // words and this lane's pair of tran floats (lanes 0-47 carry a chunk's 96).
-// Every load runs for every c: past the window's end the addresses are
-// clamped and the values are not used, except by reads of the LDS tables.
+// Every load runs for every c: past the window's end the panel and code
+// offsets are clamped and the tran floats read 0; those values are not
+// used, except by reads of the LDS tables.
 template <int BITS>
 struct ChunkIn {
   uint32_t h[4];
@@ -402,48 +430,40 @@ struct ChunkIn {
 
 template <int BITS>
 __device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsigned nch,
-                                           gptr_t<uint32_t> f0, gptr_t<uint32_t> f1,
-                                           gptr_t<uint32_t> m0, gptr_t<uint32_t> m1,
-                                           gptr_t<uint32_t> codes, size_t stride, unsigned nwords,
-                                           gptr_t<float> tran, unsigned ntr, unsigned lane) {
+                                           const PassSrc &src, unsigned nwords, unsigned lane) {
   const unsigned cc = c < nch ? c : nch - 1u;
-  in.tp = tran_pair(tran, kTranFloats * c + 2u * lane, ntr);
-  in.h[0] = f0[cc];
-  in.h[1] = f1[cc];
-  in.h[2] = m0[cc];
-  in.h[3] = m1[cc];
+  // floats 96c + 2 lane and the next (tran has 3M floats; when 3M is odd
+  // the last float stands alone and its partner reads 0)
+  const uint32_t to = 4u * (kTranFloats * c + 2u * lane);
+  in.tp = make_float2(__uint_as_float(buf_u32(src.tran, to, 0u)),
+                      __uint_as_float(buf_u32(src.tran, to + 4u, 0u)));
+#pragma unroll
+  for (int p = 0; p < 4; ++p) in.h[p] = buf_u32(src.panel, src.row[p], 4u * cc);
   // word offsets step by stride from the chunk's first word, clamped to the
   // window's last word (one add and one min per word)
-  const unsigned last = (nwords - 1u) * (unsigned)stride;
-  unsigned off = cc * BITS * (unsigned)stride;
+  const unsigned last = (nwords - 1u) * src.stride;
+  unsigned off = cc * BITS * src.stride;
 #pragma unroll
   for (int q = 0; q < BITS; ++q) {
-    in.cw[q] = codes[off < last ? off : last];
-    off += (unsigned)stride;
+    in.cw[q] = buf_u32(src.codes, src.col, off < last ? off : last);
+    off += src.stride;
   }
 }
 
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
-// pair m0, m1; rows as 32-bit words), computed by a lane pair; both lanes
-// return it.  Every lane of the wavefront must call it together (M and tran
+// pair m0, m1: src.row), computed by a lane pair; both lanes return it.
+// Every lane of the wavefront must call it together (M and tran
 // wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
 // LDS buffer wtr, during the chunk before it.
 template <int BITS, bool MIXED, bool FAST>
-__device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p, const uint32_t *__restrict__ f1p,
-                               const uint32_t *__restrict__ m0p, const uint32_t *__restrict__ m1p,
-                               const uint32_t *__restrict__ codesp, size_t stride,
-                               const float *__restrict__ tranp, unsigned M,
-                               const unsigned char *smem, unsigned lut_entries, float *wtr,
-                               float *ring, float norm) {
-  const auto f0 = gmem(f0p), f1 = gmem(f1p), m0 = gmem(m0p), m1 = gmem(m1p);
-  const auto codes = gmem(codesp);
-  const auto tran = gmem(tranp);
+__device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
+                                               const unsigned char *smem, unsigned lut_entries,
+                                               float *wtr, float *ring, float norm) {
   constexpr int CPW = 32 / BITS;
   const unsigned lane = threadIdx.x & 63u;
   // this lane's set of spread tables
   const uint2 *spread =
       reinterpret_cast<const uint2 *>(smem + 16u * lut_entries) + ((lane & 1u) ? 1024u : 0u);
-  const unsigned ntr = 3u * M;
   PairState st;
   // MIXED starts scaled by 2^kScaleT
   st.la = st.lb = MIXED ? 0x1p98f : 0.25f;
@@ -457,7 +477,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   // loop body is written out twice, so that the next chunk's loads land in
   // the other buffer while this one is used, with no register copies
   ChunkIn<BITS> X, Y;
-  load_chunk<BITS>(X, 0, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
+  load_chunk<BITS>(X, 0, nch, src, nwords, lane);
   // site 0's row becomes (1, 0, 0), so that from l = 1/4 (F32) or 2^98
   // (MIXED) the generic step gives l_xy = l * e_xy exactly
   if (lane == 0u) X.tp = make_float2(1.0f, 0.0f);
@@ -472,7 +492,7 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
     group_load_sp(B, X.h, 1, spread);
   }
   auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
-    load_chunk<BITS>(nxt, ch + 1u, nch, f0, f1, m0, m1, codes, stride, nwords, tran, ntr, lane);
+    load_chunk<BITS>(nxt, ch + 1u, nch, src, nwords, lane);
     // keep the loads here, a whole chunk ahead of their use (the scheduler
     // would otherwise sink the tran loads down to the LDS write that uses them)
     __builtin_amdgcn_sched_barrier(0);
@@ -504,9 +524,6 @@ __device__ __forceinline__ double forward_pass(const uint32_t *__restrict__ f0p,
   return (double)(st.fscore + cr_logf(last));
 }
 
-__device__ __forceinline__ const uint32_t *hap_row(const glp_window &W, unsigned h) {
-  return reinterpret_cast<const uint32_t *>(W.panel + (size_t)h * W.words_per_hap);
-}
 
 // LDS layout after the two tables: per wave, two slots of kTranSlot
 // floats of tran, then (F32 only) per wave a ring of kRingSlots x 64 floats.
@@ -521,19 +538,23 @@ __device__ __forceinline__ double score_set(const glp_window &W, const unsigned 
   float *wtr = tables_end + 2u * kTranSlot * wave;
   float *ring =
       tables_end + 2u * kTranSlot * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
-  const uint32_t *f0 = hap_row(W, P[0]), *f1 = hap_row(W, P[1]), *m0 = hap_row(W, P[2]),
-                 *m1 = hap_row(W, P[3]);
-  const double v = forward_pass<BITS, MIXED, true>(f0, f1, m0, m1, W.packed + sample, N, W.tran,
-                                                   W.num_sites, smem, lut_entries, wtr, ring,
-                                                   norm);
+  PassSrc src;
+  src.panel = buf_rsrc(W.panel, 0xFFFFFFFFu);
+  src.codes = buf_rsrc(W.packed, 0xFFFFFFFFu);
+  src.tran = buf_rsrc(W.tran, 12u * W.num_sites);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) src.row[q] = 8u * P[q] * W.words_per_hap;
+  src.col = 4u * sample;
+  src.stride = 4u * N;
+  const double v = forward_pass<BITS, MIXED, true>(src, W.num_sites, smem, lut_entries, wtr,
+                                                   ring, norm);
   // F32: a subnormal (or zero) site sum made the short reciprocal NaN: the
   // wavefront redoes its passes with the exact one (recip_normal is exact
   // for normal sums, so a value that is not NaN is already the exact one).
   // MIXED: a sum fell below 2^-99 between two scalings, or the last one is
   // not finite: the redo scales after every site.  Each lane keeps its first value unless it was NaN.
   if (__builtin_expect(__any(v != v), 0)) {
-    const double x = forward_pass<BITS, MIXED, false>(f0, f1, m0, m1, W.packed + sample, N,
-                                                      W.tran, W.num_sites, smem, lut_entries, wtr,
+    const double x = forward_pass<BITS, MIXED, false>(src, W.num_sites, smem, lut_entries, wtr,
                                                       ring, norm);
     return v != v ? x : v;
   }

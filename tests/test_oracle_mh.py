@@ -174,27 +174,31 @@ def test_kickstart_needs_reference_haplotypes():
     assert L.glp_plain_init(ctypes.byref(d), None) == 1
 
 
-def test_plan_rejects_code_arrays_past_32_bit_offsets():
-    # the kernels index a window's packed codes with 32-bit word offsets: a
-    # window of 2^20 sites x 2^14 samples at 8-bit codes holds 2^32 words,
-    # which the descriptor check refuses before any device work (the
-    # pointers are never read)
+def test_plan_rejects_arrays_past_32_bit_offsets():
+    # the kernels read a window's panel and packed codes through buffer
+    # descriptors with 32-bit byte offsets: 2^18 sites x 2^14 samples at
+    # 8-bit codes are 4 GiB of codes, and 2^19 haplotypes of 2^16 sites are
+    # 4 GiB of panel; the descriptor check refuses both before any device
+    # work (the pointers are never read)
     from glphase_amd import _lib
     L = _lib.lib()
     fake = ctypes.c_void_p(4096)
 
-    def desc(M, N):
+    def desc(M, N, H):
         w = _lib.GlpWindow(M, (M + 63) // 64, fake, fake, fake, fake, fake)
         d = _lib.GlpMhDesc()
         d.num_windows, d.windows = 1, ctypes.pointer(w)
-        d.num_samples, d.num_haps, d.bits_per_code = N, 2 * N, 8
+        d.num_samples, d.num_haps, d.bits_per_code = N, H, 8
         d.emit_lut, d.num_cycles, d.precision = fake, 4, 1
         return d, w
 
     out = ctypes.c_void_p()
-    d, w = desc(1 << 20, 1 << 14)  # w keeps the window alive
+    d, w = desc(1 << 18, 1 << 14, 1 << 15)  # w keeps the window alive
     assert L.glp_plan_create(ctypes.byref(d), 0, ctypes.byref(out)) == 1
-    assert b"2^32" in L.glp_last_error()
+    assert b"codes reach 4 GiB" in L.glp_last_error()
+    d, w = desc(1 << 16, 1 << 10, 1 << 19)
+    assert L.glp_plan_create(ctypes.byref(d), 0, ctypes.byref(out)) == 1
+    assert b"panel reaches 4 GiB" in L.glp_last_error()
 
 
 def test_kickstart_draws_from_reference_haplotypes(orc):
