@@ -33,6 +33,7 @@ using gmut_t = __attribute__((address_space(1))) T *;
 // compiler's vector types can)
 typedef float glp_f2v __attribute__((ext_vector_type(2)));
 typedef float glp_f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t glp_u2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 load_f2(gptr_t<float> p) {
   const glp_f2v v = *reinterpret_cast<gptr_t<glp_f2v>>(p);
   return make_float2(v.x, v.y);

@@ -66,37 +66,37 @@ __device__ __forceinline__ float cr_logf(float x) {
 // ---------------------------------------------------------------------------
 // LDS tables
 // ---------------------------------------------------------------------------
-// Emission table: the 16-byte entry at byte (code << 8) | (nib << 4) holds
-// (e00, e11, e01, e10) of a site with GL code `code` whose 4 parent alleles
-// are nib = f0 | f1 << 1 | m0 << 2 | m1 << 3, e_xy = FillEmit(code)[(f_x << 1)
-// | m_y] (hmmLike.cu:39-44,124-127).  The even lane of a chain's lane pair
-// reads the first 8 bytes, the odd lane the last 8.
+// LDS layout: the spread tables at address 0, the emission table after
+// them, then per wave the tran slots (and, F32 only, the log rings).
 // Spread table: for parent p and an allele byte b of 8 consecutive sites,
-// entry p*256+b puts bit i of b at bit 4+p of byte i (sites 0-3 in .x, 4-7
-// in .y), so OR-ing the 4 parents' entries gives one (nib << 4) byte per site.
-constexpr unsigned kSpreadBytes = 2 * 4 * 256 * 8;  // spread tables: even lanes', odd lanes'
+// the 8-byte entry at byte 2048p + 8b puts bit i of b at bit 4+p of byte i
+// (sites 0-3 in .x, 4-7 in .y), so OR-ing the 4 parents' entries gives one
+// (nib << 4) byte per site.  Its address is the byte shifted by 3 (one
+// instruction that selects the byte and shifts it) with 2048p in the read's
+// offset field.
+// Emission table: the 16-byte entry at byte kSpreadBytes + (code << 8) |
+// (nib << 4) holds (e00, e11, e01, e10) of a site with GL code `code` whose
+// 4 parent alleles are nib = f0 | f1 << 1 | m0 << 2 | m1 << 3, e_xy =
+// FillEmit(code)[(f_x << 1) | m_y] (hmmLike.cu:39-44,124-127).  The even
+// lane of a chain's lane pair reads the first 8 bytes, the odd lane the last
+// 8: it ORs 8 into every site's byte (its mask oddm).
+constexpr unsigned kSpreadBytes = 4 * 256 * 8;
 constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk
 constexpr unsigned kTranSlot = 128;    // a wave's LDS slot for it (a float pair per lane; a wave has two)
 constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
 
 __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
                                          unsigned lut_entries) {
-  float4 *lut = reinterpret_cast<float4 *>(smem);
+  float4 *lut = reinterpret_cast<float4 *>(smem + kSpreadBytes);
   for (unsigned q = threadIdx.x; q < lut_entries; q += blockDim.x) lut[q] = g_lut[q];
-  uint2 *spread = reinterpret_cast<uint2 *>(smem + 16u * lut_entries);
-  // two sets of 4 tables (one per parent), the odd lanes' set with the
-  // odd-lane bit 8 set in every byte of parent 0's entries
-  for (unsigned q = threadIdx.x; q < 2048u; q += blockDim.x) {
-    const unsigned p = (q >> 8) & 3u, b = q & 255u;
+  uint2 *spread = reinterpret_cast<uint2 *>(smem);
+  for (unsigned q = threadIdx.x; q < 1024u; q += blockDim.x) {
+    const unsigned p = q >> 8, b = q & 255u;
     uint32_t lo = 0, hi = 0;
 #pragma unroll
     for (unsigned i = 0; i < 4; ++i) {
       lo |= ((b >> i) & 1u) << (8 * i + 4 + p);
       hi |= ((b >> (i + 4)) & 1u) << (8 * i + 4 + p);
-    }
-    if (q >= 1024u && p == 0u) {
-      lo |= 0x08080808u;
-      hi |= 0x08080808u;
     }
     spread[q] = make_uint2(lo, hi);
   }
@@ -222,16 +222,28 @@ __device__ __forceinline__ void flush_ring(PairState &st, const float *ring) {
   st.nq = 0;
 }
 
-// The emission half-entry at byte a of the kernel's dynamic LDS.  The kernels
-// of this file declare no static LDS (checked on the host before each
-// launch), so their dynamic LDS starts at LDS address 0 and a is the address
-// itself: the read takes it as is, with no base to add.
+// The kernels of this file declare no static LDS (checked on the host before
+// each launch), so their dynamic LDS starts at LDS address 0 and a table
+// offset is an LDS address: a read takes it as is, with constant parts in
+// its offset field.
+__device__ __forceinline__ uint2 lds_u2(uint32_t a) {
+  const glp_u2v v = *reinterpret_cast<const __attribute__((address_space(3))) glp_u2v *>((uintptr_t)a);
+  return make_uint2(v.x, v.y);
+}
+
+// The emission half-entry at byte a of the emission table.
 __device__ __forceinline__ float2 lut_entry(uint32_t a) {
-  const glp_f2v v = *reinterpret_cast<const __attribute__((address_space(3))) glp_f2v *>((uintptr_t)a);
+  const glp_f2v v = *reinterpret_cast<const __attribute__((address_space(3))) glp_f2v *>(
+      (uintptr_t)(a + kSpreadBytes));
   return make_float2(v.x, v.y);
 }
 
-// Byte address of site s's emission half-entry: byte 0 = (nib << 4) | 8 *
+// Parent p's spread entry for allele byte g of word h.
+__device__ __forceinline__ uint2 spread_entry(uint32_t h, int g, int p) {
+  return lds_u2((((h >> (8 * g)) & 255u) << 3) + 2048u * p);
+}
+
+// Byte offset of site s's emission half-entry: byte 0 = (nib << 4) | 8 *
 // (odd lane) from the spread word, byte 1 = the site's code.
 template <int BITS>
 __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned s) {
@@ -255,16 +267,13 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
 template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
-                                          const unsigned char *smem, const uint2 *spread,
-                                          float norm, float nrm0, float *ring) {
+                                          uint32_t oddm, float norm, float nrm0, float *ring) {
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     if (TAIL && (unsigned)(8 * g) >= n) break;
-    const uint2 s0 = spread[0 * 256 + ((h[0] >> (8 * g)) & 255u)];
-    const uint2 s1 = spread[1 * 256 + ((h[1] >> (8 * g)) & 255u)];
-    const uint2 s2 = spread[2 * 256 + ((h[2] >> (8 * g)) & 255u)];
-    const uint2 s3 = spread[3 * 256 + ((h[3] >> (8 * g)) & 255u)];
-    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x, s0.y | s1.y | s2.y | s3.y};
+    const uint2 s0 = spread_entry(h[0], g, 0), s1 = spread_entry(h[1], g, 1);
+    const uint2 s2 = spread_entry(h[2], g, 2), s3 = spread_entry(h[3], g, 3);
+    const uint32_t nb[2] = {s0.x | s1.x | s2.x | s3.x | oddm, s0.y | s1.y | s2.y | s3.y | oddm};
     constexpr int CPW = 32 / BITS;
     // emissions and transitions of the group, loaded ahead of the recurrence
     float2 e[8];
@@ -300,22 +309,21 @@ struct GroupLds {
 };
 
 // Group g's spread entries, from the chunk's allele words h.
-__device__ __forceinline__ void group_load_sp(GroupLds &G, const uint32_t (&h)[4], int g,
-                                              const uint2 *spread) {
+__device__ __forceinline__ void group_load_sp(GroupLds &G, const uint32_t (&h)[4], int g) {
 #pragma unroll
-  for (int p = 0; p < 4; ++p) G.sp[p] = spread[p * 256 + ((h[p] >> (8 * g)) & 255u)];
+  for (int p = 0; p < 4; ++p) G.sp[p] = spread_entry(h[p], g, p);
 }
 
 // Group g's tran rows (from the chunk's LDS slot tr) and emissions, whose
 // addresses need its spread entries.
 template <int BITS>
 __device__ __forceinline__ void group_load_et(GroupLds &G, const uint32_t (&cw)[BITS],
-                                              const float *tr, int g) {
+                                              const float *tr, int g, uint32_t oddm) {
   constexpr int CPW = 32 / BITS;
 #pragma unroll
   for (int q = 0; q < 6; ++q) G.t4[q] = reinterpret_cast<const float4 *>(tr)[6 * g + q];
-  const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x,
-                          G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y};
+  const uint32_t nb[2] = {G.sp[0].x | G.sp[1].x | G.sp[2].x | G.sp[3].x | oddm,
+                          G.sp[0].y | G.sp[1].y | G.sp[2].y | G.sp[3].y | oddm};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int s = 8 * g + k;
@@ -351,30 +359,30 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
                                                const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
                                                const float *tr, const uint32_t (&hn)[4],
                                                const uint32_t (&cn)[BITS], float *trn, float2 tpn,
-                                               const uint2 *spread, float norm,
-                                               float nrm0, float *ring) {
+                                               uint32_t oddm, float norm, float nrm0,
+                                               float *ring) {
   const unsigned lane = threadIdx.x & 63u;
-  group_load_et<BITS>(B, cw, tr, 1);
+  group_load_et<BITS>(B, cw, tr, 1, oddm);
   group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
-  group_load_sp(A, h, 2, spread);
+  group_load_sp(A, h, 2);
   group_steps<MIXED, FAST>(st, A, 0, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(A, cw, tr, 2);
+  group_load_et<BITS>(A, cw, tr, 2, oddm);
   group_steps<MIXED, FAST>(st, B, 1, 0, norm, nrm0, ring);
-  group_load_sp(B, h, 3, spread);
+  group_load_sp(B, h, 3);
   group_steps<MIXED, FAST>(st, B, 1, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(B, cw, tr, 3);
+  group_load_et<BITS>(B, cw, tr, 3, oddm);
   group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
   // every lane writes (lanes 48-63 into the slot's unread tail), so that the
   // write, and with it the tran load, is not under a branch
   reinterpret_cast<float2 *>(trn)[lane] = tpn;
-  group_load_sp(A, hn, 0, spread);
+  group_load_sp(A, hn, 0);
   group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
 
-  group_load_et<BITS>(A, cn, trn, 0);
+  group_load_et<BITS>(A, cn, trn, 0, oddm);
   group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
-  group_load_sp(B, hn, 1, spread);
+  group_load_sp(B, hn, 1);
   group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
@@ -461,9 +469,8 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
                                                float *wtr, float *ring, float norm) {
   constexpr int CPW = 32 / BITS;
   const unsigned lane = threadIdx.x & 63u;
-  // this lane's set of spread tables
-  const uint2 *spread =
-      reinterpret_cast<const uint2 *>(smem + 16u * lut_entries) + ((lane & 1u) ? 1024u : 0u);
+  // the odd lane reads the odd half of every emission entry
+  const uint32_t oddm = (lane & 1u) ? 0x08080808u : 0u;
   PairState st;
   // MIXED starts scaled by 2^kScaleT
   st.la = st.lb = MIXED ? 0x1p98f : 0.25f;
@@ -487,9 +494,9 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
   reinterpret_cast<float2 *>(wtr)[lane] = X.tp;
   GroupLds A, B;
   if (M >= 32u) {
-    group_load_sp(A, X.h, 0, spread);
-    group_load_et<BITS>(A, X.cw, wtr, 0);
-    group_load_sp(B, X.h, 1, spread);
+    group_load_sp(A, X.h, 0);
+    group_load_et<BITS>(A, X.cw, wtr, 0, oddm);
+    group_load_sp(B, X.h, 1);
   }
   auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
     load_chunk<BITS>(nxt, ch + 1u, nch, src, nwords, lane);
@@ -502,10 +509,9 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
     float *tr = wtr + kTranSlot * (ch & 1u), *trn = wtr + kTranSlot * ((ch + 1u) & 1u);
     if (n == 32)
       fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, cur.h, cur.cw, tr, nxt.h, nxt.cw, trn, nxt.tp,
-                                        spread, norm, nrm0, ring);
+                                        oddm, norm, nrm0, ring);
     else  // the window's last chunk, shorter than 32 sites
-      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, smem, spread, norm,
-                                         nrm0, ring);
+      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, oddm, norm, nrm0, ring);
     if (!MIXED) flush_ring(st, ring);
   };
   for (unsigned ch = 0; ch < nch; ch += 2u) {
