@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libglphase_amd.so")
 
 GLP_OK = 0
 PROP_UNIFORM, PROP_KLIST, PROP_HOST = 0, 1, 2
-PREC_F32, PREC_MIXED = 0, 1
+PREC_F32, PREC_MIXED, PREC_MIXED_FB = 0, 1, 2
 GLP_NOALLOC_MAX_WINDOWS = 1024  # include/glphase_amd.h
 
 c_u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -107,13 +107,29 @@ struct glp_plan {
   std::vector<glp_window> wins_h;
   std::vector<unsigned> order, q;  // windows longest first; passes per chain round
   DevBuf wins, order_d, qtab, wctr, ctrs;
+  // forward-backward form: every window's layout (fbdata) and the mirror
+  // kernel's jobs (fbjobs), rebuilt from the windows before every run
+  DevBuf fbdata, fbjobs;
+  unsigned fb_items_max = 0;
   // HIP graph of one generation: kind 1 = batched (reset + k_batched), kind 2
-  // = plain (C + 1 launches); node arguments are rewritten for each run
+  // = plain (C + 1 launches), each after the layout build in the
+  // forward-backward form; node arguments are rewritten for each run
   int graph_kind = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipGraphNode_t> nodes;
+  size_t node0 = 0;  // index of the first rewritten node (k_batched, or the plain step 0)
 };
+
+static bool plan_fb(const glp_plan *p) { return p->a.precision == GLP_PREC_MIXED_FB; }
+
+// Builds every window's forward-backward layout on stream s (no-op in the
+// other forms).
+static int plan_mirror(glp_plan *p, hipStream_t s) {
+  if (!plan_fb(p)) return GLP_OK;
+  return launch_fb_mirror(p->fbjobs.as<FbWin>(), p->nwin, p->fb_items_max, p->a.N, p->a.H,
+                          p->a.bits, s);
+}
 
 static void plan_graph_reset(glp_plan *p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -124,30 +140,33 @@ static void plan_graph_reset(glp_plan *p) {
   p->graph_kind = 0;
 }
 
-// Passes per chain round (lane pairs per chain) when none is given: the
+// Passes per chain round (pass units per chain) when none is given: the
 // deepest speculation whose wavefronts still fit one per SIMD.  A chain's
 // time is its rounds times one pass, and rounds fall with depth while the
 // SIMDs are not shared; past one wavefront per SIMD two wavefronts split the
 // issue slots and every pass slows down.
-// a wavefront holds 32 lane pairs, floor(32 / Q) chain slots
+// A wavefront holds units(precision) pass units (32 lane pairs, or 16 lane
+// quads in the forward-backward form), floor(units / Q) chain slots.
 constexpr unsigned kMaxPasses = 32;
 constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 
-static double waves_for(unsigned nchains, unsigned Q) {
-  return std::ceil((double)nchains / (double)(32u / Q));
+static unsigned units(int precision) { return precision == GLP_PREC_MIXED_FB ? 16u : 32u; }
+
+static double waves_for(unsigned nchains, unsigned Q, unsigned nu) {
+  return std::ceil((double)nchains / (double)(nu / Q));
 }
 
-static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu) {
+static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu, unsigned nu) {
   const double target = (double)num_cu * 4.0;  // one wave per SIMD
   unsigned best = 1;
-  for (unsigned Q = 1; Q <= kMaxPasses && Q <= C + 1; ++Q)
-    if (waves_for(nchains, Q) <= target) best = Q;
+  for (unsigned Q = 1; Q <= nu && Q <= C + 1; ++Q)
+    if (waves_for(nchains, Q, nu) <= target) best = Q;
   return best;
 }
 
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
   double waves = 0;
-  for (unsigned w : wins) waves += waves_for(p->a.N, p->q[w]);
+  for (unsigned w : wins) waves += waves_for(p->a.N, p->q[w], units(p->a.precision));
   const unsigned blocks = (unsigned)std::ceil(waves / (double)p->wpb);
   return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
 }
@@ -186,8 +205,9 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   if (d->kickstart_from_ref && d->prop_mode != GLP_PROP_HOST &&
       d->num_haps <= 2 * d->num_samples)
     return set_error(GLP_ERR_ARG, "kickstart_from_ref needs reference haplotypes beyond 2N");
-  if (d->precision < 0 || d->precision > 1)
-    return set_error(GLP_ERR_ARG, "precision must be GLP_PREC_F32 or GLP_PREC_MIXED");
+  if (d->precision < 0 || d->precision > 2)
+    return set_error(GLP_ERR_ARG,
+                     "precision must be GLP_PREC_F32, GLP_PREC_MIXED or GLP_PREC_MIXED_FB");
   if (d->k0 + (uint64_t)d->num_cycles >= 0x80000000ull)
     return set_error(GLP_ERR_ARG, "proposal counters must stay below 2^31");
   for (unsigned w = 0; w < d->num_windows; ++w) {
@@ -204,6 +224,8 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
       return set_error(GLP_ERR_ARG, "a window's packed codes reach 4 GiB");
     if ((uint64_t)d->num_haps * W.words_per_hap * 8 >= (1ull << 32))
       return set_error(GLP_ERR_ARG, "a window's panel reaches 4 GiB");
+    if (d->precision == GLP_PREC_MIXED_FB)
+      GLP_TRY(fb_check(W.num_sites, d->bits_per_code, d->num_samples, d->num_haps));
   }
   a.N = d->num_samples;
   a.H = d->num_haps;
@@ -232,6 +254,9 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
 static int plain_launch(const glp_mh_desc *desc, unsigned step, void *stream) {
   KArgs a;
   GLP_TRY(fill_args(desc, a));
+  if (a.precision == GLP_PREC_MIXED_FB)
+    return set_error(GLP_ERR_ARG, "the forward-backward form runs through a plan "
+                                  "(glp_plan_run_plain): its window layouts are allocated");
   hipStream_t s = static_cast<hipStream_t>(stream);
   glp_window *d_wins = nullptr;
   const size_t bytes = sizeof(glp_window) * desc->num_windows;
@@ -258,7 +283,7 @@ int glp_build_emit_lut(const float *codebook, unsigned n_codes, const float *pc1
                        int precision, void *lut_dev) {
   if (!codebook || !pc16 || !lut_dev) return set_error(GLP_ERR_ARG, "null argument");
   if (n_codes != 16 && n_codes != 256) return set_error(GLP_ERR_ARG, "n_codes must be 16 or 256");
-  if (precision < 0 || precision > 1) return set_error(GLP_ERR_ARG, "bad precision");
+  if (precision < 0 || precision > 2) return set_error(GLP_ERR_ARG, "bad precision");
   std::vector<float> E(4 * (size_t)n_codes), T(64 * (size_t)n_codes);
   emit_table(codebook, n_codes, pc16, E.data());
   nibble_table(E.data(), n_codes, T.data());
@@ -280,8 +305,10 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   *out = nullptr;
   KArgs a;
   GLP_TRY(fill_args(desc, a));
-  if (lanes_per_chain > kMaxPasses)
-    return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 (automatic) or 1..32");
+  const unsigned nu = units(a.precision);
+  if (lanes_per_chain > nu)
+    return set_error(GLP_ERR_ARG, nu == 16u ? "lanes_per_chain must be 0 (automatic) or 1..16"
+                                            : "lanes_per_chain must be 0 (automatic) or 1..32");
   glp_plan *p = new glp_plan();
   p->a = a;
   int rc = hip_check(hipGetDevice(&p->dev), "hipGetDevice");
@@ -293,7 +320,7 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   for (int k = 0; k < 2 && rc == GLP_OK; ++k) {
     const unsigned w = k ? 8u : 4u;
     if (lds_bytes(a, w) <= kLdsPerCu)
-      rc = batched_occupancy(a.bits, a.precision != GLP_PREC_F32, w, lds_bytes(a, w), &p->bpc_of[k]);
+      rc = batched_occupancy(a.bits, a.precision, w, lds_bytes(a, w), &p->bpc_of[k]);
   }
   if (rc != GLP_OK) {
     delete p;
@@ -312,7 +339,8 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   std::stable_sort(p->order.begin(), p->order.end(), [&](unsigned x, unsigned y) {
     return desc->windows[x].num_sites > desc->windows[y].num_sites;
   });
-  const unsigned Q = lanes_per_chain ? lanes_per_chain : auto_lanes(a.N * p->nwin, a.C, p->num_cu);
+  const unsigned Q =
+      lanes_per_chain ? lanes_per_chain : auto_lanes(a.N * p->nwin, a.C, p->num_cu, nu);
   p->q.assign(p->nwin, Q);
   p->grid = plan_grid(p, p->order);
   rc = p->wins.ensure(sizeof(glp_window) * p->nwin);
@@ -320,8 +348,56 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   if (rc == GLP_OK) rc = p->qtab.ensure(4 * p->nwin);
   if (rc == GLP_OK) rc = p->wctr.ensure(4 * p->nwin);
   if (rc == GLP_OK) rc = p->ctrs.ensure(4 * sizeof(unsigned long long));
+  // the kernels read the windows through this table: in the forward-backward
+  // form its panel, codes and tran point at the windows' layouts
+  std::vector<glp_window> ktab(desc->windows, desc->windows + p->nwin);
+  if (rc == GLP_OK && plan_fb(p)) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    std::vector<size_t> off(p->nwin);
+    size_t total = 0;
+    // per window: panel, codes and tran parts, each 256-byte aligned
+    auto parts = [&](unsigned M, size_t *pb, size_t *cb) {
+      const unsigned n = (M + 1) / 2, wh = (n + 63) / 64, nw = (n * a.bits + 31) / 32;
+      *pb = up(16 * (size_t)a.H * wh);
+      *cb = up(8 * (size_t)nw * a.N);
+      return *pb + *cb + up(fb_layout_bytes(M, a.bits, a.N, a.H) - 16 * (size_t)a.H * wh -
+                            8 * (size_t)nw * a.N);
+    };
+    for (unsigned w = 0; w < p->nwin; ++w) {
+      size_t pb = 0, cb = 0;
+      off[w] = total;
+      total += parts(ktab[w].num_sites, &pb, &cb);
+      p->fb_items_max = std::max(p->fb_items_max, fb_items(ktab[w].num_sites, a.bits, a.N, a.H));
+    }
+    rc = p->fbdata.ensure(total);
+    if (rc == GLP_OK) rc = p->fbjobs.ensure(sizeof(FbWin) * p->nwin);
+    std::vector<FbWin> jobs(p->nwin);
+    for (unsigned w = 0; w < p->nwin && rc == GLP_OK; ++w) {
+      const glp_window &W = desc->windows[w];
+      const unsigned n = (W.num_sites + 1) / 2, wh = (n + 63) / 64;
+      size_t pb = 0, cb = 0;
+      (void)parts(W.num_sites, &pb, &cb);
+      char *base = p->fbdata.as<char>() + off[w];
+      FbWin &j = jobs[w];
+      j.panel = W.panel;
+      j.packed = W.packed;
+      j.tran = W.tran;
+      j.cpanel = reinterpret_cast<uint64_t *>(base);
+      j.ccodes = reinterpret_cast<uint32_t *>(base + pb);
+      j.ctran = reinterpret_cast<float *>(base + pb + cb);
+      j.M = W.num_sites;
+      j.wn = W.words_per_hap;
+      ktab[w].panel = j.cpanel;
+      ktab[w].packed = j.ccodes;
+      ktab[w].tran = j.ctran;
+      ktab[w].words_per_hap = wh;
+    }
+    if (rc == GLP_OK)
+      rc = hip_check(hipMemcpy(p->fbjobs.p, jobs.data(), sizeof(FbWin) * p->nwin,
+                               hipMemcpyHostToDevice), "layout job copy");
+  }
   if (rc == GLP_OK)
-    rc = hip_check(hipMemcpy(p->wins.p, desc->windows, sizeof(glp_window) * p->nwin,
+    rc = hip_check(hipMemcpy(p->wins.p, ktab.data(), sizeof(glp_window) * p->nwin,
                              hipMemcpyHostToDevice), "window table copy");
   if (rc == GLP_OK)
     rc = hip_check(hipMemcpy(p->order_d.p, p->order.data(), 4 * p->nwin, hipMemcpyHostToDevice),
@@ -343,6 +419,9 @@ int glp_mh_run_noalloc(const glp_mh_desc *desc, unsigned lanes_per_chain, void *
   GLP_TRY(fill_args(desc, a));
   if (lanes_per_chain > kMaxPasses)
     return set_error(GLP_ERR_ARG, "lanes_per_chain must be 0 (automatic) or 1..32");
+  if (a.precision == GLP_PREC_MIXED_FB)
+    return set_error(GLP_ERR_ARG, "glp_mh_run_noalloc: the forward-backward form needs a plan "
+                                  "(its window layouts are allocated)");
   int dev = 0, num_cu = 1;
   GLP_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
   GLP_TRY(hip_check(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev),
@@ -350,7 +429,7 @@ int glp_mh_run_noalloc(const glp_mh_desc *desc, unsigned lanes_per_chain, void *
   if (lds_bytes(a, 4) > kLdsPerCu)
     return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
   const unsigned Q = lanes_per_chain ? lanes_per_chain
-                                     : auto_lanes(a.N * desc->num_windows, a.C, num_cu);
+                                     : auto_lanes(a.N * desc->num_windows, a.C, num_cu, 32u);
   return run_noalloc(a, desc->windows, desc->num_windows, Q, num_cu,
                      static_cast<hipStream_t>(stream), stats_out);
 }
@@ -361,6 +440,7 @@ int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stre
   a.gen = gen;
   a.pen = pen;
   a.seed = seed;
+  GLP_TRY(plan_mirror(p, static_cast<hipStream_t>(stream)));
   return launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
                         p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
                         p->ctrs.as<unsigned long long>(), p->grid, p->wpb,
@@ -369,9 +449,12 @@ int glp_plan_run(glp_plan *p, unsigned gen, float pen, uint64_t seed, void *stre
 
 int glp_plan_set_lanes(glp_plan *p, const unsigned *lanes_per_window) {
   if (!p || !lanes_per_window) return set_error(GLP_ERR_ARG, "null argument");
+  const unsigned nu = units(p->a.precision);
   for (unsigned w = 0; w < p->nwin; ++w) {
     const unsigned Q = lanes_per_window[w];
-    if (Q == 0 || Q > kMaxPasses) return set_error(GLP_ERR_ARG, "lanes per chain must be 1..32");
+    if (Q == 0 || Q > nu)
+      return set_error(GLP_ERR_ARG, nu == 16u ? "lanes per chain must be 1..16"
+                                              : "lanes per chain must be 1..32");
   }
   p->q.assign(lanes_per_window, lanes_per_window + p->nwin);
   plan_graph_reset(p);
@@ -407,6 +490,8 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
     return GLP_OK;
   };
   GLP_TRY(copy_all(true));
+  // the windows' inputs do not change during the trials: one layout build
+  GLP_TRY(plan_mirror(p, s));
   hipEvent_t e0, e1;
   GLP_TRY(hip_check(hipEventCreate(&e0), "event"));
   GLP_TRY(hip_check(hipEventCreate(&e1), "event"));
@@ -454,7 +539,7 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
     if (rc != GLP_OK || p->bpc_of[w == 8 ? 1 : 0] < 1) continue;
     rc = plan_use_waves(p, w);
     for (unsigned Q : kTrials) {
-      if (rc != GLP_OK || Q > a.C + 1) break;
+      if (rc != GLP_OK || Q > a.C + 1 || Q > units(a.precision)) break;
       std::fill(p->q.begin(), p->q.end(), Q);
       float ms = 0;
       rc = trial(&ms);
@@ -469,7 +554,7 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
   if (buckets.size() > 1) {
     for (auto &b : buckets) {
       for (unsigned Q : kTrials) {
-        if (rc != GLP_OK || Q > a.C + 1) break;
+        if (rc != GLP_OK || Q > a.C + 1 || Q > units(a.precision)) break;
         if (Q == best_q[b[0]]) continue;
         p->q = best_q;
         for (unsigned w : b) p->q[w] = Q;
@@ -511,6 +596,7 @@ int glp_plan_run_plain(glp_plan *p, unsigned gen, float pen, uint64_t seed, void
   a.pen = pen;
   a.seed = seed;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  GLP_TRY(plan_mirror(p, s));
   for (unsigned step = 0; step <= a.C; ++step)
     GLP_TRY(launch_plain(a, p->wins.as<glp_window>(), p->nwin, step, s));
   return GLP_OK;
@@ -568,39 +654,57 @@ hipKernelNodeParams node_params(const void *fn, dim3 grid, unsigned shmem, void 
 static int plan_graph_build(glp_plan *p, int kind) {
   plan_graph_reset(p);
   const KArgs &a = p->a;
-  const bool mixed = a.precision != GLP_PREC_F32;
   const unsigned shmem = (unsigned)lds_bytes(a, kind == 1 ? p->wpb : 4u);
   GLP_TRY(hip_check(hipGraphCreate(&p->graph, 0), "hipGraphCreate"));
   int rc = GLP_OK;
+  hipGraphNode_t prev = nullptr;
+  auto add = [&](const hipKernelNodeParams &k) {
+    hipGraphNode_t n = nullptr;
+    if (rc == GLP_OK)
+      rc = hip_check(hipGraphAddKernelNode(&n, p->graph, prev ? &prev : nullptr, prev ? 1 : 0, &k),
+                     "graph node");
+    if (rc == GLP_OK) {
+      p->nodes.push_back(n);
+      prev = n;
+    }
+  };
+  if (plan_fb(p)) {
+    // the windows' forward-backward layouts, from the windows' current data
+    const FbWin *jobs = p->fbjobs.as<FbWin>();
+    unsigned N = a.N, H = a.H, bits = a.bits;
+    void *margs[] = {&jobs, &N, &H, &bits};
+    unsigned gx = std::max(1u, std::min((p->fb_items_max + 255u) / 256u,
+                                        std::max(1u, 4096u / std::max(1u, p->nwin))));
+    hipKernelNodeParams mk{};
+    mk.func = const_cast<void *>(fb_mirror_kernel_fn());
+    mk.gridDim = dim3(gx, p->nwin);
+    mk.blockDim = dim3(256);
+    mk.sharedMemBytes = 0;
+    mk.kernelParams = margs;
+    mk.extra = nullptr;
+    add(mk);
+  }
   if (kind == 1) {
-    const void *fn = batched_kernel_fn(a.bits, mixed, p->wpb);
-    rc = prepare_kernel_lds(fn, shmem);
+    const void *fn = batched_kernel_fn(a.bits, a.precision, p->wpb);
+    if (rc == GLP_OK) rc = prepare_kernel_lds(fn, shmem);
     unsigned long long *ctrs = p->ctrs.as<unsigned long long>();
     unsigned *wctr = p->wctr.as<unsigned>();
     unsigned nwin = p->nwin;
     void *rargs[] = {&ctrs, &wctr, &nwin};
-    hipKernelNodeParams rk = node_params(reset_kernel_fn(), dim3(1), 0, rargs);
-    hipGraphNode_t n0 = nullptr, n1 = nullptr;
-    if (rc == GLP_OK) rc = hip_check(hipGraphAddKernelNode(&n0, p->graph, nullptr, 0, &rk), "graph node");
+    add(node_params(reset_kernel_fn(), dim3(1), 0, rargs));
     BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
                    p->nwin, wctr, ctrs, a, {}};
     ba.bind();
-    hipKernelNodeParams bk = node_params(fn, dim3(p->grid), shmem, ba.ptrs, 64 * p->wpb);
-    if (rc == GLP_OK) rc = hip_check(hipGraphAddKernelNode(&n1, p->graph, &n0, 1, &bk), "graph node");
-    if (rc == GLP_OK) p->nodes = {n0, n1};
+    p->node0 = p->nodes.size();
+    add(node_params(fn, dim3(p->grid), shmem, ba.ptrs, 64 * p->wpb));
   } else {
-    const void *fn = plain_kernel_fn(a.bits, mixed);
-    rc = prepare_kernel_lds(fn, shmem);
-    hipGraphNode_t prev = nullptr;
+    const void *fn = plain_kernel_fn(a.bits, a.precision);
+    if (rc == GLP_OK) rc = prepare_kernel_lds(fn, shmem);
+    p->node0 = p->nodes.size();
     for (unsigned step = 0; step <= a.C && rc == GLP_OK; ++step) {
       PlainArgs pa{p->wins.as<glp_window>(), a, step, {}};
       pa.bind();
-      hipKernelNodeParams k = node_params(fn, dim3((2 * a.N + 255) / 256, p->nwin), shmem, pa.ptrs);
-      hipGraphNode_t n = nullptr;
-      rc = hip_check(hipGraphAddKernelNode(&n, p->graph, prev ? &prev : nullptr, prev ? 1 : 0, &k),
-                     "graph node");
-      p->nodes.push_back(n);
-      prev = n;
+      add(node_params(fn, dim3((plain_lanes(a) + 255) / 256, p->nwin), shmem, pa.ptrs));
     }
   }
   if (rc == GLP_OK) rc = hip_check(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0),
@@ -621,22 +725,24 @@ static int plan_graph_run(glp_plan *p, int kind, unsigned gen, float pen, uint64
   a.gen = gen;
   a.pen = pen;
   a.seed = seed;
-  const bool mixed = a.precision != GLP_PREC_F32;
   const unsigned shmem = (unsigned)lds_bytes(a, kind == 1 ? p->wpb : 4u);
   if (kind == 1) {
     BatchedArgs ba{p->wins.as<glp_window>(), p->order_d.as<unsigned>(), p->qtab.as<unsigned>(),
                    p->nwin, p->wctr.as<unsigned>(), p->ctrs.as<unsigned long long>(), a, {}};
     ba.bind();
-    hipKernelNodeParams bk = node_params(batched_kernel_fn(a.bits, mixed, p->wpb), dim3(p->grid),
-                                         shmem, ba.ptrs, 64 * p->wpb);
-    GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[1], &bk), "graph update"));
+    hipKernelNodeParams bk = node_params(batched_kernel_fn(a.bits, a.precision, p->wpb),
+                                         dim3(p->grid), shmem, ba.ptrs, 64 * p->wpb);
+    GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[p->node0], &bk),
+                      "graph update"));
   } else {
-    const void *fn = plain_kernel_fn(a.bits, mixed);
+    const void *fn = plain_kernel_fn(a.bits, a.precision);
     for (unsigned step = 0; step <= a.C; ++step) {
       PlainArgs pa{p->wins.as<glp_window>(), a, step, {}};
       pa.bind();
-      hipKernelNodeParams k = node_params(fn, dim3((2 * a.N + 255) / 256, p->nwin), shmem, pa.ptrs);
-      GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[step], &k), "graph update"));
+      hipKernelNodeParams k =
+          node_params(fn, dim3((plain_lanes(a) + 255) / 256, p->nwin), shmem, pa.ptrs);
+      GLP_TRY(hip_check(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[p->node0 + step], &k),
+                        "graph update"));
     }
   }
   return hip_check(hipGraphLaunch(p->exec, s), "hipGraphLaunch");
@@ -677,7 +783,8 @@ int glp_plan_lanes(glp_plan *p, unsigned *lanes, unsigned *grid) {
 int glp_plan_destroy(glp_plan *p) {
   if (!p) return GLP_OK;
   plan_graph_reset(p);
-  for (DevBuf *b : {&p->wins, &p->order_d, &p->qtab, &p->wctr, &p->ctrs}) b->release();
+  for (DevBuf *b : {&p->wins, &p->order_d, &p->qtab, &p->wctr, &p->ctrs, &p->fbdata, &p->fbjobs})
+    b->release();
   delete p;
   return GLP_OK;
 }

@@ -153,6 +153,7 @@ enum PropMode : int {
 enum Precision : int {
   kPrecF32 = 0,    // FP32 recurrence and FP32 log accumulation (reference)
   kPrecMixed = 1,  // FP32 recurrence, FP64 log accumulation and MH ratio
+  kPrecMixedFB = 2,  // the same arithmetic over two half-window passes joined in FP64
 };
 
 }  // namespace glp

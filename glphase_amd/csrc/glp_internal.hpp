@@ -46,11 +46,36 @@ int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_o
                    hipStream_t stream);
 // Kernel entry points for HIP graph nodes, and the dynamic-LDS attribute they
 // need before any launch.
-const void *plain_kernel_fn(unsigned bits, bool mixed);
-const void *batched_kernel_fn(unsigned bits, bool mixed, unsigned waves);
+const void *plain_kernel_fn(unsigned bits, int precision);
+const void *batched_kernel_fn(unsigned bits, int precision, unsigned waves);
 const void *reset_kernel_fn();
 int prepare_kernel_lds(const void *fn, size_t bytes);
-int batched_occupancy(unsigned bits, bool mixed, unsigned waves, size_t smem, int *blocks_per_cu);
+int batched_occupancy(unsigned bits, int precision, unsigned waves, size_t smem,
+                      int *blocks_per_cu);
+// lanes of the plain path's grid for one window (2 or 4 per chain)
+unsigned plain_lanes(const KArgs &a);
+
+// The forward-backward form (GLP_PREC_MIXED_FB) reads each window through a
+// layout the library builds before every run (k_fb_mirror): the forward
+// half's data and the backward half's mirrored copy.  One job per window:
+// the window's own arrays and where its layout goes.
+struct FbWin {
+  const uint64_t *panel;   // [H][wn]
+  const uint32_t *packed;  // [nwords][N]
+  const float *tran;       // [M][3]
+  uint64_t *cpanel;        // [2][H][wh]
+  uint32_t *ccodes;        // [2][nw][N]
+  float *ctran;            // forward rows, then backward rows
+  unsigned M, wn;
+};
+size_t fb_layout_bytes(unsigned M, unsigned bits, unsigned N, unsigned H);
+// GLP_OK, or an error when a layout array passes 32-bit byte offsets
+int fb_check(unsigned M, unsigned bits, unsigned N, unsigned H);
+// work items of one window's layout (threads of the mirror kernel)
+unsigned fb_items(unsigned M, unsigned bits, unsigned N, unsigned H);
+int launch_fb_mirror(const FbWin *d_fw, unsigned nwin, unsigned max_items, unsigned N,
+                     unsigned H, unsigned bits, hipStream_t stream);
+const void *fb_mirror_kernel_fn();
 // One batched generation with every device buffer beyond the caller's in
 // the file-scope scratch array of glp_kernels.hip (glp_mh_run_noalloc).
 int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigned Q,

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/glphase_amd.h"
@@ -84,6 +85,18 @@ constexpr unsigned kSpreadBytes = 4 * 256 * 8;
 constexpr unsigned kTranFloats = 96;   // tran of one 32-site chunk
 constexpr unsigned kTranSlot = 128;    // a wave's LDS slot for it (a float pair per lane; a wave has two)
 constexpr unsigned kRingSlots = 32;    // per lane, F32 accumulation only
+// Forward-backward form: a slot holds the forward half's chunk of tran at
+// float 0 and the backward half's at float kTranBOff (a float4 per lane:
+// lanes 0-31 stage the forward rows, lanes 32-63 the backward ones; the
+// 16-float gap keeps lanes 24-31's unread tail writes clear of the backward
+// rows and puts the two halves' row reads in different banks).
+constexpr unsigned kTranBOff = 144;
+constexpr unsigned kTranSlotFB = kTranBOff + 128;
+
+template <bool FB>
+struct TranSlot {
+  static constexpr unsigned floats = FB ? kTranSlotFB : kTranSlot;
+};
 
 __device__ __forceinline__ void init_lds(unsigned char *smem, const float4 *__restrict__ g_lut,
                                          unsigned lut_entries) {
@@ -263,11 +276,13 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
 // allele words, cw = the chunk's code words, tr = the wave's LDS copy of the
 // chunk's tran, n = sites in the chunk (TAIL: n < 32), nrm0 = the threshold
 // of the chunk's first site (0 at the window's site 0, which only
-// initialises, impute.cpp:339-347).
+// initialises, impute.cpp:339-347); one0: the chunk's first site emits 1
+// (the identity step that opens the backward half of an odd window).
 template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
-                                          uint32_t oddm, float norm, float nrm0, float *ring) {
+                                          uint32_t oddm, float norm, float nrm0, float *ring,
+                                          bool one0 = false) {
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     if (TAIL && (unsigned)(8 * g) >= n) break;
@@ -287,6 +302,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       if (TAIL && (unsigned)s >= n) break;
       const uint32_t a = lut_addr<BITS>(cw[s / CPW], nb[k >> 2], (unsigned)(BITS == 8 ? s : k));
       e[k] = lut_entry(a);
+      if (s == 0 && one0) e[k] = make_float2(1.0f, 1.0f);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -352,16 +368,16 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
 // their use) and, from its middle, group g + 2's spread entries (which the
 // emission addresses need).  On entry A holds group 0's inputs and B group
 // 1's spread entries; at exit the same holds for the next chunk (allele
-// words hn, code words cn, tran in LDS slot trn, written from tpn during
-// group 2).  LDS reads past the window's end only read the tables.
-template <int BITS, bool MIXED, bool FAST>
+// words hn, code words cn, tran in LDS slot trn, which this lane's part tpn
+// reaches at trnw during group 2).  LDS reads past the window's end only
+// read the tables.
+template <int BITS, bool MIXED, bool FAST, class TV>
 __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, GroupLds &B,
                                                const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
                                                const float *tr, const uint32_t (&hn)[4],
-                                               const uint32_t (&cn)[BITS], float *trn, float2 tpn,
-                                               uint32_t oddm, float norm, float nrm0,
-                                               float *ring) {
-  const unsigned lane = threadIdx.x & 63u;
+                                               const uint32_t (&cn)[BITS], const float *trn,
+                                               float *trnw, TV tpn, uint32_t oddm, float norm,
+                                               float nrm0, float *ring) {
   group_load_et<BITS>(B, cw, tr, 1, oddm);
   group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
   group_load_sp(A, h, 2);
@@ -374,9 +390,9 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
 
   group_load_et<BITS>(B, cw, tr, 3, oddm);
   group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
-  // every lane writes (lanes 48-63 into the slot's unread tail), so that the
-  // write, and with it the tran load, is not under a branch
-  reinterpret_cast<float2 *>(trn)[lane] = tpn;
+  // every lane writes (the last lanes of each part into the slot's unread
+  // tail), so that the write, and with it the tran load, is not under a branch
+  *reinterpret_cast<TV *>(trnw) = tpn;
   group_load_sp(A, hn, 0);
   group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
 
@@ -416,35 +432,49 @@ __device__ __forceinline__ uint32_t buf_u32(brsrc_t r, uint32_t voff, uint32_t s
 }
 
 // Where one pass reads: the window's panel, codes and tran, the 4 parents'
-// row offsets and the sample's code column (bytes, per lane).
+// row offsets and the sample's code column (bytes, per lane); in the
+// forward-backward form also this lane's byte offset of its part of a
+// chunk's tran (the staging lanes' float4: forward rows from lanes 0-31,
+// backward rows from lanes 32-63).
 struct PassSrc {
   brsrc_t panel, codes, tran;
   uint32_t row[4];
   uint32_t col;     // 4 * sample
   uint32_t stride;  // 4 * number of samples: bytes between a sample's code words
+  uint32_t tlane;
 };
 
 // The global inputs of chunk c: the 4 parents' allele words, the code
-// words and this lane's pair of tran floats (lanes 0-47 carry a chunk's 96).
-// Every load runs for every c: past the window's end the panel and code
-// offsets are clamped and the tran floats read 0; those values are not
-// used, except by reads of the LDS tables.
-template <int BITS>
+// words and this lane's part of the chunk's tran (a float pair: lanes 0-47
+// carry a chunk's 96 floats; forward-backward form: a float4, lanes 0-23
+// the forward half's 96 and lanes 32-55 the backward half's).  Every load
+// runs for every c: past the window's end the panel and code offsets are
+// clamped and the tran floats read 0 (or a later part's rows); those values
+// are not used, except by reads of the LDS tables.
+template <int BITS, bool FB>
 struct ChunkIn {
+  using TV = typename std::conditional<FB, float4, float2>::type;
   uint32_t h[4];
   uint32_t cw[BITS];
-  float2 tp;
+  TV tp;
 };
 
-template <int BITS>
-__device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsigned nch,
+template <int BITS, bool FB>
+__device__ __forceinline__ void load_chunk(ChunkIn<BITS, FB> &in, unsigned c, unsigned nch,
                                            const PassSrc &src, unsigned nwords, unsigned lane) {
   const unsigned cc = c < nch ? c : nch - 1u;
-  // floats 96c + 2 lane and the next (tran has 3M floats; when 3M is odd
-  // the last float stands alone and its partner reads 0)
-  const uint32_t to = 4u * (kTranFloats * c + 2u * lane);
-  in.tp = make_float2(__uint_as_float(buf_u32(src.tran, to, 0u)),
-                      __uint_as_float(buf_u32(src.tran, to + 4u, 0u)));
+  if constexpr (FB) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+        src.tran, (int)src.tlane, (int)__builtin_amdgcn_readfirstlane(4u * kTranFloats * c), 0);
+    in.tp = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                        __uint_as_float(v[3]));
+  } else {
+    // floats 96c + 2 lane and the next (tran has 3M floats; when 3M is odd
+    // the last float stands alone and its partner reads 0)
+    const uint32_t to = 4u * (kTranFloats * c + 2u * lane);
+    in.tp = make_float2(__uint_as_float(buf_u32(src.tran, to, 0u)),
+                        __uint_as_float(buf_u32(src.tran, to + 4u, 0u)));
+  }
 #pragma unroll
   for (int p = 0; p < 4; ++p) in.h[p] = buf_u32(src.panel, src.row[p], 4u * cc);
   // word offsets step by stride from the chunk's first word, clamped to the
@@ -458,60 +488,115 @@ __device__ __forceinline__ void load_chunk(ChunkIn<BITS> &in, unsigned c, unsign
   }
 }
 
+// The forward-backward form of a pass (mixed precision).  The likelihood is
+// l_{n-1}^T T_n beta_n for the forward variable l, the backward variable
+// beta and the symmetric switch matrix T_n of tran row n.  The
+// emission-weighted backward variable u_m = e_m o beta_m obeys u_{m-1} =
+// e_{m-1} o (T_m u_m), which is the forward step itself.  So a lane quad
+// runs both halves side by side with one instruction stream: lanes 0-1 the
+// forward pair over sites 0 .. n-1 (n = ceil(M/2)), lanes 2-3 the backward
+// pair over sites M-1 down to n, each n steps.  The backward half reads the
+// window's data mirrored: its panel rows bit-reversed, its codes and tran
+// rows in reverse order (k_fb_mirror lays them out after the forward
+// half's), starting from u = 1 with an identity first row, and for odd M an
+// identity first step (row (1, 0, 0), emission 1: one0).  Each half keeps
+// the mixed pass's scaling and underflow guard.  The join runs in FP64: the
+// backward pair forms g = T_n u_n (its row n, staged with its data), the
+// quad exchanges halves, and every lane of it returns
+// log((l00 g00 + l11 g11) + (l01 g01 + l10 g10)) - (kF + kB) ln 2.
+struct FbInfo {
+  bool back;  // this lane belongs to the backward pair
+  bool one0;  // its first step is the identity (odd M)
+  float t0, t1, t2;  // tran row n
+};
+
+template <bool FAST>
+__device__ __forceinline__ double fb_join(const PairState &st, float last, const FbInfo &fb) {
+  const int bad = (FAST && (st.kmax >= kTinyK || !(last >= 0x1p-99f && last <= 0x1.fffffep127f)))
+                      ? 1 : 0;
+  const double a = st.la, b = st.lb;
+  const double partner = __shfl_xor(a + b, 1, 64);
+  const double t0 = fb.t0, t1 = fb.t1, t2 = fb.t2;
+  const double ga = (t0 * a + t1 * partner) + t2 * b;
+  const double gb = (t0 * b + t1 * partner) + t2 * a;
+  // the forward pair sends l, the backward pair g = T_n u
+  const double ra = __shfl_xor(fb.back ? ga : a, 2, 64);
+  const double rb = __shfl_xor(fb.back ? gb : b, 2, 64);
+  const double d = fb.back ? ra * ga + rb * gb : a * ra + b * rb;
+  const double tot = d + __shfl_xor(d, 1, 64);
+  const int ks = st.nres + __shfl_xor(st.nres, 2, 64);
+  if ((bad | __shfl_xor(bad, 2, 64)) != 0) return __builtin_nan("");
+  return log(tot) - (double)ks * kLn2;
+}
+
 // Log-likelihood of one sample given 4 parents (father pair f0, f1, mother
-// pair m0, m1: src.row), computed by a lane pair; both lanes return it.
-// Every lane of the wavefront must call it together (M and tran
-// wave-uniform): the lanes copy each 32-site chunk of tran into the wave's
-// LDS buffer wtr, during the chunk before it.
-template <int BITS, bool MIXED, bool FAST>
+// pair m0, m1: src.row), computed by a lane pair; both lanes return it
+// (FB: by a lane quad over M = n steps per half, see fb_join; all four
+// return it).  Every lane of the wavefront must call it together (M and
+// tran wave-uniform): the lanes copy each 32-site chunk of tran into the
+// wave's LDS buffer wtr, during the chunk before it.
+template <int BITS, bool MIXED, bool FAST, bool FB>
 __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
                                                const unsigned char *smem, unsigned lut_entries,
-                                               float *wtr, float *ring, float norm) {
+                                               float *wtr, float *ring, float norm,
+                                               const FbInfo &fb) {
   constexpr int CPW = 32 / BITS;
+  constexpr unsigned SLOT = TranSlot<FB>::floats;
+  using TV = typename ChunkIn<BITS, FB>::TV;
   const unsigned lane = threadIdx.x & 63u;
   // the odd lane reads the odd half of every emission entry
   const uint32_t oddm = (lane & 1u) ? 0x08080808u : 0u;
   PairState st;
-  // MIXED starts scaled by 2^kScaleT
-  st.la = st.lb = MIXED ? 0x1p98f : 0.25f;
+  // MIXED starts scaled by 2^kScaleT (l = 1/4; the backward half u = 1)
+  st.la = st.lb = FB ? (fb.back ? 0x1p100f : 0x1p98f) : (MIXED ? 0x1p98f : 0.25f);
   st.nres = MIXED ? kScaleT : 0;
   st.kmax = 0;
   st.fscore = 0.0f;
   st.nq = 0;
   const unsigned nch = (M + 31u) >> 5;
   const unsigned nwords = (M + CPW - 1) / CPW;
+  // this lane's tran write (its staging part) and read (its half) offsets
+  // within a slot
+  const unsigned woff = FB ? (lane >= 32u ? kTranBOff : 0u) + 4u * (lane & 31u) : 2u * lane;
+  const unsigned roff = FB && fb.back ? kTranBOff : 0u;
   // chunk ch's global inputs sit in buffer X (ch even) or Y (ch odd): the
   // loop body is written out twice, so that the next chunk's loads land in
   // the other buffer while this one is used, with no register copies
-  ChunkIn<BITS> X, Y;
-  load_chunk<BITS>(X, 0, nch, src, nwords, lane);
-  // site 0's row becomes (1, 0, 0), so that from l = 1/4 (F32) or 2^98
-  // (MIXED) the generic step gives l_xy = l * e_xy exactly
-  if (lane == 0u) X.tp = make_float2(1.0f, 0.0f);
-  if (lane == 1u) X.tp.x = 0.0f;
+  ChunkIn<BITS, FB> X, Y;
+  load_chunk<BITS, FB>(X, 0, nch, src, nwords, lane);
+  if constexpr (!FB) {
+    // site 0's row becomes (1, 0, 0), so that from l = 1/4 (F32) or 2^98
+    // (MIXED) the generic step gives l_xy = l * e_xy exactly (the
+    // forward-backward layout holds these rows already)
+    if (lane == 0u) X.tp = make_float2(1.0f, 0.0f);
+    if (lane == 1u) X.tp.x = 0.0f;
+  }
   // chunk ch's tran sits in LDS slot ch & 1 of the wave's two; a full
   // chunk's last group writes the next chunk's slot
-  reinterpret_cast<float2 *>(wtr)[lane] = X.tp;
+  *reinterpret_cast<TV *>(wtr + woff) = X.tp;
   GroupLds A, B;
   if (M >= 32u) {
     group_load_sp(A, X.h, 0);
-    group_load_et<BITS>(A, X.cw, wtr, 0, oddm);
+    group_load_et<BITS>(A, X.cw, wtr + roff, 0, oddm);
+    if (FB && fb.one0) A.e[0] = make_float2(1.0f, 1.0f);
     group_load_sp(B, X.h, 1);
   }
-  auto chunk = [&](unsigned ch, const ChunkIn<BITS> &cur, ChunkIn<BITS> &nxt) {
-    load_chunk<BITS>(nxt, ch + 1u, nch, src, nwords, lane);
+  auto chunk = [&](unsigned ch, const ChunkIn<BITS, FB> &cur, ChunkIn<BITS, FB> &nxt) {
+    load_chunk<BITS, FB>(nxt, ch + 1u, nch, src, nwords, lane);
     // keep the loads here, a whole chunk ahead of their use (the scheduler
     // would otherwise sink the tran loads down to the LDS write that uses them)
     __builtin_amdgcn_sched_barrier(0);
     const unsigned base = ch << 5;
     const unsigned n = M - base < 32u ? M - base : 32u;
     const float nrm0 = ch == 0 ? 0.0f : norm;
-    float *tr = wtr + kTranSlot * (ch & 1u), *trn = wtr + kTranSlot * ((ch + 1u) & 1u);
+    float *slot = wtr + SLOT * (ch & 1u), *slotn = wtr + SLOT * ((ch + 1u) & 1u);
     if (n == 32)
-      fwd_chunk_pipe<BITS, MIXED, FAST>(st, A, B, cur.h, cur.cw, tr, nxt.h, nxt.cw, trn, nxt.tp,
-                                        oddm, norm, nrm0, ring);
+      fwd_chunk_pipe<BITS, MIXED, FAST, TV>(st, A, B, cur.h, cur.cw, slot + roff, nxt.h, nxt.cw,
+                                            slotn + roff, slotn + woff, nxt.tp, oddm, norm, nrm0,
+                                            ring);
     else  // the window's last chunk, shorter than 32 sites
-      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, tr, n, oddm, norm, nrm0, ring);
+      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, slot + roff, n, oddm, norm, nrm0,
+                                         ring, FB && fb.one0 && ch == 0u);
     if (!MIXED) flush_ring(st, ring);
   };
   for (unsigned ch = 0; ch < nch; ch += 2u) {
@@ -520,7 +605,9 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
     chunk(ch + 1u, Y, X);
   }
   const float last = pair_sum(st);
-  if (MIXED) {
+  if constexpr (FB) {
+    return fb_join<FAST>(st, last, fb);
+  } else if (MIXED) {
     // a sum below 2^-99 (or 0), or one that is not finite: NaN asks
     // score_set for the redo
     if (FAST && (st.kmax >= kTinyK || !(last >= 0x1p-99f && last <= 0x1.fffffep127f)))
@@ -530,38 +617,76 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
   return (double)(st.fscore + cr_logf(last));
 }
 
+// Sizes of a window's forward-backward layout (k_fb_mirror): n steps per
+// half, half-rows of wh 64-bit words, nw code words per half column, nch
+// chunks per half; tran holds the forward half's 96 nch floats, then the
+// backward half's 96 nch + 128.
+struct FbGeom {
+  unsigned n, wh, nw, nch;
+  __host__ __device__ FbGeom(unsigned M, unsigned bits) {
+    n = (M + 1u) >> 1;
+    wh = (n + 63u) >> 6;
+    nw = (n * bits + 31u) >> 5;
+    nch = (n + 31u) >> 5;
+  }
+  __host__ __device__ unsigned tran_floats() const { return 192u * nch + 128u; }
+};
 
 // LDS layout after the two tables: per wave, two slots of kTranSlot
-// floats of tran, then (F32 only) per wave a ring of kRingSlots x 64 floats.
-template <int BITS, bool MIXED>
+// (FB: kTranSlotFB) floats of tran, then (F32 only) per wave a ring of
+// kRingSlots x 64 floats.  FB: W's panel, packed codes and tran are the
+// window's forward-backward layout (k_fb_mirror); H = its panel's rows.
+template <int BITS, bool MIXED, bool FB = false>
 __device__ __forceinline__ double score_set(const glp_window &W, const unsigned (&P)[4],
-                                            unsigned sample, unsigned N,
+                                            unsigned sample, unsigned N, unsigned H,
                                             const unsigned char *smem, unsigned lut_entries,
                                             float norm) {
+  constexpr unsigned SLOT = TranSlot<FB>::floats;
+  const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   float *tables_end = reinterpret_cast<float *>(const_cast<unsigned char *>(smem) +
                                                 16u * lut_entries + kSpreadBytes);
-  float *wtr = tables_end + 2u * kTranSlot * wave;
-  float *ring =
-      tables_end + 2u * kTranSlot * nwave + kRingSlots * 64u * wave + (threadIdx.x & 63u);
+  float *wtr = tables_end + 2u * SLOT * wave;
+  float *ring = tables_end + 2u * SLOT * nwave + kRingSlots * 64u * wave + lane;
   PassSrc src;
+  FbInfo fb{false, false, 1.0f, 0.0f, 0.0f};
+  unsigned steps = W.num_sites;
   src.panel = buf_rsrc(W.panel, 0xFFFFFFFFu);
   src.codes = buf_rsrc(W.packed, 0xFFFFFFFFu);
-  src.tran = buf_rsrc(W.tran, 12u * W.num_sites);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) src.row[q] = 8u * P[q] * W.words_per_hap;
-  src.col = 4u * sample;
   src.stride = 4u * N;
-  const double v = forward_pass<BITS, MIXED, true>(src, W.num_sites, smem, lut_entries, wtr,
-                                                   ring, norm);
+  if constexpr (FB) {
+    const FbGeom g(W.num_sites, BITS);
+    steps = g.n;
+    fb.back = ((lane >> 1) & 1u) != 0u;
+    fb.one0 = fb.back && (W.num_sites & 1u) != 0u;
+    src.tran = buf_rsrc(W.tran, 4u * g.tran_floats());
+    const uint32_t hb = fb.back ? H * g.wh : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) src.row[q] = 8u * (P[q] * g.wh + hb);
+    src.col = 4u * sample + (fb.back ? 4u * g.nw * N : 0u);
+    src.tlane = 4u * ((lane >= 32u ? 96u * g.nch : 0u) + 4u * (lane & 31u));
+    // the join's row n: the backward part's row n (tran row n of the window)
+    const uint32_t ro = 4u * (96u * g.nch + 3u * g.n);
+    fb.t0 = __uint_as_float(buf_u32(src.tran, 0u, ro));
+    fb.t1 = __uint_as_float(buf_u32(src.tran, 0u, ro + 4u));
+    fb.t2 = __uint_as_float(buf_u32(src.tran, 0u, ro + 8u));
+  } else {
+    src.tran = buf_rsrc(W.tran, 12u * W.num_sites);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) src.row[q] = 8u * P[q] * W.words_per_hap;
+    src.col = 4u * sample;
+    src.tlane = 0u;
+  }
+  const double v =
+      forward_pass<BITS, MIXED, true, FB>(src, steps, smem, lut_entries, wtr, ring, norm, fb);
   // F32: a subnormal (or zero) site sum made the short reciprocal NaN: the
   // wavefront redoes its passes with the exact one (recip_normal is exact
   // for normal sums, so a value that is not NaN is already the exact one).
   // MIXED: a sum fell below 2^-99 between two scalings, or the last one is
   // not finite: the redo scales after every site.  Each lane keeps its first value unless it was NaN.
   if (__builtin_expect(__any(v != v), 0)) {
-    const double x = forward_pass<BITS, MIXED, false>(src, W.num_sites, smem, lut_entries, wtr,
-                                                      ring, norm);
+    const double x =
+        forward_pass<BITS, MIXED, false, FB>(src, steps, smem, lut_entries, wtr, ring, norm, fb);
     return v != v ? x : v;
   }
   return v;
@@ -630,18 +755,20 @@ __device__ __forceinline__ void proposal_draws(const KArgs &a, unsigned w,
 }
 
 // ---------------------------------------------------------------------------
-// plain path: one lane pair per (window, sample), one launch per MH iteration
+// plain path: one lane pair (FB: lane quad) per (window, sample), one
+// launch per MH iteration
 // ---------------------------------------------------------------------------
-template <int BITS, bool MIXED>
+template <int BITS, bool MIXED, bool FB>
 __global__ void __launch_bounds__(256)
     k_plain(const glp_window *__restrict__ wins, KArgs a, unsigned step) {
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
+  constexpr unsigned UL = FB ? 4u : 2u;  // lanes per chain
   const unsigned w = blockIdx.y;
-  const unsigned ig = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
-  // the forward pass runs on whole wavefronts: pairs past N score a copy of
-  // sample N-1 and write nothing; the even lane of a pair writes
-  const bool valid = ig < a.N && (threadIdx.x & 1u) == 0u;
+  const unsigned ig = (blockIdx.x * blockDim.x + threadIdx.x) / UL;
+  // the forward pass runs on whole wavefronts: chains past N score a copy of
+  // sample N-1 and write nothing; the first lane of a chain writes
+  const bool valid = ig < a.N && (threadIdx.x & (UL - 1u)) == 0u;
   if (__ballot(ig < a.N) == 0ull) return;
   const unsigned i = ig < a.N ? ig : a.N - 1;
   const glp_window W = wins[w];
@@ -649,7 +776,7 @@ __global__ void __launch_bounds__(256)
   unsigned P[4];
   if (step == 0) {
     initial_parents(a, W, w, i, P);
-    const double v = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+    const double v = score_set<BITS, MIXED, FB>(W, P, i, a.N, a.H, smem, a.lut_entries, a.norm);
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) W.parents[4 * (size_t)i + q] = P[q];
@@ -668,7 +795,7 @@ __global__ void __launch_bounds__(256)
   P[1] = rp == 1 ? cand : P[1];
   P[2] = rp == 2 ? cand : P[2];
   P[3] = rp == 3 ? cand : P[3];
-  const double prop = score_set<BITS, MIXED>(W, P, i, a.N, smem, a.lut_entries, a.norm);
+  const double prop = score_set<BITS, MIXED, FB>(W, P, i, a.N, a.H, smem, a.lut_entries, a.norm);
   if (!valid) return;
   const bool acc = mh_accept(prop, curr, (double)a.pen, ub);
   if (acc) {
@@ -706,15 +833,19 @@ __global__ void __launch_bounds__(256)
 // tables: 4 (one workgroup per CU) or 8 (two wavefronts per SIMD where the
 // LDS of the F32 ring or of fewer tables allows it; the compiler then keeps
 // the kernel within 256 VGPRs).
-template <int BITS, bool MIXED, int WAVES>
+// FB (the forward-backward form): a pass takes a lane quad instead of a
+// pair, so the units below are quads and a wavefront holds 16 of them.
+template <int BITS, bool MIXED, int WAVES, bool FB>
 __global__ void __launch_bounds__(64 * WAVES)
     k_batched(const glp_window *__restrict__ wins, const unsigned *__restrict__ order,
               const unsigned *__restrict__ qtab, unsigned nwin, unsigned *__restrict__ wctr,
               unsigned long long *__restrict__ ctrs, KArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   init_lds(smem, reinterpret_cast<const float4 *>(a.lut), a.lut_entries);
-  const unsigned lane = threadIdx.x & 63u, pair = lane >> 1;
-  const bool even = (lane & 1u) == 0u;
+  constexpr unsigned UL = FB ? 4u : 2u;  // lanes per pass unit
+  constexpr unsigned NU = 64u / UL;      // units per wavefront
+  const unsigned lane = threadIdx.x & 63u, pair = lane / UL;
+  const bool even = (lane & (UL - 1u)) == 0u;
   const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double pen = (double)a.pen;
 
@@ -722,7 +853,7 @@ __global__ void __launch_bounds__(64 * WAVES)
   // home = the lane belongs to a home slot (pairs past the last whole slot
   // only score)
   int wo = -1;  // position in `order`
-  unsigned bw = 0, Q = 1, sbase = lane & ~1u;
+  unsigned bw = 0, Q = 1, sbase = lane & ~(UL - 1u);
   bool home = false;
   glp_window W{};
   // home-slot chain state, replicated over the slot's home lanes
@@ -775,8 +906,8 @@ __global__ void __launch_bounds__(64 * WAVES)
       bw = __builtin_amdgcn_readfirstlane(order[wo]);
       Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
       const unsigned slot = pair / Q;
-      sbase = 2u * Q * slot;
-      home = slot < 32u / Q;
+      sbase = UL * Q * slot;
+      home = slot < NU / Q;
       W = wins[bw];
       continue;
     }
@@ -784,7 +915,7 @@ __global__ void __launch_bounds__(64 * WAVES)
     // 3. the pairs of the round: chain rank c (in home-slot order) holds the
     //    contiguous pairs [start_c, start_c + q_c), q_c = base + [c < extra]
     const unsigned A = (unsigned)__popcll(am);
-    const unsigned base = 32u / A, extra = 32u % A, big = extra * (base + 1u);
+    const unsigned base = NU / A, extra = NU % A, big = extra * (base + 1u);
     unsigned c, j;  // this pair's chain rank and place in the chain's run
     if (pair < big) {
       c = pair / (base + 1u);
@@ -821,7 +952,7 @@ __global__ void __launch_bounds__(64 * WAVES)
     }
     // the whole wavefront runs the pass; pairs without work score their
     // chain's current set and the value is ignored
-    double v = score_set<BITS, MIXED>(W, P, ci, a.N, smem, a.lut_entries, a.norm);
+    double v = score_set<BITS, MIXED, FB>(W, P, ci, a.N, a.H, smem, a.lut_entries, a.norm);
     if (work && even) ++n_eval;
 
     // 5. serial MH walk over each active chain's values, replayed by every
@@ -833,7 +964,7 @@ __global__ void __launch_bounds__(64 * WAVES)
     const unsigned start = myc < extra ? myc * (base + 1u) : big + (myc - extra) * base;
     const unsigned qmax = base + (extra ? 1u : 0u);
     const int jstart = fresh ? 1 : 0;
-    const double v0 = __shfl(v, (2u * start) & 63u, 64);
+    const double v0 = __shfl(v, (UL * start) & 63u, 64);
     if (mine && fresh) {
       cur = v0;
       if (lane == sbase && a.trace_like) a.trace_like[((size_t)bw * a.N + i) * (a.C + 1)] = v0;
@@ -842,7 +973,7 @@ __global__ void __launch_bounds__(64 * WAVES)
     bool stop = !mine;
     for (unsigned jj = 0; jj < qmax; ++jj) {
       // the reads run on every lane; only a chain's own run is used
-      const unsigned src = (2u * (start + jj)) & 63u;
+      const unsigned src = (UL * (start + jj)) & 63u;
       const double vj = __shfl(v, src, 64);
       const unsigned rpj = __shfl(rp, src, 64);
       const unsigned cj = __shfl(cand, src, 64);
@@ -919,7 +1050,8 @@ __global__ void __launch_bounds__(256)
   const glp_window W = wins[0];
   const unsigned P[4] = {parents[4 * (size_t)e], parents[4 * (size_t)e + 1],
                          parents[4 * (size_t)e + 2], parents[4 * (size_t)e + 3]};
-  const float v = (float)score_set<BITS, false>(W, P, samples[e], a.N, smem, a.lut_entries, a.norm);
+  const float v =
+      (float)score_set<BITS, false>(W, P, samples[e], a.N, a.H, smem, a.lut_entries, a.norm);
   if (valid) out[e] = v;
 }
 
@@ -948,7 +1080,8 @@ __global__ void k_selftest_recip(uint32_t lo, uint32_t hi, unsigned long long *b
 // launchers
 // ---------------------------------------------------------------------------
 size_t lds_bytes(const KArgs &a, unsigned waves) {
-  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * kTranSlot * 4;
+  const size_t slot = a.precision == kPrecMixedFB ? kTranSlotFB : kTranSlot;
+  size_t b = (size_t)a.lut_entries * 16 + kSpreadBytes + waves * 2 * slot * 4;
   if (a.precision == kPrecF32) b += waves * kRingSlots * 64 * 4;
   return b;
 }
@@ -963,47 +1096,47 @@ static int allow_lds(const void *fn, size_t bytes) {
                    "dynamic LDS attribute");
 }
 
-int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
-                 unsigned step, hipStream_t stream) {
-  const dim3 block(256), grid((2 * a.N + 255) / 256, nwin);
-  const size_t smem = lds_bytes(a, 4);
-  const bool mixed_ = a.precision != kPrecF32;
-  const void *fn = a.bits == 8 ? (mixed_ ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>)
-                               : (mixed_ ? (const void *)&k_plain<4, true> : (const void *)&k_plain<4, false>);
-  GLP_TRY_K(allow_lds(fn, smem));
-  const bool mixed = a.precision != kPrecF32;
-  if (a.bits == 8) {
-    if (mixed)
-      hipLaunchKernelGGL((k_plain<8, true>), grid, block, smem, stream, d_wins, a, step);
-    else
-      hipLaunchKernelGGL((k_plain<8, false>), grid, block, smem, stream, d_wins, a, step);
-  } else {
-    if (mixed)
-      hipLaunchKernelGGL((k_plain<4, true>), grid, block, smem, stream, d_wins, a, step);
-    else
-      hipLaunchKernelGGL((k_plain<4, false>), grid, block, smem, stream, d_wins, a, step);
-  }
-  return hip_check(hipGetLastError(), "k_plain launch");
+template <int BITS>
+static const void *plain_kernel_b(int precision) {
+  if (precision == kPrecMixedFB) return (const void *)&k_plain<BITS, true, true>;
+  if (precision == kPrecMixed) return (const void *)&k_plain<BITS, true, false>;
+  return (const void *)&k_plain<BITS, false, false>;
 }
 
-const void *plain_kernel_fn(unsigned bits, bool mixed) {
-  if (bits == 8) return mixed ? (const void *)&k_plain<8, true> : (const void *)&k_plain<8, false>;
-  return mixed ? (const void *)&k_plain<4, true> : (const void *)&k_plain<4, false>;
+const void *plain_kernel_fn(unsigned bits, int precision) {
+  return bits == 8 ? plain_kernel_b<8>(precision) : plain_kernel_b<4>(precision);
+}
+
+// lanes of the plain path's grid: one pair per chain, a quad in the
+// forward-backward form
+unsigned plain_lanes(const KArgs &a) { return (a.precision == kPrecMixedFB ? 4u : 2u) * a.N; }
+
+int launch_plain(const KArgs &a, const glp_window *d_wins, unsigned nwin,
+                 unsigned step, hipStream_t stream) {
+  const dim3 block(256), grid((plain_lanes(a) + 255) / 256, nwin);
+  const size_t smem = lds_bytes(a, 4);
+  const void *fn = plain_kernel_fn(a.bits, a.precision);
+  GLP_TRY_K(allow_lds(fn, smem));
+  KArgs aa = a;
+  void *args[] = {(void *)&d_wins, (void *)&aa, (void *)&step};
+  return hip_check(hipLaunchKernel(fn, grid, block, args, smem, stream), "k_plain launch");
 }
 
 const void *reset_kernel_fn() { return (const void *)&k_reset_ctrs; }
 
 int prepare_kernel_lds(const void *fn, size_t bytes) { return allow_lds(fn, bytes); }
 
-template <int W>
-static const void *batched_kernel_w(unsigned bits, bool mixed) {
-  if (bits == 8)
-    return mixed ? (const void *)&k_batched<8, true, W> : (const void *)&k_batched<8, false, W>;
-  return mixed ? (const void *)&k_batched<4, true, W> : (const void *)&k_batched<4, false, W>;
+template <int BITS, int W>
+static const void *batched_kernel_bw(int precision) {
+  if (precision == kPrecMixedFB) return (const void *)&k_batched<BITS, true, W, true>;
+  if (precision == kPrecMixed) return (const void *)&k_batched<BITS, true, W, false>;
+  return (const void *)&k_batched<BITS, false, W, false>;
 }
 
-const void *batched_kernel_fn(unsigned bits, bool mixed, unsigned waves) {
-  return waves == 8 ? batched_kernel_w<8>(bits, mixed) : batched_kernel_w<4>(bits, mixed);
+const void *batched_kernel_fn(unsigned bits, int precision, unsigned waves) {
+  if (bits == 8)
+    return waves == 8 ? batched_kernel_bw<8, 8>(precision) : batched_kernel_bw<8, 4>(precision);
+  return waves == 8 ? batched_kernel_bw<4, 8>(precision) : batched_kernel_bw<4, 4>(precision);
 }
 
 int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_order,
@@ -1012,7 +1145,7 @@ int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_o
                    hipStream_t stream) {
   if (waves != 4 && waves != 8) return set_error(GLP_ERR_ARG, "waves per workgroup must be 4 or 8");
   hipLaunchKernelGGL(k_reset_ctrs, dim3(1), dim3(256), 0, stream, d_ctrs, d_wctr, nwin);
-  const void *fn = batched_kernel_fn(a.bits, a.precision != kPrecF32, waves);
+  const void *fn = batched_kernel_fn(a.bits, a.precision, waves);
   void *args[] = {(void *)&d_wins, (void *)&d_order, (void *)&d_qtab, (void *)&nwin,
                   (void *)&d_wctr, (void *)&d_ctrs, (void *)&a};
   hipError_t err = hipLaunchKernel((void *)fn, dim3(grid), dim3(64 * waves), args,
@@ -1020,12 +1153,120 @@ int launch_batched(const KArgs &a, const glp_window *d_wins, const unsigned *d_o
   return hip_check(err, "k_batched launch");
 }
 
-int batched_occupancy(unsigned bits, bool mixed, unsigned waves, size_t smem, int *blocks_per_cu) {
-  const void *fn = batched_kernel_fn(bits, mixed, waves);
+int batched_occupancy(unsigned bits, int precision, unsigned waves, size_t smem,
+                      int *blocks_per_cu) {
+  const void *fn = batched_kernel_fn(bits, precision, waves);
   GLP_TRY_K(allow_lds(fn, smem));
   return hip_check(
       hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 64 * (int)waves, smem),
       "occupancy query");
+}
+
+// ---------------------------------------------------------------------------
+// the forward-backward layout (k_fb_mirror)
+// ---------------------------------------------------------------------------
+// For a window of M sites (p = M mod 2, FbGeom g), built from the window's
+// own panel ([H][wn] u64), packed codes ([nwords][N] u32, MSB-first) and
+// tran ([M][3]):
+//   panel [2][H][g.wh] u64: part 0 = the first g.wh words of each row;
+//     part 1, bit j of row h = site M-1+p-j of row h (0 outside [0, M));
+//   codes [2][g.nw][N] u32: part 0 = the first g.nw words of each column;
+//     part 1, code slot j of column i (MSB-first) = the code of site
+//     M-1+p-j (0 outside [0, M));
+//   tran, forward rows r < 32 g.nch: (1, 0, 0) at r = 0, tran[r] for
+//     r < M, else 0; then backward rows r (96 g.nch + 128 floats): (1, 0, 0)
+//     for r <= p, tran[M+p-r] for r > p while M+p-r >= 1, else 0.
+// Step j of the backward half then reads everything at index j.
+__device__ __forceinline__ uint64_t sites64(const uint64_t *row, unsigned wn, unsigned M,
+                                            long long lo) {
+  // bits [lo, lo + 64) of the row (bit i = site lo + i), 0 outside [0, M)
+  const long long w0 = lo >= 0 ? lo / 64 : -((63 - lo) / 64);
+  const unsigned sh = (unsigned)(lo - 64 * w0);
+  const uint64_t a = (w0 >= 0 && w0 < (long long)wn) ? row[w0] : 0ull;
+  const uint64_t b = (w0 + 1 >= 0 && w0 + 1 < (long long)wn) ? row[w0 + 1] : 0ull;
+  uint64_t x = sh ? (a >> sh) | (b << (64u - sh)) : a;
+  const long long keep = (long long)M - lo;  // sites lo + i < M
+  if (keep <= 0) return 0ull;
+  if (keep < 64) x &= (1ull << keep) - 1ull;
+  if (lo < 0) x &= ~0ull << (unsigned)(-lo < 64 ? -lo : 63);
+  return x;
+}
+
+__global__ void __launch_bounds__(256)
+    k_fb_mirror(const FbWin *__restrict__ fw, unsigned N, unsigned H, unsigned bits) {
+  const FbWin f = fw[blockIdx.y];
+  const unsigned M = f.M, p = M & 1u;
+  const FbGeom g(M, bits);
+  const unsigned cpw = 32u / bits, mask = (1u << bits) - 1u;
+  const size_t np = (size_t)H * g.wh, nc = (size_t)g.nw * N, ntf = 96u * g.nch;
+  const size_t ntb = 96u * g.nch + 128u;
+  const size_t total = np + nc + ntf + ntb;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    if (t < np) {
+      const unsigned h = (unsigned)(t / g.wh), k = (unsigned)(t % g.wh);
+      const uint64_t *row = f.panel + (size_t)h * f.wn;
+      f.cpanel[t] = row[k];
+      const long long lo = (long long)M + p - 64ll * (k + 1);
+      f.cpanel[np + t] = __builtin_bitreverse64(sites64(row, f.wn, M, lo));
+    } else if (t < np + nc) {
+      const size_t u = t - np;
+      const unsigned q = (unsigned)(u / N), i = (unsigned)(u % N);
+      f.ccodes[u] = f.packed[(size_t)q * N + i];
+      uint32_t w = 0;
+      for (unsigned r = 0; r < cpw; ++r) {
+        const long long s = (long long)M - 1 + p - ((long long)q * cpw + r);
+        if (s < 0 || s >= (long long)M) continue;
+        const uint32_t src = f.packed[(size_t)(s / cpw) * N + i];
+        const uint32_t code = (src >> (32u - bits - bits * (unsigned)(s % cpw))) & mask;
+        w |= code << (32u - bits - bits * r);
+      }
+      f.ccodes[nc + u] = w;
+    } else {
+      const size_t u = t - np - nc;
+      const bool back = u >= ntf;
+      const unsigned fl = (unsigned)(back ? u - ntf : u), r = fl / 3u, c = fl % 3u;
+      float v = 0.0f;
+      if (!back) {
+        if (r == 0u) v = c == 0u ? 1.0f : 0.0f;
+        else if (r < M) v = f.tran[3u * r + c];
+      } else if (r <= p) {
+        v = c == 0u ? 1.0f : 0.0f;
+      } else if ((long long)M + p - r >= 1) {
+        v = f.tran[3u * (M + p - r) + c];
+      }
+      f.ctran[u] = v;
+    }
+  }
+}
+
+size_t fb_layout_bytes(unsigned M, unsigned bits, unsigned N, unsigned H) {
+  const FbGeom g(M, bits);
+  return 16u * (size_t)H * g.wh + 8u * (size_t)g.nw * N + 4u * (size_t)g.tran_floats();
+}
+
+int fb_check(unsigned M, unsigned bits, unsigned N, unsigned H) {
+  const FbGeom g(M, bits);
+  // buffer descriptors with 32-bit byte offsets over each array
+  if (16ull * H * g.wh >= (1ull << 32) || 8ull * g.nw * N >= (1ull << 32))
+    return set_error(GLP_ERR_ARG, "a window's forward-backward layout reaches 4 GiB");
+  return GLP_OK;
+}
+
+int launch_fb_mirror(const FbWin *d_fw, unsigned nwin, unsigned max_items, unsigned N,
+                     unsigned H, unsigned bits, hipStream_t stream) {
+  unsigned gx = (max_items + 255u) / 256u;
+  const unsigned cap = std::max(1u, 4096u / std::max(1u, nwin));
+  gx = std::max(1u, std::min(gx, cap));
+  hipLaunchKernelGGL(k_fb_mirror, dim3(gx, nwin), dim3(256), 0, stream, d_fw, N, H, bits);
+  return hip_check(hipGetLastError(), "k_fb_mirror launch");
+}
+
+const void *fb_mirror_kernel_fn() { return (const void *)&k_fb_mirror; }
+
+unsigned fb_items(unsigned M, unsigned bits, unsigned N, unsigned H) {
+  const FbGeom g(M, bits);
+  return (unsigned)((size_t)H * g.wh + (size_t)g.nw * N + 192u * g.nch + 128u);
 }
 
 int launch_eval(const KArgs &a, const glp_window *d_wins, const unsigned *d_samples,
@@ -1111,9 +1352,12 @@ int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigne
   auto *order = reinterpret_cast<unsigned *>(base + kNoAllocOrder);
   auto *qtab = reinterpret_cast<unsigned *>(base + kNoAllocQtab);
   auto *wins = reinterpret_cast<glp_window *>(base + kNoAllocWins);
+  if (a.precision == kPrecMixedFB)
+    return set_error(GLP_ERR_ARG, "glp_mh_run_noalloc: the forward-backward form needs a plan "
+                                  "(its window layouts are allocated)");
   const size_t smem = lds_bytes(a, 4);
   int bpc = 0;
-  GLP_TRY_K(batched_occupancy(a.bits, a.precision != kPrecF32, 4, smem, &bpc));
+  GLP_TRY_K(batched_occupancy(a.bits, a.precision, 4, smem, &bpc));
   if (bpc < 1) return set_error(GLP_ERR_ARG, "the batched kernel's workgroup does not fit a CU (LDS)");
   // the plan's grid: one wave per floor(32 / Q) chains, 4 waves a block
   const double waves = (double)nwin * std::ceil((double)a.N / (double)(32u / Q));

@@ -49,9 +49,14 @@ enum { GLP_PROP_UNIFORM = 0, GLP_PROP_KLIST = 1, GLP_PROP_HOST = 2 };
  * recurrence and FP32 log accumulation (bit-identical to Impute::hmm_like);
  * GLP_PREC_MIXED: the same FP32 recurrence, kept in range by exact power-of-two
  * scalings after every 8th site instead of the per-site 1/sum rescale, FP64
- * log-likelihood and FP64 MH ratio.  An FP16 emission table was measured at up to 1.3e-4 relative error
- * against the FP64 forward pass (bound 1e-5) and is not provided. */
-enum { GLP_PREC_F32 = 0, GLP_PREC_MIXED = 1 };
+ * log-likelihood and FP64 MH ratio; GLP_PREC_MIXED_FB: the mixed arithmetic
+ * evaluated as two halves side by side (the forward variable over sites
+ * 0 .. ceil(M/2)-1 and the emission-weighted backward variable over the rest,
+ * joined in FP64), which halves a pass's latency; it needs allocated window
+ * layouts, so it runs through plans (glp_plan_*) only.  An FP16 emission table
+ * was measured at up to 1.3e-4 relative error against the FP64 forward pass
+ * (bound 1e-5) and is not provided. */
+enum { GLP_PREC_F32 = 0, GLP_PREC_MIXED = 1, GLP_PREC_MIXED_FB = 2 };
 
 const char *glp_last_error(void);
 const char *glp_version(void);
@@ -181,11 +186,15 @@ int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
 
 /* Batched form: a persistent work-queue kernel runs init + all C proposals of
  * every (window, sample) chain, windows longest first; each round,
- * lanes_per_chain forward passes (each on a pair of lanes) evaluate
- * consecutive proposals of one chain speculatively from the same state, and
- * a sequential walk keeps exactly the serial MH result.  lanes_per_chain is
- * 1..32 for every window (a wavefront holds floor(32 / lanes) chains; 0 = the
- * deepest value whose wavefronts fit one per SIMD; see glp_plan_autotune). */
+ * lanes_per_chain forward passes (each on a pair of lanes; a quad under
+ * GLP_PREC_MIXED_FB) evaluate consecutive proposals of one chain
+ * speculatively from the same state, and a sequential walk keeps exactly the
+ * serial MH result.  lanes_per_chain is 1..32 for every window (1..16 under
+ * GLP_PREC_MIXED_FB: a wavefront holds floor(32 / lanes) chains, or
+ * floor(16 / lanes); 0 = the deepest value whose wavefronts fit one per
+ * SIMD; see glp_plan_autotune).  Under GLP_PREC_MIXED_FB the plan allocates
+ * each window's forward-backward layout (about its panel and codes once
+ * more) and every run rebuilds it from the windows' current data first. */
 typedef struct glp_plan glp_plan;
 int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain,
                     glp_plan **out);
@@ -199,7 +208,7 @@ int glp_plan_stats(glp_plan *plan, uint64_t *evals, uint64_t *useful);
 /* Lanes per chain of the longest window and grid size the plan launches
  * with. */
 int glp_plan_lanes(glp_plan *plan, unsigned *lanes_per_chain, unsigned *grid);
-/* Sets passes per chain round per window (1..32). */
+/* Sets passes per chain round per window (1..32; 1..16 under GLP_PREC_MIXED_FB). */
 int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
 /* Wavefronts per workgroup of the batched kernel: 4 (the default; one
  * workgroup per CU with the 8-bit emission table) or 8 (the workgroup's 8

@@ -53,7 +53,7 @@ def L():
         lib.orc_unif_excl.restype = ctypes.c_uint32
         lib.orc_hmm_like_f32.restype = ctypes.c_float
         for f in ("orc_hmm_like_mixed", "orc_hmm_like_mixed_pass", "orc_hmm_like_f64",
-                  "orc_hmm_like_brute"):
+                  "orc_hmm_like_brute", "orc_hmm_like_fb", "orc_hmm_like_fb_pass"):
             getattr(lib, f).restype = ctypes.c_double
         lib.orc_vq_find.restype = ctypes.c_uint
         lib.orc_mu.restype = ctypes.c_float
@@ -165,6 +165,24 @@ def hmm_like_mixed_pass(haps, parents, emit, tran, every):
     v = L().orc_hmm_like_mixed_pass(P(h, u64p), ctypes.c_uint(h.shape[1]), P(pr, u32p),
                                     P(e, f32p), P(t, f32p), ctypes.c_uint(e.shape[0]),
                                     ctypes.c_uint(every), ctypes.byref(tiny))
+    return v, bool(tiny.value)
+
+
+def hmm_like_fb(haps, parents, emit, tran):
+    return _like(L().orc_hmm_like_fb, haps, parents, emit, tran)
+
+
+def hmm_like_fb_pass(haps, parents, emit, tran, every):
+    """One forward-backward pass (both halves scaled every 8th step or every
+    step) and whether either half flagged a tiny or non-finite sum."""
+    h = np.ascontiguousarray(haps, np.uint64)
+    pr = np.ascontiguousarray(parents, np.uint32)
+    e = np.ascontiguousarray(emit, np.float32)
+    t = np.ascontiguousarray(tran, np.float32)
+    tiny = ctypes.c_int(0)
+    v = L().orc_hmm_like_fb_pass(P(h, u64p), ctypes.c_uint(h.shape[1]), P(pr, u32p),
+                                 P(e, f32p), P(t, f32p), ctypes.c_uint(e.shape[0]),
+                                 ctypes.c_uint(every), ctypes.byref(tiny))
     return v, bool(tiny.value)
 
 

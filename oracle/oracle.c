@@ -263,6 +263,119 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
   return orc_hmm_like_mixed_pass(haps, wn, P, emit, tran, M, 1, &tiny);
 }
 
+/* Mixed precision, forward-backward form.  For any split, the likelihood of
+ * the model of impute.cpp:327-375 is l_{n-1}^T T_n beta_n, where l is the
+ * forward variable, beta_{m-1} = T_m (e_m o beta_m) the backward one
+ * (beta_{M-1} = 1) and T_m the symmetric 4-state switch matrix of tran row m
+ * (hmmLike.h:18-20).  The emission-weighted backward variable u_m = e_m o
+ * beta_m follows u_{m-1} = e_{m-1} o (T_m u_m): the forward step's shape,
+ * transition then emission.  So two halves of n = ceil(M/2) steps each run
+ * the mixed pass's step (orc_hmm_like_mixed_pass): the forward half over
+ * sites 0 .. n-1 from 1/4, the backward half over sites M-1 down to n from
+ * 1 (step j at site M-1+p-j with tran row M+p-j, p = M mod 2; the first
+ * step has tran row (1, 0, 0), and for odd M its emission is 1, an exact
+ * identity step).  Each half starts scaled by 2^100 and scales by powers of
+ * two after every 8th of its own steps (every = 8) or after every step
+ * (every = 1), as the mixed pass does.  The halves join in FP64:
+ *   g = T_n u_n   (tran row n, or the identity row when n = M, i.e. M = 1),
+ *   L = (l00 g00 + l11 g11) + (l01 g01 + l10 g10),
+ * and the score is log(L) - (kF + kB) ln 2.  *tiny is set when either half
+ * had a scaling with k >= 200 or a last sum below 2^-99 or not finite. */
+static void fb_half(const uint64_t *const par[4], const float *emit, const float *tran,
+                    unsigned M, int backward, unsigned every, float x[4], long *ksum,
+                    int *tiny) {
+  const unsigned n = (M + 1) / 2, p = M & 1u;
+  float l00, l01, l10, l11;
+  l00 = l01 = l10 = l11 = backward ? 0x1p100f : 0x1p98f;
+  long ks = 100;
+  int ex;
+  for (unsigned j = 0; j < n; ++j) {
+    const unsigned s = backward ? M - 1 + p - j : j;
+    float t0 = 1.0f, t1 = 0.0f, t2 = 0.0f;
+    if (j > 0 && !(backward && j <= p)) {
+      const float *t = tran + 3 * (size_t)(backward ? s + 1 : s);
+      t0 = t[0];
+      t1 = t[1];
+      t2 = t[2];
+    }
+    float e00 = 1.0f, e01 = 1.0f, e10 = 1.0f, e11 = 1.0f;
+    if (!(backward && p && j == 0)) {
+      const float *e = emit + 4 * (size_t)s;
+      e00 = e[(hbit(par[0], s) << 1) | hbit(par[2], s)];
+      e01 = e[(hbit(par[0], s) << 1) | hbit(par[3], s)];
+      e10 = e[(hbit(par[1], s) << 1) | hbit(par[2], s)];
+      e11 = e[(hbit(par[1], s) << 1) | hbit(par[3], s)];
+    }
+    const float pe = (l00 + l11) * t1, po = (l01 + l10) * t1;
+    const float b00 = fmaf(l11, t2, fmaf(l00, t0, po));
+    const float b01 = fmaf(l10, t2, fmaf(l01, t0, pe));
+    const float b10 = fmaf(l01, t2, fmaf(l10, t0, pe));
+    const float b11 = fmaf(l00, t2, fmaf(l11, t0, po));
+    l00 = b00 * e00;
+    l01 = b01 * e01;
+    l10 = b10 * e10;
+    l11 = b11 * e11;
+    if (every == 1 || (j & 7u) == 7u) {
+      (void)frexpf((l00 + l11) + (l01 + l10), &ex);
+      const int k = 101 - ex;
+      if (k >= 200)
+        *tiny = 1;
+      l00 = ldexpf(l00, k);
+      l01 = ldexpf(l01, k);
+      l10 = ldexpf(l10, k);
+      l11 = ldexpf(l11, k);
+      ks += k;
+    }
+  }
+  const float last = l00 + l01 + l10 + l11;
+  if (!(last >= 0x1p-99f && last <= FLT_MAX))
+    *tiny = 1;
+  x[0] = l00;
+  x[1] = l01;
+  x[2] = l10;
+  x[3] = l11;
+  *ksum = ks;
+}
+
+double orc_hmm_like_fb_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                            const float *emit, const float *tran, unsigned M,
+                            unsigned every, int *tiny) {
+  const uint64_t *const par[4] = {haps + (size_t)P[0] * wn, haps + (size_t)P[1] * wn,
+                                  haps + (size_t)P[2] * wn, haps + (size_t)P[3] * wn};
+  const unsigned n = (M + 1) / 2;
+  float l[4], u[4];
+  long kf = 0, kb = 0;
+  fb_half(par, emit, tran, M, 0, every, l, &kf, tiny);
+  fb_half(par, emit, tran, M, 1, every, u, &kb, tiny);
+  double t0 = 1.0, t1 = 0.0, t2 = 0.0;
+  if (n < M) {
+    t0 = tran[3 * (size_t)n];
+    t1 = tran[3 * (size_t)n + 1];
+    t2 = tran[3 * (size_t)n + 2];
+  }
+  const double u00 = u[0], u01 = u[1], u10 = u[2], u11 = u[3];
+  const double se = u00 + u11, so = u01 + u10;
+  const double g00 = (t0 * u00 + t1 * so) + t2 * u11;
+  const double g11 = (t0 * u11 + t1 * so) + t2 * u00;
+  const double g01 = (t0 * u01 + t1 * se) + t2 * u10;
+  const double g10 = (t0 * u10 + t1 * se) + t2 * u01;
+  const double de = (double)l[0] * g00 + (double)l[3] * g11;
+  const double dd = (double)l[1] * g01 + (double)l[2] * g10;
+  return log(de + dd) - (double)(kf + kb) * ORC_LN2;
+}
+
+/* The forward-backward form: both halves scaled every 8th step; when either
+ * flagged a sum below 2^-99 or a last sum not finite, or the result is NaN,
+ * both scaled after every step. */
+double orc_hmm_like_fb(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                       const float *emit, const float *tran, unsigned M) {
+  int tiny = 0;
+  const double v = orc_hmm_like_fb_pass(haps, wn, P, emit, tran, M, 8, &tiny);
+  if (!tiny && v == v)
+    return v;
+  return orc_hmm_like_fb_pass(haps, wn, P, emit, tran, M, 1, &tiny);
+}
+
 /* FP64 forward oracle: the same model, normalised at every site. */
 double orc_hmm_like_f64(const uint64_t *haps, unsigned wn, const unsigned P[4],
                         const float *emit, const float *tran, unsigned M) {
@@ -340,6 +453,8 @@ static double like_of(const orc_window *w, const unsigned P[4],
                       const float *emit, int precision) {
   if (precision == ORC_PREC_MIXED)
     return orc_hmm_like_mixed(w->haps, w->wn, P, emit, w->tran, w->M);
+  if (precision == ORC_PREC_MIXED_FB)
+    return orc_hmm_like_fb(w->haps, w->wn, P, emit, w->tran, w->M);
   return (double)orc_hmm_like_f32(w->haps, w->wn, P, emit, w->tran, w->M);
 }
 

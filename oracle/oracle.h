@@ -21,6 +21,9 @@
  *   orc_hmm_like_f32    closed forms (r=0.5, r=0, constant emission), 4^M
  *                       brute force in FP64 (M<=8), FP64 forward, swap
  *                       invariances.
+ *   orc_hmm_like_fb     FP64 forward, 4^M brute force (odd and even M), the
+ *                       mixed pass's closed forms, its forward half against
+ *                       the mixed pass, the underflow guard's mu^2 runs.
  *   orc_hmm_like_f64    brute force (M<=8).
  *   orc_mh_window       MHSampler semantics of other_unittests.cpp:256-283,
  *                       planted-parent behaviour of cuda_tests.cpp:118-656.
@@ -73,6 +76,13 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
 double orc_hmm_like_mixed_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
                                const float *emit, const float *tran, unsigned M,
                                unsigned every, int *tiny);
+/* mixed precision, forward-backward form: two halves of ceil(M/2) steps
+ * joined in FP64 at the middle (see oracle.c); the single pass as above */
+double orc_hmm_like_fb(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                       const float *emit, const float *tran, unsigned M);
+double orc_hmm_like_fb_pass(const uint64_t *haps, unsigned wn, const unsigned P[4],
+                            const float *emit, const float *tran, unsigned M,
+                            unsigned every, int *tiny);
 double orc_hmm_like_f64(const uint64_t *haps, unsigned wn, const unsigned P[4],
                         const float *emit, const float *tran, unsigned M);
 double orc_hmm_like_brute(const uint64_t *haps, unsigned wn,
@@ -81,7 +91,7 @@ double orc_hmm_like_brute(const uint64_t *haps, unsigned wn,
 
 /* ---- MH chain over a range of samples ----------------------------------- */
 enum { ORC_PROP_UNIFORM = 0, ORC_PROP_KLIST = 1, ORC_PROP_HOST = 2 };
-enum { ORC_PREC_F32 = 0, ORC_PREC_MIXED = 1 };
+enum { ORC_PREC_F32 = 0, ORC_PREC_MIXED = 1, ORC_PREC_MIXED_FB = 2 };
 
 typedef struct {
   const uint64_t *haps;    /* [nhaps][wn] */
