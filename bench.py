@@ -8,8 +8,10 @@ N * (C + 1) * M: one forward-recurrence step of one chain at one site.
 Default workload: 5,000 samples x 4,096 sites, 64 proposals per sample,
 synthetic founder-mosaic haplotypes and Poisson(2)-depth GLs, 8-bit VQ codes
 (codebook trained on every GL), batched persistent kernel with FP32
-recurrence and FP64 log accumulation.  The plain path (one launch per MH
-iteration) runs on the same inputs; one generation is replayed on the host
+recurrence and FP64 log accumulation, each pass run as a forward and a
+backward half joined in FP64 (GLP_PREC_MIXED_FB; --precision mixed runs the
+forward-only pass).  The forward-only pass and the plain path (one launch per
+MH iteration) run on the same inputs; one generation is replayed on the host
 against an FP64 forward pass (log-likelihood error, accept decisions); the
 other benchmark workloads run beside it, and the CPU oracle (C, -O3) is
 timed on a bounded sample of each.
@@ -129,18 +131,17 @@ def fp64_forward(panel, codes, E, tran, samples, parents):
     return score
 
 
-def accuracy_vs_fp64(w, C, pen, seed, lanes, gen=0, n_chains=160, tol=1e-5):
+def accuracy_vs_fp64(w, C, pen, seed, lanes, precision, gen=0, n_chains=160, tol=1e-5):
     """Runs generation `gen` of the timed workload once more with traces and
     replays n_chains chains on the host (their Philox draws, the device's
     accept decisions): every proposal's log-likelihood against FP64, and the
     accept decisions FP64 values would take, outside a band of relative
     width tol around the accept threshold."""
     import torch  # noqa: F401
-    from glphase_amd import PREC_MIXED
     from glphase_amd.engine import Job
     N, H = w["N"], w["H"]
     job = Job([dict(panel=w["panel"], packed=w["packed"], tran=w["tran"])], N, H, w["codebook"],
-              w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED, trace=True)
+              w["pc"], bits=8, num_cycles=C, precision=precision, trace=True)
     job.make_plan(lanes)
     job.run_batched(gen=gen, pen=pen, seed=seed)
     tl, ta = job.trace()
@@ -330,9 +331,10 @@ def aux_knn(torch, steps, seed=11, cpu=True):
 def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64, cpu=True):
     """The MH update of the reference-panel workload: 10,000 samples whose
     candidates are their K = 200 nearest haplotypes of panel + samples
-    (lists from the kNN build), 64 proposals each, mixed precision,
-    autotuned depth.  Cost unit: the headline's site-eval."""
-    from glphase_amd import PREC_MIXED, PROP_KLIST, host
+    (lists from the kNN build), 64 proposals each, mixed precision in the
+    forward-backward form, autotuned depth.  Cost unit: the headline's
+    site-eval."""
+    from glphase_amd import PREC_MIXED_FB as PREC, PROP_KLIST, host
     from glphase_amd.engine import Job
     N, M = d["gls"].shape[0], d["gls"].shape[1]
     tran, pc, _ = host.model_params(2 * N, d["r_morgans"])  # hn = 2N (insti.cpp:1205)
@@ -340,7 +342,7 @@ def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64, cpu=True):
     packed = host.pack_codes(host.vq_codes(d["gls"], cb, device=True), 8)
     job = Job([dict(panel=panel, packed=packed, tran=tran)], N, panel.shape[0], cb, pc, bits=8,
               num_cycles=C, prop_mode=PROP_KLIST, klist=klist.cpu().numpy().view(np.uint32)[None],
-              precision=PREC_MIXED)
+              precision=PREC)
     job.make_plan(0)
     lanes = int(job.autotune(gen=0, pen=1.0, seed=seed)[0])
     gen = [0]
@@ -366,10 +368,11 @@ def aux_reference_panel_mh(torch, d, panel, klist, steps, seed, C=64, cpu=True):
            "lanes_per_chain": lanes,
            "plain": {"ms_per_step": pms, "value": se / (pms / 1e3)},
            "config": {"samples": N, "panel_haplotypes": panel.shape[0] - 2 * N, "sites": M,
-                      "proposals": C, "K": int(klist.shape[1]), "precision": "mixed"}}
+                      "proposals": C, "K": int(klist.shape[1]),
+                      "precision": "mixed, forward-backward"}}
     if cpu:
         out["cpu"] = oracle_rates([dict(panel=panel, packed=packed, tran=tran)], cb, pc, N, C,
-                                  1.0, seed, PREC_MIXED, prop_mode=PROP_KLIST,
+                                  1.0, seed, PREC, prop_mode=PROP_KLIST,
                                   klist=klist.cpu().numpy().view(np.uint32))
     return out
 
@@ -599,6 +602,11 @@ def main():
     ap.add_argument("--cycles", type=int, default=64)
     ap.add_argument("--depth", type=float, default=2.0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per chain (0 = auto)")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="wavefronts per workgroup with --lanes (4 or 8; 0 = the plan's default)")
+    ap.add_argument("--precision", choices=("fb", "mixed"), default="fb",
+                    help="fb: two half-window passes joined in FP64 (GLP_PREC_MIXED_FB); "
+                         "mixed: the forward-only pass (GLP_PREC_MIXED)")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aux", action="store_true", help="skip the kNN and VQ measurements")
@@ -616,16 +624,19 @@ def main():
 
     import torch
 
-    from glphase_amd import PREC_MIXED
+    from glphase_amd import PREC_MIXED, PREC_MIXED_FB
     from glphase_amd.engine import Job
 
     N, M, C = args.samples, args.sites, args.cycles
     pen, seed = 1.0, 20240601
+    prec = PREC_MIXED_FB if args.precision == "fb" else PREC_MIXED
     w = make_workload(N, M, C, args.depth, args.seed)
     win = dict(panel=w["panel"], packed=w["packed"], tran=w["tran"])
-    job = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C,
-              precision=PREC_MIXED)
+    job = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C, precision=prec)
     lanes, grid = job.make_plan(args.lanes)
+    if args.lanes and args.waves:
+        job.set_waves(args.waves)
+        lanes, grid = job.lanes()
     if not args.lanes:
         t0 = time.time()
         tuned = job.autotune(gen=0, pen=pen, seed=seed)
@@ -690,32 +701,60 @@ def main():
     fms = fsecs / args.steps * 1e3
     log(f"full generation (MH + hmm_work + adopt + replace): {fms:.3f} ms/step")
 
-    # plain path on the same inputs: one launch per MH iteration
-    job_p = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C, precision=PREC_MIXED)
-    job_p.make_plan(1)
-    gp = [0]
+    # plain path on the same inputs: one launch per MH iteration, in the
+    # headline's arithmetic and in the forward-only one
+    def plain_rate(precision):
+        job_p = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C,
+                    precision=precision)
+        job_p.make_plan(1)
+        gp = [0]
 
-    def pstep():
-        job_p.run_plain(gen=gp[0], pen=pen, seed=seed)
-        gp[0] += 1
+        def pstep():
+            job_p.run_plain(gen=gp[0], pen=pen, seed=seed)
+            gp[0] += 1
 
-    psecs, _ = time_steps(torch, pstep, args.plain_steps, 1)
-    pms = psecs / args.plain_steps * 1e3
-    pvalue = se_step * args.plain_steps / psecs
-    log(f"plain: {pms:.3f} ms/step, {pvalue:.3e} SE/s")
+        psecs, _ = time_steps(torch, pstep, args.plain_steps, 1)
+        pg = [0]
 
-    pg = [0]
+        def pgstep():
+            job_p.run_plain_graph(gen=pg[0], pen=pen, seed=seed)
+            pg[0] += 1
 
-    def pgstep():
-        job_p.run_plain_graph(gen=pg[0], pen=pen, seed=seed)
-        pg[0] += 1
+        pgsecs, _ = time_steps(torch, pgstep, args.plain_steps, 1)
+        job_p.close()
+        return psecs / args.plain_steps * 1e3, pgsecs / args.plain_steps * 1e3
 
-    pgsecs, _ = time_steps(torch, pgstep, args.plain_steps, 1)
-    pgms = pgsecs / args.plain_steps * 1e3
-    log(f"plain (HIP graph): {pgms:.3f} ms/step")
+    pms, pgms = plain_rate(prec)
+    pvalue = se_step / (pms / 1e3)
+    log(f"plain: {pms:.3f} ms/step, {pvalue:.3e} SE/s; HIP graph {pgms:.3f} ms/step")
+    forward_only = None
+    if prec == PREC_MIXED_FB:
+        # the forward-only mixed pass (GLP_PREC_MIXED) on the same inputs
+        job_m = Job([win], N, w["H"], w["codebook"], w["pc"], bits=8, num_cycles=C,
+                    precision=PREC_MIXED)
+        job_m.make_plan(args.lanes)
+        if not args.lanes:
+            job_m.autotune(gen=0, pen=pen, seed=seed)
+        mgen = [0]
+
+        def mstep():
+            job_m.run_batched(gen=mgen[0], pen=pen, seed=seed)
+            mgen[0] += 1
+
+        msecs, _ = time_steps(torch, mstep, args.steps, 1)
+        mms = msecs / args.steps * 1e3
+        mlanes = job_m.lanes()[0]
+        job_m.close()
+        mpms, _ = plain_rate(PREC_MIXED)
+        forward_only = {"ms_per_step": mms, "value": se_step / (mms / 1e3),
+                        "lanes_per_chain": mlanes, "plain_ms_per_step": mpms,
+                        "plain_value": se_step / (mpms / 1e3),
+                        "fb_speedup": mms / ms, "batched_fb_over_plain_forward_only": mpms / ms}
+        log(f"forward-only mixed pass: batched {mms:.3f} ms/step (lanes {mlanes}), plain "
+            f"{mpms:.3f} ms/step; the forward-backward form is {mms / ms:.2f}x faster")
 
     achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
-    workload_key = f"{N}x{M}x{C}"
+    workload_key = f"{N}x{M}x{C}" + (" fb" if prec == PREC_MIXED_FB else "")
     traffic = recorded_traffic(workload_key)
     out = {
         "metric": "HMM site-evals/s (sample x proposal x site), batched MH generation",
@@ -728,10 +767,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 recurrence, fp64 log accumulation",
+        "dtype": "fp32 recurrence, fp64 log accumulation" +
+                 (" (forward and backward halves joined in fp64)" if prec == PREC_MIXED_FB else ""),
         "data": "synthetic founder-mosaic haplotypes and GLs",
         "config": {"workload": f"{N} samples x {M} sites, {C} proposals/sample, 8-bit VQ GLs, "
-                               "batched persistent kernel (mixed precision)",
+                               "batched persistent kernel (mixed precision" +
+                               (", forward-backward form)" if prec == PREC_MIXED_FB else ")"),
+                   "precision": args.precision,
                    "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes,
                    "waves_per_workgroup": job.waves(),
                    "vq_training_gls": N * M, "vq_iterations": w["vq_iters"]},
@@ -741,6 +783,7 @@ def main():
                   "batched_over_plain": value / pvalue, "graph_ms_per_step": pgms,
                   "roofline_frac": FLOP_PER_SE * se_step / (pms / 1e3) / 1e12 / PEAK_FP32_TFLOPS},
         "graph": {"batched_ms_per_step": gms, "plain_ms_per_step": pgms},
+        "forward_only": forward_only,
         "hmm_work": {"ms_per_step": wms, "generation_ms": ms + wms, "full_generation_ms": fms,
                      "note": "Impute::hmm_work for all samples after the MH update"},
         "speculation": {"chain_passes": evals, "useful_passes": useful},
@@ -749,7 +792,7 @@ def main():
                      "flop_per_site_eval": FLOP_PER_SE,
                      "hbm_gbps": None if traffic is None else traffic / (ms / 1e3) / 1e9},
     }
-    out["accuracy"] = accuracy_vs_fp64(w, C, pen, seed, lanes)
+    out["accuracy"] = accuracy_vs_fp64(w, C, pen, seed, lanes, prec)
     a = out["accuracy"]
     log(f"accuracy vs FP64 ({a['chains']} chains, {a['proposals']} proposals of generation "
         f"{a['generation']}): max {a['max_rel_err']:.2e}, p99 {a['p99_rel_err']:.2e}; accept "
@@ -763,7 +806,7 @@ def main():
         out["small_run"] = aux_small(torch, args.steps, 100, 1024, 200, 41, cpu=cpu)
         out["whole_run"] = aux_whole_run(torch, w, C)
     if cpu:
-        out["cpu_baseline"] = cpu_baseline(w, C, pen, seed)
+        out["cpu_baseline"] = cpu_baseline(w, C, pen, seed, precision=prec)
     if not args.no_aux:
         out["configs"] = config_table(out)
     print(json.dumps(out), flush=True)
@@ -794,9 +837,10 @@ def config_table(out):
                   workload="2500 x 32 windows (lognormal sites), C = 32, fp32"),
         "4": dict(row(cb and {"cpu_1core": cb["single_core"]["value"], "cpu_allcore": cb["value"]},
                       out["plain"]["value"], out["value"], "batched"),
-                  workload="5000 x 4096, C = 64, mixed precision"),
+                  workload="5000 x 4096, C = 64, mixed precision, forward-backward form"),
         "5": dict(row(c5.get("cpu"), c5["plain"]["value"], c5["value"], "batched"),
-                  workload="10000 + 20000 panel x 2048, K = 200, C = 64, mixed precision"),
+                  workload="10000 + 20000 panel x 2048, K = 200, C = 64, mixed precision, "
+                           "forward-backward form"),
     }
 
 

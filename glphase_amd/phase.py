@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from . import host
-from ._lib import PREC_MIXED, PROP_KLIST, PROP_UNIFORM, check, lib
+from ._lib import PREC_MIXED_FB, PROP_KLIST, PROP_UNIFORM, check, lib
 from .engine import Job, Resampler, _dev, _torch, knn_lists
 
 
@@ -56,7 +56,7 @@ class PhaseRun:
 
     def __init__(self, het_aa, pos=None, tran=None, ref_haps=None, bits=8, cycles=None,
                  burnin=56, sampling=200, sa_burnin=28, seed=1, knn_k=0, recluster_every=0,
-                 training_size=0, precision=PREC_MIXED, lanes=0, kickstart=False,
+                 training_size=0, precision=PREC_MIXED_FB, lanes=0, kickstart=False,
                  device="cuda:0"):
         torch = _torch()
         self.torch = torch

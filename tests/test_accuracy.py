@@ -1,7 +1,8 @@
 """Accuracy of the forward-pass arithmetic against the FP64 forward oracle
 (per-proposal |dloglik| / |loglik_FP64|) for the reference FP32 arithmetic
 (which the GPU reproduces bit for bit), the mixed form (FP32 recurrence,
-FP64 log-scale accumulation) and an FP16 emission table; and genotype
+FP64 log-scale accumulation), its forward-backward form (two halves joined
+in FP64) and an FP16 emission table; and genotype
 discordance of whole GPU generations against the simulated truth.
 
 The CPU part prints its statistics (pytest -s shows them); the bounds it
@@ -25,7 +26,7 @@ def rel_errors(orc, p, n_eval, seed, lut_variant=None):
     table, for n_eval random (sample, 4 parents) against FP64."""
     rng = np.random.default_rng(seed)
     lut = orc.emit_lut(p["codebook"], p["pc"])
-    out = {"f32": [], "mixed": [], "variant": []}
+    out = {"f32": [], "mixed": [], "fb": [], "variant": []}
     for _ in range(n_eval):
         i = int(rng.integers(p["N"]))
         P = rng.integers(0, p["H"], 4).astype(np.uint32)
@@ -33,6 +34,7 @@ def rel_errors(orc, p, n_eval, seed, lut_variant=None):
         f64 = orc.hmm_like_f64(p["panel"], P, emit, p["tran"])
         out["f32"].append(abs(orc.hmm_like_f32(p["panel"], P, emit, p["tran"]) - f64) / abs(f64))
         out["mixed"].append(abs(orc.hmm_like_mixed(p["panel"], P, emit, p["tran"]) - f64) / abs(f64))
+        out["fb"].append(abs(orc.hmm_like_fb(p["panel"], P, emit, p["tran"]) - f64) / abs(f64))
         if lut_variant is not None:
             ev = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut_variant)
             v = orc.hmm_like_mixed(p["panel"], P, ev, p["tran"])
@@ -50,6 +52,8 @@ def test_forward_error_against_fp64(orc):
     assert e["f32"].max() < 1e-5
     assert e["mixed"].max() < 1e-5
     assert e["mixed"].max() <= e["f32"].max()
+    assert e["fb"].max() < 1e-5
+    assert e["fb"].max() <= e["f32"].max()
     # the FP16 table is held to the same bound; record whether it meets it
     say(f"fp16 emission table within 1e-5 of FP64: {bool(e['variant'].max() < 1e-5)}")
 

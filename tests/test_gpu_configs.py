@@ -11,10 +11,12 @@ ABI, against the CPU oracle:
      clipped to 256..4,096), 32 proposals, FP32, autotuned depths, eager and
      HIP-graph runs, sampled chains of every window;
   4. (5,000 x 4,096 x 64, mixed precision: test_gpu_parity.py
-     test_large_window_sampled_parity);
+     test_large_window_sampled_parity, and its forward-backward form:
+     test_gpu_fb.py test_fb_large_window_sampled_parity);
   5. the K-list MH update of 10,000 samples + 20,000 panel haplotypes x
      2,048 sites, K = 200 lists built on the GPU, 64 proposals, mixed
-     precision, read depth 8 (confident GLs), every chain.
+     precision (the forward-only and the forward-backward form), read depth
+     8 (confident GLs), every chain.
 
 Parents, accept decisions and FP32 log-likelihoods are compared bit for
 bit; mixed-precision log-likelihoods within 1e-12 relative."""
@@ -23,7 +25,7 @@ import sys
 import numpy as np
 import pytest
 
-from glphase_amd import PREC_F32, PREC_MIXED, PROP_KLIST, host, synth
+from glphase_amd import PREC_F32, PREC_MIXED, PREC_MIXED_FB, PROP_KLIST, host, synth
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -196,7 +198,7 @@ def test_config3_uneven_windows(orc):
 
 
 def test_config5_klist_reference_panel(orc):
-    say("config 5: K-list MH, 10000 samples + 20000 panel x 2048, K = 200, mixed")
+    say("config 5: K-list MH, 10000 samples + 20000 panel x 2048, K = 200, mixed and FB")
     import torch
 
     from glphase_amd.engine import Job, knn_lists
@@ -207,24 +209,27 @@ def test_config5_klist_reference_panel(orc):
     kl = knn_lists(panel_d, M, N, K).cpu().numpy().view(np.uint32)
     assert kl.shape == (N, K)
     win = dict(panel=p["panel"], packed=p["packed"], tran=p["tran"])
-    job = Job([win], N, p["H"], p["codebook"], p["pc"], bits=8, num_cycles=C,
-              prop_mode=PROP_KLIST, klist=kl[None], precision=PREC_MIXED)
-    job.make_plan(0)
-    job.run_batched(gen=20, pen=pen, seed=seed)
-    par, like = job.parents()[0], job.like()[0]
-    assert np.isfinite(like).all()
-    for i in range(N):
-        if i % 1000 == 0:
-            assert set(par[i]) <= set(kl[i])
     lut = orc.emit_lut(p["codebook"], p["pc"])
     mh = orc.MH(p["panel"], p["packed"], 8, lut, p["tran"], N)
-    # every chain (OpenMP over samples), on the GPU-built lists
-    po, co = mh.run(C, pen, seed, gen=20, prop_mode=PROP_KLIST, klist=kl, precision=PREC_MIXED,
-                    nthreads=8)
-    assert (par == po).all()
-    assert (np.abs(like - co) <= 1e-12 * np.abs(co)).all()
+    # both mixed forms: the forward-only pass and the forward-backward one
+    for prec in (PREC_MIXED, PREC_MIXED_FB):
+        say(f"  precision {prec}")
+        job = Job([win], N, p["H"], p["codebook"], p["pc"], bits=8, num_cycles=C,
+                  prop_mode=PROP_KLIST, klist=kl[None], precision=prec)
+        job.make_plan(0)
+        job.run_batched(gen=20, pen=pen, seed=seed)
+        par, like = job.parents()[0], job.like()[0]
+        job.close()
+        assert np.isfinite(like).all()
+        for i in range(N):
+            if i % 1000 == 0:
+                assert set(par[i]) <= set(kl[i])
+        # every chain (OpenMP over samples), on the GPU-built lists
+        po, co = mh.run(C, pen, seed, gen=20, prop_mode=PROP_KLIST, klist=kl, precision=prec,
+                        nthreads=8)
+        assert (par == po).all()
+        assert (np.abs(like - co) <= 1e-12 * np.abs(co)).all()
     # and a sample of the lists themselves against the oracle's
     rng = np.random.default_rng(8)
     rows = np.unique(np.concatenate([[0, N - 1], rng.choice(N, 200, replace=False)])).astype(np.uint32)
     assert (kl[rows] == orc.knn_rows(p["panel"], M, rows, K, nthreads=8)).all()
-    job.close()
