@@ -754,7 +754,8 @@ def main():
             f"{mpms:.3f} ms/step; the forward-backward form is {mms / ms:.2f}x faster")
 
     achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
-    workload_key = f"{N}x{M}x{C}" + (" fb" if prec == PREC_MIXED_FB else "")
+    workload_key = (f"{N}x{M}x{C}" + (" fb" if prec == PREC_MIXED_FB else "") +
+                    f" q{lanes} w{job.waves()}")
     traffic = recorded_traffic(workload_key)
     out = {
         "metric": "HMM site-evals/s (sample x proposal x site), batched MH generation",
