@@ -278,9 +278,7 @@ __device__ __forceinline__ uint32_t lut_addr(uint32_t cw, uint32_t nbw, unsigned
 // of the chunk's first site (0 at the window's site 0, which only
 // initialises, impute.cpp:339-347); one0: the chunk's first site emits 1
 // (the identity step that opens the backward half of an odd window).
-// LONG: the fast pass scales after every 32nd site instead of every 8th (the
-// forward-backward form).
-template <int BITS, bool MIXED, bool TAIL, bool FAST, bool LONG = false>
+template <int BITS, bool MIXED, bool TAIL, bool FAST>
 __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
                                           const uint32_t (&cw)[BITS], const float *tr, unsigned n,
                                           uint32_t oddm, float norm, float nrm0, float *ring,
@@ -313,7 +311,7 @@ __device__ __forceinline__ void fwd_chunk(PairState &st, const uint32_t (&h)[4],
       pair_step<MIXED, FAST>(st, e[k], tg[3 * k], tg[3 * k + 1], tg[3 * k + 2], s == 0 ? nrm0 : norm,
                        ring);
       // MIXED: every 8th site (FAST), or every site (the redo)
-      if (MIXED && (!FAST || (k == 7 && (!LONG || g == 3)))) mixed_scale<FAST>(st);
+      if (MIXED && (!FAST || k == 7)) mixed_scale<FAST>(st);
     }
   }
 }
@@ -351,7 +349,7 @@ __device__ __forceinline__ void group_load_et(GroupLds &G, const uint32_t (&cw)[
 }
 
 // Sites k0 .. k0 + 3 of group g (k0 = 0 or 4), from its loaded inputs.
-template <bool MIXED, bool FAST, bool LONG = false>
+template <bool MIXED, bool FAST>
 __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, int g, int k0,
                                             float norm, float nrm0, float *ring) {
   const float *tg = reinterpret_cast<const float *>(G.t4);
@@ -361,8 +359,7 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
                            (g == 0 && k == 0) ? nrm0 : norm, ring);
     if (MIXED && !FAST) mixed_scale<false>(st);  // the redo: every site
   }
-  // after sites 7, 15, ... (LONG: after site 31 of the chunk)
-  if (MIXED && FAST && k0 == 4 && (!LONG || g == 3)) mixed_scale<true>(st);
+  if (MIXED && FAST && k0 == 4) mixed_scale<true>(st);  // after sites 7, 15, ...
 }
 
 // A full 32-site chunk as a software pipeline over its 4 groups, in two
@@ -374,7 +371,7 @@ __device__ __forceinline__ void group_steps(PairState &st, const GroupLds &G, in
 // words hn, code words cn, tran in LDS slot trn, which this lane's part tpn
 // reaches at trnw during group 2).  LDS reads past the window's end only
 // read the tables.
-template <int BITS, bool MIXED, bool FAST, class TV, bool LONG = false>
+template <int BITS, bool MIXED, bool FAST, class TV>
 __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, GroupLds &B,
                                                const uint32_t (&h)[4], const uint32_t (&cw)[BITS],
                                                const float *tr, const uint32_t (&hn)[4],
@@ -382,27 +379,27 @@ __device__ __forceinline__ void fwd_chunk_pipe(PairState &st, GroupLds &A, Group
                                                float *trnw, TV tpn, uint32_t oddm, float norm,
                                                float nrm0, float *ring) {
   group_load_et<BITS>(B, cw, tr, 1, oddm);
-  group_steps<MIXED, FAST, LONG>(st, A, 0, 0, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, A, 0, 0, norm, nrm0, ring);
   group_load_sp(A, h, 2);
-  group_steps<MIXED, FAST, LONG>(st, A, 0, 4, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, A, 0, 4, norm, nrm0, ring);
 
   group_load_et<BITS>(A, cw, tr, 2, oddm);
-  group_steps<MIXED, FAST, LONG>(st, B, 1, 0, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, B, 1, 0, norm, nrm0, ring);
   group_load_sp(B, h, 3);
-  group_steps<MIXED, FAST, LONG>(st, B, 1, 4, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, B, 1, 4, norm, nrm0, ring);
 
   group_load_et<BITS>(B, cw, tr, 3, oddm);
-  group_steps<MIXED, FAST, LONG>(st, A, 2, 0, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, A, 2, 0, norm, nrm0, ring);
   // every lane writes (the last lanes of each part into the slot's unread
   // tail), so that the write, and with it the tran load, is not under a branch
   *reinterpret_cast<TV *>(trnw) = tpn;
   group_load_sp(A, hn, 0);
-  group_steps<MIXED, FAST, LONG>(st, A, 2, 4, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, A, 2, 4, norm, nrm0, ring);
 
   group_load_et<BITS>(A, cn, trn, 0, oddm);
-  group_steps<MIXED, FAST, LONG>(st, B, 3, 0, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, B, 3, 0, norm, nrm0, ring);
   group_load_sp(B, hn, 1);
-  group_steps<MIXED, FAST, LONG>(st, B, 3, 4, norm, nrm0, ring);
+  group_steps<MIXED, FAST>(st, B, 3, 4, norm, nrm0, ring);
 }
 
 // The global inputs of a pass are read through raw buffer loads: the
@@ -503,10 +500,7 @@ __device__ __forceinline__ void load_chunk(ChunkIn<BITS, FB> &in, unsigned c, un
 // rows in reverse order (k_fb_mirror lays them out after the forward
 // half's), starting from u = 1 with an identity first row, and for odd M an
 // identity first step (row (1, 0, 0), emission 1: one0).  Each half keeps
-// the mixed pass's exact power-of-two scaling and underflow guard, the fast
-// pass scaling after every 32nd of its steps (a sum never grows, and 32
-// steps leave a 199-binade drop to the guard); the redo scales after every
-// step.  The join runs in FP64: the
+// the mixed pass's scaling and underflow guard.  The join runs in FP64: the
 // backward pair forms g = T_n u_n (its row n, staged with its data), the
 // quad exchanges halves, and every lane of it returns
 // log((l00 g00 + l11 g11) + (l01 g01 + l10 g10)) - (kF + kB) ln 2.
@@ -597,12 +591,12 @@ __device__ __forceinline__ double forward_pass(const PassSrc &src, unsigned M,
     const float nrm0 = ch == 0 ? 0.0f : norm;
     float *slot = wtr + SLOT * (ch & 1u), *slotn = wtr + SLOT * ((ch + 1u) & 1u);
     if (n == 32)
-      fwd_chunk_pipe<BITS, MIXED, FAST, TV, FB>(st, A, B, cur.h, cur.cw, slot + roff, nxt.h,
-                                                nxt.cw, slotn + roff, slotn + woff, nxt.tp, oddm,
-                                                norm, nrm0, ring);
+      fwd_chunk_pipe<BITS, MIXED, FAST, TV>(st, A, B, cur.h, cur.cw, slot + roff, nxt.h, nxt.cw,
+                                            slotn + roff, slotn + woff, nxt.tp, oddm, norm, nrm0,
+                                            ring);
     else  // the window's last chunk, shorter than 32 sites
-      fwd_chunk<BITS, MIXED, true, FAST, FB>(st, cur.h, cur.cw, slot + roff, n, oddm, norm, nrm0,
-                                             ring, FB && fb.one0 && ch == 0u);
+      fwd_chunk<BITS, MIXED, true, FAST>(st, cur.h, cur.cw, slot + roff, n, oddm, norm, nrm0,
+                                         ring, FB && fb.one0 && ch == 0u);
     if (!MIXED) flush_ring(st, ring);
   };
   for (unsigned ch = 0; ch < nch; ch += 2u) {

@@ -173,7 +173,7 @@ def hmm_like_fb(haps, parents, emit, tran):
 
 
 def hmm_like_fb_pass(haps, parents, emit, tran, every):
-    """One forward-backward pass (both halves scaled every 32nd step, or every
+    """One forward-backward pass (both halves scaled every 8th step or every
     step) and whether either half flagged a tiny or non-finite sum."""
     h = np.ascontiguousarray(haps, np.uint64)
     pr = np.ascontiguousarray(parents, np.uint32)

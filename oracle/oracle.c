@@ -275,10 +275,8 @@ double orc_hmm_like_mixed(const uint64_t *haps, unsigned wn,
  * 1 (step j at site M-1+p-j with tran row M+p-j, p = M mod 2; the first
  * step has tran row (1, 0, 0), and for odd M its emission is 1, an exact
  * identity step).  Each half starts scaled by 2^100 and scales by powers of
- * two, as the mixed pass does, after every 32nd of its own steps (every =
- * 32: a sum never grows, so only a drop of 199 binades within 32 steps can
- * reach the guard) or after every step (every = 1).  The halves join in
- * FP64:
+ * two after every 8th of its own steps (every = 8) or after every step
+ * (every = 1), as the mixed pass does.  The halves join in FP64:
  *   g = T_n u_n   (tran row n, or the identity row when n = M, i.e. M = 1),
  *   L = (l00 g00 + l11 g11) + (l01 g01 + l10 g10),
  * and the score is log(L) - (kF + kB) ln 2.  *tiny is set when either half
@@ -317,7 +315,7 @@ static void fb_half(const uint64_t *const par[4], const float *emit, const float
     l01 = b01 * e01;
     l10 = b10 * e10;
     l11 = b11 * e11;
-    if (every == 1 || (j & (every - 1)) == every - 1) {
+    if (every == 1 || (j & 7u) == 7u) {
       (void)frexpf((l00 + l11) + (l01 + l10), &ex);
       const int k = 101 - ex;
       if (k >= 200)
@@ -366,13 +364,13 @@ double orc_hmm_like_fb_pass(const uint64_t *haps, unsigned wn, const unsigned P[
   return log(de + dd) - (double)(kf + kb) * ORC_LN2;
 }
 
-/* The forward-backward form: both halves scaled every 32nd step; when either
+/* The forward-backward form: both halves scaled every 8th step; when either
  * flagged a sum below 2^-99 or a last sum not finite, or the result is NaN,
  * both scaled after every step. */
 double orc_hmm_like_fb(const uint64_t *haps, unsigned wn, const unsigned P[4],
                        const float *emit, const float *tran, unsigned M) {
   int tiny = 0;
-  const double v = orc_hmm_like_fb_pass(haps, wn, P, emit, tran, M, 32, &tiny);
+  const double v = orc_hmm_like_fb_pass(haps, wn, P, emit, tran, M, 8, &tiny);
   if (!tiny && v == v)
     return v;
   return orc_hmm_like_fb_pass(haps, wn, P, emit, tran, M, 1, &tiny);

@@ -3,8 +3,8 @@ runs as a forward half and a mirrored backward half on one lane quad,
 joined in FP64.  Chosen parents and accept decisions must equal the oracle's
 (orc_hmm_like_fb fed the same Philox streams) exactly; log-likelihoods agree
 within 1e-12 relative (device vs host libm log).  Site counts cover odd
-windows (the backward half's identity first step), the 32-step scaling
-and chunk boundaries of each half, and the layout's word edges."""
+windows (the backward half's identity first step), the 8-site scaling and
+32-site chunk boundaries of each half, and the layout's word edges."""
 import sys
 
 import numpy as np
@@ -146,9 +146,9 @@ def test_fb_klist_and_kickstart(orc):
 @pytest.mark.parametrize("hn", [10_000, 40_000])
 def test_fb_smallest_emissions(orc, hn):
     # runs of sites where every state emits mu^2 at hn haplotypes' mutation
-    # rate (in both halves of the window): a half's sum falls below 2^-99
-    # within its 32-step scaling interval, the wavefront redoes its passes
-    # scaled every step, and the result equals the oracle's, finite
+    # rate (in both halves of the window): the 8-step scaling of a half
+    # falls below 2^-99, the wavefront redoes its passes scaled every step,
+    # and the result equals the oracle's, finite
     say(f"fb smallest emissions hn={hn}")
     p = helpers.hom_alt_run_problem(hn, seed=hn % 89, M=300, runs=((20, 12), (150, 9), (262, 30)))
     C, pen, seed = 20, 1.0, 4243

@@ -165,7 +165,7 @@ def test_fb_planted_mistakes_fail(orc, M):
 
 @pytest.mark.parametrize("hn", [10_000, 40_000])
 def test_fb_smallest_emissions_match_fp64(orc, hn):
-    # mu^2 runs in both halves: the 32-step pass flags the sum below 2^-99,
+    # mu^2 runs in both halves: the 8-step pass flags the sum below 2^-99,
     # the result comes from the pass scaled after every step, within 1e-7
     # of FP64
     from tests import helpers
@@ -177,9 +177,9 @@ def test_fb_smallest_emissions_match_fp64(orc, hn):
         P = rng.choice([h for h in range(p["H"]) if h // 2 != i], 4).astype(np.uint32)
         emit = orc.sample_emit(p["packed"], i, p["M"], p["bits"], lut)
         f64 = orc.hmm_like_f64(p["panel"], P, emit, p["tran"])
-        v32, tiny32 = orc.hmm_like_fb_pass(p["panel"], P, emit, p["tran"], 32)
+        v8, tiny8 = orc.hmm_like_fb_pass(p["panel"], P, emit, p["tran"], 8)
         v1, tiny1 = orc.hmm_like_fb_pass(p["panel"], P, emit, p["tran"], 1)
-        assert tiny32 and not tiny1
+        assert tiny8 and not tiny1
         got = orc.hmm_like_fb(p["panel"], P, emit, p["tran"])
         assert got == v1
         assert abs(got - f64) <= 1e-7 * abs(f64)
@@ -188,6 +188,6 @@ def test_fb_smallest_emissions_match_fp64(orc, hn):
 def test_fb_guard_keeps_ordinary_passes(orc):
     haps, emit, tran = problem(2001, seed=7)
     P = np.array([1, 6, 3, 2], np.uint32)
-    v32, tiny = orc.hmm_like_fb_pass(haps, P, emit, tran, 32)
+    v8, tiny = orc.hmm_like_fb_pass(haps, P, emit, tran, 8)
     assert not tiny
-    assert orc.hmm_like_fb(haps, P, emit, tran) == v32
+    assert orc.hmm_like_fb(haps, P, emit, tran) == v8
