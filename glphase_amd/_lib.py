@@ -43,6 +43,7 @@ class GlpWorkWindow(ctypes.Structure):
         ("parents", ctypes.c_void_p),
         ("tran", ctypes.c_void_p),
         ("het_aa", ctypes.c_void_p),
+        ("emit", ctypes.c_void_p),
         ("hnew", ctypes.c_void_p),
     ]
 
@@ -147,6 +148,8 @@ SIGNATURES = [
                                                 ctypes.c_uint64, ctypes.c_void_p]),
     ("glp_work_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(GlpWorkDesc)]),
     ("glp_hmm_work", ctypes.c_int, [ctypes.POINTER(GlpWorkDesc), ctypes.c_void_p]),
+    ("glp_work_emit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, c_f32p,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
     ("glp_selftest_recip", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(ctypes.c_uint32)]),

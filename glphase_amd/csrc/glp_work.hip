@@ -45,7 +45,6 @@ __device__ __forceinline__ unsigned hbit64(uint64_t w, unsigned m) {
   return (unsigned)(w >> (m & 63u)) & 1u;
 }
 
-constexpr int kWorkBlock = 8;  // sites per prefetch block
 
 // Genotype choice of impute.cpp:472-484 from the weights and the site's
 // uniform word.
@@ -65,45 +64,13 @@ __device__ __forceinline__ void load_pc_rows(float4 *s_pc, const WorkArgs &a) {
   __syncthreads();
 }
 
-// One block of the recursions: sites 8 blk .. 8 blk + 7 (aligned, so a block
-// lies inside one 64-site allele word), their GL pairs and transitions, and
-// the lane's two parent words of that 64-site word.
-struct WorkBlock {
-  float2 q[kWorkBlock];
-  float t[3 * kWorkBlock];
-  uint64_t wx, wy;
-};
-
-// Loads block blk's inputs.  Site indices are clamped into [0, hi] (the
-// window), so every load goes to a valid address and none sits behind a
-// branch: one wait then covers a whole block, a block ahead.  Sites of the
-// block outside the window get a copy of a real site's inputs, which the
-// recursions never use.  The parent words come first, since a block's
-// recursion needs them before its sums.
-__device__ __forceinline__ void load_block(WorkBlock &b, int blk, int hi, gptr_t<float> q2,
-                                           gptr_t<float> tran, gptr_t<uint64_t> rx,
-                                           gptr_t<uint64_t> ry, unsigned N, unsigned i) {
-  const int top = hi / kWorkBlock;  // the window's last block
-  const int cb = blk < 0 ? 0 : (blk > top ? top : blk);
-  const unsigned word = (unsigned)(kWorkBlock * cb) >> 6;
-  b.wx = rx[word];
-  b.wy = ry[word];
-#pragma unroll
-  for (int k = 0; k < kWorkBlock; ++k) {
-    const int m = kWorkBlock * cb + k;
-    const int mc = m > hi ? hi : m;
-    b.q[k] = load_f2(q2 + 2 * ((size_t)mc * N + i));
-    b.t[3 * k] = tran[3 * mc];
-    b.t[3 * k + 1] = tran[3 * mc + 1];
-    b.t[3 * k + 2] = tran[3 * mc + 2];
-  }
-}
-
 // quad_perm DPP moves within each group of 4 lanes (one sample's 4 states,
-// lane s = state s = 2 x + y for parents f_x, m_y)
+// lane s = state s = 2 x + y for parents f_x, m_y).  Every lane of a
+// quad_perm move has a source lane, so the move needs no old value and the
+// compiler folds it into the consuming add or multiply.
 template <int CTRL>
 __device__ __forceinline__ float qperm(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 constexpr int kQSwap1 = 0xB1;  // [1,0,3,2]
 constexpr int kQSwap2 = 0x4E;  // [2,3,0,1]
@@ -111,21 +78,24 @@ constexpr int kQOpp = 0x1B;    // [3,2,1,0]: state 3 - s
 constexpr int kQ0 = 0x00, kQ1 = 0x55, kQ2 = 0xAA, kQ3 = 0xFF;  // broadcasts of lane 0..3
 
 // ((x0 + x1) + x2) + x3 of the quad's four values, in every lane: the
-// reference's order of a 4-state sum.
+// reference's order of a 4-state sum.  The four broadcasts are separate
+// moves of x (a DPP operation whose other operand a VALU has just written
+// waits two wait states; moves of x wait once, for x, and the three sums
+// are plain adds).  The two wait states x needs are taken in the same block.
 __device__ __forceinline__ float quad_sum(float x) {
-  float s = qperm<kQ0>(x) + qperm<kQ1>(x);
-  s = s + qperm<kQ2>(x);
-  return s + qperm<kQ3>(x);
+  float a, b, c, d;
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_mov_b32_dpp %0, %4 quad_perm:[0,0,0,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_mov_b32_dpp %1, %4 quad_perm:[1,1,1,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_mov_b32_dpp %2, %4 quad_perm:[2,2,2,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_mov_b32_dpp %3, %4 quad_perm:[3,3,3,3] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+      : "v"(x));
+  float s = a + b;
+  s = s + c;
+  return s + d;
 }
-
-// One lane's view of a sample for the two recursions.
-struct WorkLane {
-  gptr_t<uint64_t> rx, ry;  // this state's parents f_x, m_y
-  gptr_t<float> q2, tran;
-  float *out;               // [M][N] x 4: beta (backward) or scaled l (forward)
-  const float4 *s_pc;       // rows pc[s][0..3] in LDS
-  unsigned M, N, i, st;
-};
 
 // 1.0f / x: the IEEE division, or (FAST) the 3-instruction reciprocal, which
 // equals it for normal x up to 2^126 and is NaN below (and 0 above) -- a NaN
@@ -135,172 +105,430 @@ __device__ __forceinline__ float work_recip(float x) {
   return FAST ? recip_normal(x) : 1.0f / x;
 }
 
-// Backward step over block blk (sites 8 blk + 7 down to 8 blk, those in
-// [1, M-1]) of impute.cpp:398-416: site m turns beta_m into beta_{m-1},
-// written to the workspace.
-template <bool FAST>
-__device__ __forceinline__ void bwd_block(const WorkLane &L, const WorkBlock &cur, int blk,
-                                          float &b) {
-  constexpr int B = kWorkBlock;
-  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
-  float e[B];
+// ---- the emission table (k_work_emit) ----
+
+// Insti::initialize's emissions (insti.cpp:1345-1378) for every (site,
+// sample) of a window: the four pc rows against the genotype probabilities
+// of the GL pair, one float4 per (site, sample), site-major.
+__global__ void __launch_bounds__(256)
+    k_work_emit(const float2 *__restrict__ het_aa, size_t n, float4 *__restrict__ emit,
+                WorkArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float3 p = work_prob(load_f2(gmem(reinterpret_cast<const float *>(het_aa + t))));
+  float e[4];
 #pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const unsigned m = (unsigned)(B * blk + k);
-    const unsigned row = (hbit64(cur.wx, m) << 1) | hbit64(cur.wy, m);
-    e[k] = work_emit(L.s_pc[row], work_prob(cur.q[k]));
+  for (int r = 0; r < 4; ++r)
+    e[r] = work_emit(make_float4(a.pc[4 * r], a.pc[4 * r + 1], a.pc[4 * r + 2], a.pc[4 * r + 3]),
+                     p);
+  store_f4(reinterpret_cast<float *>(emit + t), e[0], e[1], e[2], e[3]);
+}
+
+// ---- the two recursions (k_hmm_work_pass) ----
+//
+// Sites go in aligned blocks of 32 (half a 64-site panel word).  Each lane
+// gathers its state's emission at every site of a block from the window's
+// emission table -- the float of the row its two parental alleles select --
+// a whole block ahead, into the other of two register buffers; the parents'
+// allele bits that pick the rows arrive a block before that.  The window's
+// transitions are staged through LDS in chunks of 256 sites,
+// double-buffered with one workgroup barrier per chunk: the forward
+// recursion reads, per site and state, the coefficient vector
+// (t[popcount(s ^ j)], j = 0..3) in one LDS read, the backward one
+// (t0, t1, t2).  The emission table and the output tables are reached
+// through buffer descriptors, rebased per block, with the site's row offset
+// in the scalar offset.
+constexpr int kPassB = 32;                     // sites per block
+constexpr int kPassChunk = 256;                // sites per staged chunk of transitions
+constexpr int kPassBPC = kPassChunk / kPassB;  // blocks per chunk
+constexpr int kPassWaves = 2;                  // wavefronts per workgroup, one recursion
+constexpr int kPassThreads = 64 * kPassWaves;
+constexpr int kPassStage = kPassChunk / kPassThreads;  // chunk sites staged per lane
+static_assert(kPassChunk % kPassThreads == 0, "chunk sites split evenly over the lanes");
+// the largest sample count whose per-block offsets stay below 2^31 bytes
+constexpr unsigned kPassMaxSamples = 1u << 22;
+
+typedef __amdgpu_buffer_rsrc_t buf_rsrc;
+
+// Raw (untyped, unswizzled) buffer over device memory from byte address
+// base, the same in every lane (read from the first, so the descriptor is
+// scalar); word 3 selects 32-bit data, the record count covers every offset
+// below 2^31.
+__device__ __forceinline__ buf_rsrc raw_buffer(uintptr_t base) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc(
+      reinterpret_cast<void *>(((uintptr_t)hi << 32) | lo), (short)0, 0x7fffffff, 0x00020000);
+}
+
+// One lane's view of a sample for one recursion.
+struct PassLane {
+  uintptr_t emit, out;       // byte addresses: emission table and output table, [M][N] float4
+  uint32_t row;              // bytes per site row of both: 16 N
+  uint32_t e_off, out_off;   // the lane's offsets in a row: 16 i, 16 i + 4 st
+  gptr_t<uint32_t> px, py;   // parents f_x, m_y: allele bits, 32 sites per word
+  int M;
+};
+
+struct PassBlock {
+  float e[kPassB];  // the lane's emission at the block's sites
+};
+
+// Issues the gathers of block blk, whose parents' allele bits are (x, y).
+// FULL: all 32 sites lie in the window; otherwise site indices are clamped
+// to M - 1 (with the row bits of the unclamped site: any row of a valid
+// site is a valid address), so every gather reads inside the table.
+template <bool FULL>
+__device__ __forceinline__ void pass_gather(PassBlock &b, const PassLane &L, int blk, uint32_t x,
+                                            uint32_t y) {
+  const int m0 = kPassB * blk;
+  const buf_rsrc r = raw_buffer(L.emit + (uintptr_t)m0 * L.row);
+  const int last = L.M - 1 - m0;
+#pragma unroll
+  for (int k = 0; k < kPassB; ++k) {
+    const int kc = FULL ? k : (k < last ? k : last);
+    const uint32_t off = L.e_off + (((x >> k) & 1u) << 3) + (((y >> k) & 1u) << 2);
+    b.e[k] = __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, kc * (int)L.row, 0));
   }
+}
+
+__device__ __forceinline__ void pass_gather_any(PassBlock &b, const PassLane &L, int blk,
+                                                uint32_t x, uint32_t y) {
+  if (kPassB * blk + kPassB <= L.M)
+    pass_gather<true>(b, L, blk, x, y);
+  else
+    pass_gather<false>(b, L, blk, x, y);
+}
+
+__device__ __forceinline__ void pass_copy(PassBlock &dst, const PassBlock &src) {
 #pragma unroll
-  for (int k = B - 1; k >= 0; --k) {
-    const int mi = B * blk + k;
-    if (mi > (int)M - 1 || mi < 1) continue;
-    const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-    const float bb = b * e[k];
+  for (int k = 0; k < kPassB; ++k) dst.e[k] = src.e[k];
+}
+
+__device__ __forceinline__ void pass_store(float v, const buf_rsrc &r, const PassLane &L, int k) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)L.out_off,
+                                        k * (int)L.row, 0);
+}
+
+// Forward steps over block blk (impute.cpp:418-455); co: the block's first
+// site's coefficient vector for this lane's state (site k at co[4 k]).  Each
+// site's row of the table takes the scaled l.  FULL: sites 1 .. M-1 only;
+// otherwise site 0 starts the recursion and sites past M - 1 are skipped.
+template <bool FAST, bool FULL>
+__device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
+                                               const float4 *co, float &l) {
+  const int m0 = kPassB * blk;
+  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)m0 * L.row);
+  float4 cn = co[0];  // each site's LDS read is issued a site ahead of its use
+#pragma unroll
+  for (int k = 0; k < kPassB; ++k) {
+    const int m = m0 + k;
+    if (!FULL && m >= L.M) break;
+    const float4 c = cn;
+    if (k + 1 < kPassB) cn = co[4 * (k + 1)];
+    __builtin_amdgcn_sched_barrier(0);
+    if (FULL || m) {
+      const float n = qperm<kQ0>(l) * c.x + qperm<kQ1>(l) * c.y + qperm<kQ2>(l) * c.z +
+                      qperm<kQ3>(l) * c.w;
+      l = n * b.e[k];
+    } else {
+      l = 0.25f * b.e[k];
+    }
+    l *= work_recip<FAST>(quad_sum(l));
+    pass_store(l, r, L, k);
+  }
+}
+
+// Backward steps over block blk, sites 32 blk + 31 down to 32 blk
+// (impute.cpp:398-416); tr: the block's first site's (t0, t1, t2).  Site m
+// turns beta_m into beta_{m-1}, written to row m - 1.  FULL: every site in
+// 1 .. M-1; otherwise the others are skipped.
+template <bool FAST, bool FULL>
+__device__ __forceinline__ void bwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
+                                               const float4 *tr, float &v) {
+  const int m0 = kPassB * blk;
+  // rows m0 - 1 + k; in block 0 no site k = 0 is stored, so the descriptor's
+  // base may lie one row before the table
+  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)m0 * L.row - L.row);
+  float4 tn = tr[kPassB - 1];  // each site's LDS read is issued a site ahead of its use
+#pragma unroll
+  for (int k = kPassB - 1; k >= 0; --k) {
+    const int m = m0 + k;
+    const float4 t = tn;
+    if (k > 0) tn = tr[k - 1];
+    __builtin_amdgcn_sched_barrier(0);
+    if (!FULL && (m > L.M - 1 || m < 1)) continue;
+    const float bb = v * b.e[k];
     // bb_s t0 + (pair-cross sum) t1 + bb_{3-s} t2; the cross sum is
     // bb01 + bb10 for s = 00, 11 and bb00 + bb11 for s = 01, 10
     const float cross = qperm<kQSwap1>(bb) + qperm<kQSwap2>(bb);
-    const float n = bb * t0 + cross * t1 + qperm<kQOpp>(bb) * t2;
-    const float sum = work_recip<FAST>(quad_sum(n));
-    b = n * sum;
-    L.out[4 * ((size_t)(mi - 1) * N + i) + st] = b;
+    const float n = bb * t.x + cross * t.y + qperm<kQOpp>(bb) * t.z;
+    v = n * work_recip<FAST>(quad_sum(n));
+    pass_store(v, r, L, k);
   }
 }
 
-// Backward recursion (impute.cpp:398-416) over the window, blocks from the
-// last to the first, each block's inputs loaded a block ahead into the
-// other of two buffers (no copy between them, so no wait for the load just
-// issued).  Returns whether the last beta is a proper one (its quad sum
-// positive), which fails exactly when a FAST reciprocal met a subnormal,
-// zero or huge sum.
-template <bool FAST>
-__device__ __forceinline__ bool work_backward(const WorkLane &L) {
-  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
-  const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
-  float b = 1;
-  L.out[4 * ((size_t)(M - 1) * N + i) + st] = 1.0f;
-  WorkBlock x, y;
-  load_block(x, top_blk, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-  for (int blk = top_blk;;) {
-    load_block(y, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-    bwd_block<FAST>(L, x, blk, b);
-    if (--blk < 0) break;
-    load_block(x, blk - 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-    bwd_block<FAST>(L, y, blk, b);
-    if (--blk < 0) break;
+// Transitions of one chunk held in registers between their load and their
+// LDS write: rows min(256 ch + s, M - 1) for the lane's sites s.
+struct PassStage {
+  float t[kPassStage][3];
+};
+
+__device__ __forceinline__ void stage_load(PassStage &S, gptr_t<float> tran, int ch, int M) {
+#pragma unroll
+  for (int r = 0; r < kPassStage; ++r) {
+    const int m = kPassChunk * ch + (int)threadIdx.x + kPassThreads * r;
+    const int mc = m < M - 1 ? m : M - 1;
+    S.t[r][0] = tran[3 * mc];
+    S.t[r][1] = tran[3 * mc + 1];
+    S.t[r][2] = tran[3 * mc + 2];
   }
-  return quad_sum(b) > 0.0f;
 }
 
-// Forward step over block blk (sites 8 blk .. 8 blk + 7, those below M) of
-// impute.cpp:418-455; each site's slot of the forward table takes the
-// scaled state probability l.
-template <bool FAST>
-__device__ __forceinline__ void fwd_block(const WorkLane &L, const WorkBlock &cur, int blk,
-                                          float &l) {
-  constexpr int B = kWorkBlock;
-  const unsigned M = L.M, N = L.N, i = L.i, st = L.st;
-  // forward coefficient of state j's value in this state's update:
-  // t[popcount(st ^ j)] (impute.cpp:435-445)
-  const bool own[4] = {st == 0, st == 1, st == 2, st == 3};
-  const bool opp[4] = {st == 3, st == 2, st == 1, st == 0};
-  float e[B];
+// Writes a chunk into LDS: forward, per site the four states' coefficient
+// vectors (state s: t[popcount(s ^ j)] for j = 0..3, impute.cpp:435-445);
+// backward, per site (t0, t1, t2, 0).
+template <bool FWD>
+__device__ __forceinline__ void stage_write(const PassStage &S, float4 *buf) {
 #pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const unsigned m = (unsigned)(B * blk + k);
-    const unsigned row = (hbit64(cur.wx, m) << 1) | hbit64(cur.wy, m);
-    e[k] = work_emit(L.s_pc[row], work_prob(cur.q[k]));
-  }
-#pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const unsigned m = (unsigned)(B * blk + k);
-    if (m >= M) break;
-    if (m) {
-      const float t0 = cur.t[3 * k], t1 = cur.t[3 * k + 1], t2 = cur.t[3 * k + 2];
-      float c[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) c[j] = own[j] ? t0 : (opp[j] ? t2 : t1);
-      const float n = qperm<kQ0>(l) * c[0] + qperm<kQ1>(l) * c[1] + qperm<kQ2>(l) * c[2] +
-                      qperm<kQ3>(l) * c[3];
-      l = n * e[k];
+  for (int r = 0; r < kPassStage; ++r) {
+    const int s = (int)threadIdx.x + kPassThreads * r;
+    float t0 = S.t[r][0], t1 = S.t[r][1], t2 = S.t[r][2];
+    // the values are taken here, a chunk after their loads: no copy of them
+    // may be scheduled next to the loads (it would wait for them there)
+    asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2));
+    if (FWD) {
+      buf[4 * s] = make_float4(t0, t1, t1, t2);
+      buf[4 * s + 1] = make_float4(t1, t0, t2, t1);
+      buf[4 * s + 2] = make_float4(t1, t2, t0, t1);
+      buf[4 * s + 3] = make_float4(t2, t1, t1, t0);
     } else {
-      l = 0.25f * e[k];
+      buf[s] = make_float4(t0, t1, t2, 0.0f);
     }
-    const float sum = work_recip<FAST>(quad_sum(l));
-    l *= sum;
-    L.out[4 * ((size_t)m * N + i) + st] = l;
   }
 }
 
-// Forward recursion (impute.cpp:418-455), sites 0 .. M-1, with the same two
-// buffers as work_backward.  It needs no beta, so it runs beside the
-// backward recursion.  Returns whether the last state is a proper one.
+// Workgroup barrier that orders LDS accesses only: global loads and stores
+// in flight stay in flight (a full __syncthreads waits for all of them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Moves the workgroup onto chunk ch: writes its staged transitions into LDS
+// buffer ch & 1 (last read two chunks ago, before the previous barrier),
+// waits for every wavefront, and loads chunk next (if in the window).
+template <bool FWD>
+__device__ __forceinline__ void stage_enter(PassStage &S, float4 *stage, gptr_t<float> tran,
+                                            int ch, int next, int nch, int M) {
+  stage_write<FWD>(S, stage + (FWD ? 4 : 1) * kPassChunk * (ch & 1));
+  lds_barrier();
+  if (next >= 0 && next < nch) stage_load(S, tran, next, M);
+}
+
+// Forward recursion (impute.cpp:418-455), sites 0 .. M-1.  Returns whether
+// the last state is a proper one (its quad sum positive), which fails
+// exactly when a FAST reciprocal met a subnormal, zero or huge sum.  Every
+// wavefront of the workgroup runs the same chunk steps and barriers; idle
+// ones (no sample) skip the loads and steps.  At each block the gathers of
+// the next block are issued first, then the allele bits of the one after,
+// then the block's steps.
 template <bool FAST>
-__device__ __forceinline__ bool work_forward(const WorkLane &L) {
-  const unsigned M = L.M, N = L.N, i = L.i;
-  const int hi = (int)M - 1, top_blk = hi / kWorkBlock;
+__device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran, float4 *stage,
+                             int st) {
+  const int M = L.M, top = (M - 1) / kPassB, nch = top / kPassBPC + 1;
+  PassStage S;
+  stage_load(S, tran, 0, M);
+  stage_enter<true>(S, stage, tran, 0, 1, nch, M);
+  auto co = [&](int blk) {
+    const int ch = blk / kPassBPC;
+    return stage + 4 * (kPassChunk * (ch & 1) + kPassB * (blk % kPassBPC)) + st;
+  };
+  auto enter = [&](int blk) {
+    if (blk % kPassBPC == 0) {
+      const int ch = blk / kPassBPC;
+      stage_enter<true>(S, stage, tran, ch, ch + 1, nch, M);
+    }
+  };
   float l = 0;
-  WorkBlock x, y;
-  load_block(x, 0, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-  for (int blk = 0;;) {
-    load_block(y, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-    fwd_block<FAST>(L, x, blk, l);
-    if (++blk > top_blk) break;
-    load_block(x, blk + 1, hi, L.q2, L.tran, L.rx, L.ry, N, i);
-    fwd_block<FAST>(L, y, blk, l);
-    if (++blk > top_blk) break;
+  PassBlock A, B;
+  uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
+  if (active) {
+    const uint32_t x0 = L.px[0], y0 = L.py[0];
+    if (top >= 1) {
+      bx = L.px[1];
+      by = L.py[1];
+    }
+    pass_gather_any(A, L, 0, x0, y0);
   }
-  return quad_sum(l) > 0.0f;
+  if (top > 0) {
+    if (active) {
+      pass_gather_any(B, L, 1, bx, by);
+      if (top >= 2) {
+        bx = L.px[2];
+        by = L.py[2];
+      }
+      fwd_pass_block<FAST, false>(L, A, 0, co(0), l);
+    }
+    int blk = 1;  // in B
+    enter(blk);
+    for (;;) {
+      if (blk == top) {
+        if (active) pass_copy(A, B);
+        break;
+      }
+      if (active) {
+        pass_gather_any(A, L, blk + 1, bx, by);
+        if (blk + 2 <= top) {
+          bx = L.px[blk + 2];
+          by = L.py[blk + 2];
+        }
+        fwd_pass_block<FAST, true>(L, B, blk, co(blk), l);
+      }
+      enter(++blk);  // in A
+      if (blk == top) break;
+      if (active) {
+        pass_gather_any(B, L, blk + 1, bx, by);
+        if (blk + 2 <= top) {
+          bx = L.px[blk + 2];
+          by = L.py[blk + 2];
+        }
+        fwd_pass_block<FAST, true>(L, A, blk, co(blk), l);
+      }
+      enter(++blk);  // in B
+    }
+  }
+  if (active) fwd_pass_block<FAST, false>(L, A, top, co(top), l);
+  return !active || quad_sum(l) > 0.0f;
 }
 
-// The two recursions of impute.cpp:398-470, side by side: one lane per
-// (window, sample, state, recursion); even wavefronts run the backward
-// recursion of 16 samples, odd ones the forward recursion of the same 16
-// (each wavefront takes one code path).  Lane s of a quad holds state s's
-// value, the cross-state terms come by quad DPP moves in the reference's
-// operand order, and the four lanes compute each site's normaliser
-// together.  The backward recursion writes beta to the first table of the
-// window's workspace, the forward one the scaled l to the second ([M][N]
-// float4 each, site-major: a quad stores one float4, a wavefront 16
-// samples' contiguous float4s); the sampling step multiplies them into the
-// posteriors l * beta, the reference's product.  Sites go in aligned blocks
-// of 8 whose GL pairs, transitions and parent words are loaded a block
-// ahead.  A recursion runs with the short reciprocal first; a wavefront in
-// which some sample met a sum outside its exact range redoes it with the
-// IEEE division, so every stored value is the division's.  Parent indices
-// past the panel are clamped (the MH update never produces them).
-__global__ void __launch_bounds__(256)
+// Backward recursion (impute.cpp:398-416), sites M-1 down to 1, with the
+// same buffers and chunk steps in descending order.
+template <bool FAST>
+__device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran, float4 *stage) {
+  const int M = L.M, top = (M - 1) / kPassB, nch = top / kPassBPC + 1;
+  PassStage S;
+  stage_load(S, tran, nch - 1, M);
+  stage_enter<false>(S, stage, tran, nch - 1, nch - 2, nch, M);
+  auto tr = [&](int blk) {
+    const int ch = blk / kPassBPC;
+    return stage + kPassChunk * (ch & 1) + kPassB * (blk % kPassBPC);
+  };
+  auto enter = [&](int blk) {
+    if (blk % kPassBPC == kPassBPC - 1) {
+      const int ch = blk / kPassBPC;
+      stage_enter<false>(S, stage, tran, ch, ch - 1, nch, M);
+    }
+  };
+  float v = 1;
+  PassBlock A, B;
+  uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
+  if (active) {
+    pass_store(1.0f, raw_buffer(L.out + (uintptr_t)(M - 1) * L.row), L, 0);
+    const uint32_t xt = L.px[top], yt = L.py[top];
+    if (top >= 1) {
+      bx = L.px[top - 1];
+      by = L.py[top - 1];
+    }
+    pass_gather_any(A, L, top, xt, yt);
+  }
+  if (top > 0) {
+    if (active) {
+      pass_gather<true>(B, L, top - 1, bx, by);
+      if (top >= 2) {
+        bx = L.px[top - 2];
+        by = L.py[top - 2];
+      }
+      bwd_pass_block<FAST, false>(L, A, top, tr(top), v);
+    }
+    int blk = top - 1;  // in B
+    enter(blk);
+    for (;;) {
+      if (blk == 0) {
+        if (active) pass_copy(A, B);
+        break;
+      }
+      if (active) {
+        pass_gather<true>(A, L, blk - 1, bx, by);
+        if (blk >= 2) {
+          bx = L.px[blk - 2];
+          by = L.py[blk - 2];
+        }
+        bwd_pass_block<FAST, true>(L, B, blk, tr(blk), v);
+      }
+      enter(--blk);  // in A
+      if (blk == 0) break;
+      if (active) {
+        pass_gather<true>(B, L, blk - 1, bx, by);
+        if (blk >= 2) {
+          bx = L.px[blk - 2];
+          by = L.py[blk - 2];
+        }
+        bwd_pass_block<FAST, true>(L, A, blk, tr(blk), v);
+      }
+      enter(--blk);  // in B
+    }
+  }
+  if (active) bwd_pass_block<FAST, false>(L, A, 0, tr(0), v);
+  return !active || quad_sum(v) > 0.0f;
+}
+
+// The two recursions of impute.cpp:398-470, side by side: workgroups of
+// even blockIdx.x run the backward recursion of 16 kPassWaves samples, odd
+// ones the forward recursion of the same samples; one lane per (sample,
+// state), lane s of a quad holding state s's value.  The cross-state terms come by quad DPP
+// moves in the reference's operand order, and the four lanes compute each
+// site's normaliser together.  The backward recursion writes beta to the
+// first table of the window's workspace, the forward one the scaled l to
+// the second ([M][N] float4 each, site-major: a quad stores one float4, a
+// wavefront 16 samples' contiguous float4s); the sampling step multiplies
+// them into the posteriors l * beta, the reference's product.  A workgroup
+// runs its recursion with the short reciprocal first; if some sample met a
+// sum outside its exact range, the workgroup redoes it with the IEEE
+// division, so every stored value is the division's.  Lanes past the last
+// sample repeat the last sample's work and store its (identical) values.
+// Parent indices past the panel are clamped (the MH update never produces
+// them).
+__global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
     k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ ws_off,
                     float4 *__restrict__ ws, WorkArgs a) {
-  __shared__ float4 s_pc[4];  // rows pc[s][0..3]
-  load_pc_rows(s_pc, a);
+  __shared__ float4 s_stage[2 * 4 * kPassChunk];
   const unsigned w = blockIdx.y;
-  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned wave = g >> 6, fwd = wave & 1u;
-  const unsigned i = (wave >> 1) * 16u + ((g & 63u) >> 2), st = g & 3u;
-  if (i >= a.N) return;  // whole quads leave together
+  const bool fwd = blockIdx.x & 1u;
+  const unsigned lane = threadIdx.x & 63u;
+  // the wavefront's first sample (uniform: the branches on it are too)
+  const unsigned i0 = __builtin_amdgcn_readfirstlane(
+      ((blockIdx.x >> 1) * kPassWaves + (threadIdx.x >> 6)) * 16u);
+  const bool active = i0 < a.N;
+  const unsigned i = min(i0 + (lane >> 2), a.N - 1), st = lane & 3u;
   const glp_work_window W = wins[w];
-  WorkLane L;
-  L.M = W.num_sites;
-  L.N = a.N;
-  L.i = i;
-  L.st = st;
-  L.out = reinterpret_cast<float *>(ws + ws_off[w] + (fwd ? (size_t)L.M * a.N : 0));
-  L.q2 = gmem(W.het_aa);
-  L.tran = gmem(W.tran);
-  L.s_pc = s_pc;
+  PassLane L;
+  L.M = (int)W.num_sites;
+  L.emit = reinterpret_cast<uintptr_t>(W.emit);
+  L.out = reinterpret_cast<uintptr_t>(ws + ws_off[w] + (fwd ? (size_t)W.num_sites * a.N : 0));
+  L.row = 16u * a.N;
+  L.e_off = 16u * i;
+  L.out_off = 16u * i + 4u * st;
   {
     const unsigned wn = W.words_per_hap;
     unsigned hx = gmem(W.parents)[4 * (size_t)i + (st >> 1)];
     unsigned hy = gmem(W.parents)[4 * (size_t)i + 2 + (st & 1u)];
     hx = hx < a.H ? hx : a.H - 1;
     hy = hy < a.H ? hy : a.H - 1;
-    L.rx = gmem(W.panel) + (size_t)hx * wn;
-    L.ry = gmem(W.panel) + (size_t)hy * wn;
+    L.px = gmem(reinterpret_cast<const uint32_t *>(W.panel + (size_t)hx * wn));
+    L.py = gmem(reinterpret_cast<const uint32_t *>(W.panel + (size_t)hy * wn));
   }
+  const auto tran = gmem(W.tran);
   if (fwd) {
-    if (__builtin_expect(__any(!work_forward<true>(L)), 0)) work_forward<false>(L);
+    const bool ok = pass_forward<true>(L, active, tran, s_stage, (int)st);
+    if (__builtin_expect(__syncthreads_or(!ok), 0))
+      pass_forward<false>(L, active, tran, s_stage, (int)st);
   } else {
-    if (__builtin_expect(__any(!work_backward<true>(L)), 0)) work_backward<false>(L);
+    const bool ok = pass_backward<true>(L, active, tran, s_stage);
+    if (__builtin_expect(__syncthreads_or(!ok), 0))
+      pass_backward<false>(L, active, tran, s_stage);
   }
 }
 
@@ -378,6 +606,8 @@ int check_work_desc(const glp_work_desc *d) {
     return set_error(GLP_ERR_ARG, "descriptor has no windows");
   if (d->num_samples == 0) return set_error(GLP_ERR_ARG, "num_samples is 0");
   if (d->num_haps == 0) return set_error(GLP_ERR_ARG, "num_haps is 0");
+  if (d->num_samples > kPassMaxSamples)
+    return set_error(GLP_ERR_ARG, "num_samples above 4,194,304");
   for (unsigned w = 0; w < d->num_windows; ++w) {
     const glp_work_window &W = d->windows[w];
     if (W.num_sites == 0) return set_error(GLP_ERR_ARG, "window with 0 sites");
@@ -391,12 +621,47 @@ int check_work_desc(const glp_work_desc *d) {
 
 }  // namespace
 
+// Workspace layout: per window, the beta table then the scaled-l table
+// ([M][N] float4 each), windows in order; after them the emission tables
+// of the windows given none, in window order.
 extern "C" size_t glp_work_workspace_bytes(const glp_work_desc *d) {
   if (!d || !d->windows) return 0;
   size_t b = 0;
-  for (unsigned w = 0; w < d->num_windows; ++w)
-    b += 2 * (size_t)d->windows[w].num_sites * d->num_samples * sizeof(float4);
+  for (unsigned w = 0; w < d->num_windows; ++w) {
+    const size_t cells = (size_t)d->windows[w].num_sites * d->num_samples;
+    b += (d->windows[w].emit ? 2 : 3) * cells * sizeof(float4);
+  }
   return b;
+}
+
+namespace {
+
+WorkArgs work_args(unsigned N, unsigned H, const float *pc16) {
+  WorkArgs a{};
+  a.N = N;
+  a.H = H;
+  for (int k = 0; k < 16; ++k) a.pc[k] = pc16[k];
+  return a;
+}
+
+void launch_emit(const float *het_aa, size_t cells, float *emit, const WorkArgs &a,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_work_emit, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const float2 *>(het_aa), cells,
+                     reinterpret_cast<float4 *>(emit), a);
+}
+
+}  // namespace
+
+extern "C" int glp_work_emit(const float *het_aa, unsigned num_sites, unsigned num_samples,
+                             const float *pc16, float *emit, void *stream) {
+  if (!het_aa || !pc16 || !emit) return set_error(GLP_ERR_ARG, "null pointer");
+  if (num_sites == 0 || num_samples == 0) return set_error(GLP_ERR_ARG, "empty window");
+  const size_t cells = (size_t)num_sites * num_samples;
+  if ((cells + 255) / 256 > 0x7FFFFFFFu) return set_error(GLP_ERR_ARG, "window too large");
+  launch_emit(het_aa, cells, emit, work_args(num_samples, 1, pc16),
+              static_cast<hipStream_t>(stream));
+  return hip_check(hipGetLastError(), "work emit launch");
 }
 
 extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
@@ -412,39 +677,48 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
   if (rc != GLP_OK) return rc;
   char *host = new char[tbl + offb];
   std::memcpy(host, d->windows, tbl);
+  auto *hw = reinterpret_cast<glp_work_window *>(host);
   size_t *off = reinterpret_cast<size_t *>(host + tbl);
+  float4 *ws = static_cast<float4 *>(d->workspace);
+  WorkArgs a = work_args(d->num_samples, d->num_haps, d->pc16);
+  a.pen = d->pen;
+  a.seed = d->seed;
+  a.gen = d->gen;
   size_t o = 0;
   for (unsigned w = 0; w < d->num_windows; ++w) {
     off[w] = o;
     o += 2 * (size_t)d->windows[w].num_sites * d->num_samples;
   }
+  // windows without an emission table get one built in the workspace
+  for (unsigned w = 0; w < d->num_windows; ++w) {
+    if (hw[w].emit) continue;
+    const size_t cells = (size_t)hw[w].num_sites * d->num_samples;
+    float *e = reinterpret_cast<float *>(ws + o);
+    launch_emit(hw[w].het_aa, cells, e, a, s);
+    hw[w].emit = e;
+    o += cells;
+  }
   // a copy from pageable memory has left the host buffer when it returns
   rc = hip_check(hipMemcpyAsync(dev, host, tbl + offb, hipMemcpyHostToDevice, s), "table copy");
   delete[] host;
   if (rc == GLP_OK) {
-    WorkArgs a;
-    a.N = d->num_samples;
-    a.H = d->num_haps;
-    a.pen = d->pen;
-    a.seed = d->seed;
-    a.gen = d->gen;
-    for (int k = 0; k < 16; ++k) a.pc[k] = d->pc16[k];
     unsigned max_wn = 0;
     for (unsigned w = 0; w < d->num_windows; ++w)
       max_wn = std::max(max_wn, d->windows[w].words_per_hap);
-    // two wavefronts (backward, forward) per 16 samples
-    const size_t waves = 2 * (((size_t)d->num_samples + 15) / 16);
-    const dim3 grid((unsigned)((64 * waves + 255) / 256), d->num_windows), block(256);
+    // two workgroups (backward, forward) per 16 kPassWaves samples
+    const unsigned per_group = 16u * kPassWaves;
+    const dim3 grid(2u * ((d->num_samples + per_group - 1) / per_group), d->num_windows),
+        block(kPassThreads);
     const size_t words = (size_t)d->num_samples * max_wn;
     const dim3 grid2((unsigned)((words + 255) / 256), d->num_windows);
     const auto *wt = reinterpret_cast<const glp_work_window *>(dev);
     const auto *wo = reinterpret_cast<const size_t *>(dev + tbl);
-    float4 *ws = static_cast<float4 *>(d->workspace);
     hipLaunchKernelGGL(k_hmm_work_pass, grid, block, 0, s, wt, wo, ws, a);
+    const dim3 block2(256);
     if (d->pen == 1.0f)
-      hipLaunchKernelGGL((k_hmm_work_sample<false>), grid2, block, 0, s, wt, wo, ws, a);
+      hipLaunchKernelGGL((k_hmm_work_sample<false>), grid2, block2, 0, s, wt, wo, ws, a);
     else
-      hipLaunchKernelGGL((k_hmm_work_sample<true>), grid2, block, 0, s, wt, wo, ws, a);
+      hipLaunchKernelGGL((k_hmm_work_sample<true>), grid2, block2, 0, s, wt, wo, ws, a);
     rc = hip_check(hipGetLastError(), "hmm_work launch");
   }
   (void)hipFreeAsync(dev, s);

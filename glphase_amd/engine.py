@@ -258,14 +258,18 @@ class Resampler:
     het_aa: per window, the input GL pairs (Het, AA) as a host array
     [N][M][2] (Insti's prob layout); they are stored site-major on the device.
     pc: the 4 x 4 mutation matrix.
+    emit_table: build each window's emission table once here (glp_work_emit,
+    Insti::initialize's emit arrays) for every run to read; False leaves it
+    to glp_hmm_work, which then rebuilds it in its workspace on every call.
     """
 
-    def __init__(self, job, het_aa, pc):
+    def __init__(self, job, het_aa, pc, emit_table=True):
         torch = job.torch
         self.job = job
         self.torch = torch
         assert len(het_aa) == job.W
-        self.q_d, self.hnew_d = [], []
+        self.q_d, self.hnew_d, self.emit_d = [], [], []
+        pc16 = np.ascontiguousarray(np.asarray(pc, np.float32).reshape(16))
         wins = (GlpWorkWindow * job.W)()
         for w in range(job.W):
             M = job.M[w]
@@ -276,9 +280,17 @@ class Resampler:
             hn = torch.zeros((2 * job.N, wn), dtype=torch.int64, device=job.device)
             self.q_d.append(qd)
             self.hnew_d.append(hn)
+            ep = None
+            if emit_table:
+                ed = torch.empty((M, job.N, 4), dtype=torch.float32, device=job.device)
+                check(lib().glp_work_emit(qd.data_ptr(), M, job.N,
+                                          pc16.ctypes.data_as(c_f32p), ed.data_ptr(),
+                                          job.stream_handle()), "glp_work_emit")
+                self.emit_d.append(ed)
+                ep = ed.data_ptr()
             panel, tran = job.keep[3 * w], job.keep[3 * w + 2]
             wins[w] = GlpWorkWindow(M, wn, panel.data_ptr(), job.parents_d[w].data_ptr(),
-                                    tran.data_ptr(), qd.data_ptr(), hn.data_ptr())
+                                    tran.data_ptr(), qd.data_ptr(), ep, hn.data_ptr())
         self.wins = wins
         d = GlpWorkDesc()
         d.num_windows = job.W

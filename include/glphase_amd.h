@@ -286,6 +286,9 @@ typedef struct {
   const unsigned *parents;   /* [N][4] chosen parents (f0, f1, m0, m1) */
   const float *tran;         /* [M][3] */
   const float *het_aa;       /* [M][N][2] input GL pairs (Het, AA), site-major */
+  const float *emit;         /* [M][N][4] emission table of glp_work_emit (from the same
+                                het_aa and pc16), or NULL: glp_hmm_work then builds it
+                                in the workspace on every call */
   uint64_t *hnew;            /* out: [2N][wn], rows 2i, 2i+1 = sample i's */
 } glp_work_window;
 
@@ -303,8 +306,19 @@ typedef struct {
 
 /* Device workspace the resampling needs: 32 B per sample and site of every
  * window (the scaled backward and forward probabilities, computed side by
- * side). */
+ * side), plus 16 B per sample and site of every window whose emit is NULL. */
 size_t glp_work_workspace_bytes(const glp_work_desc *desc);
+
+/* Insti::initialize's emission table (insti.cpp:1345-1378) of one window,
+ * site-major: emit[(m N + i) 4 + j] = pc[j][0] t0 + pc[j][1] t1 +
+ * pc[j][2] t1 + pc[j][3] t2 with t = (max(1 - Het - AA, 0), max(Het, 0),
+ * max(AA, 0)) of sample i's GL pair at site m, het_aa[m][i] (the reference's
+ * per-sample emit arrays, [M][4] each, interleaved by site).  It depends on
+ * the GLs and pc only, so a run builds it once and every generation's
+ * glp_hmm_work reads it.  het_aa and emit are device pointers, pc16 a host
+ * array (pc[i][j] at 4 i + j).  Stream-ordered. */
+int glp_work_emit(const float *het_aa, unsigned num_sites, unsigned num_samples,
+                  const float *pc16, float *emit, void *stream);
 
 /* Impute::hmm_work for every sample of every window: scaled backward
  * recursion, forward pass with per-site posteriors, genotype weights

@@ -61,6 +61,117 @@ def test_hmm_work_matches_oracle(orc, N, M, S):
         assert (got[:, -1] >> np.uint64(M % 64) == 0).all()
 
 
+def posteriors(rs, w):
+    """Window w's posteriors l * beta [N][M][4] from the resampling step's
+    workspace: per window a beta table, then a scaled-l table, [M][N] x 4
+    floats each, windows in order."""
+    job = rs.job
+    N = job.N
+    off = sum(2 * job.M[v] * N for v in range(w))
+    M = job.M[w]
+    ws = rs.ws.cpu().numpy().view(np.float32).reshape(-1, 4)
+    beta = ws[off:off + M * N].reshape(M, N, 4)
+    lf = ws[off + M * N:off + 2 * M * N].reshape(M, N, 4)
+    return (lf * beta).transpose(1, 0, 2)
+
+
+def check_tables(orc, rs, w, p, parents, pc):
+    q = het_aa_of(p)
+    got = posteriors(rs, w)
+    for i in range(p["N"]):
+        prob, emit = orc.prob_emit(q[i], pc)
+        _, post = orc.hmm_work_weights(p["panel"], parents[i], emit, p["tran"], prob, pc, 1.0)
+        assert (got[i].view(np.uint32) == post.view(np.uint32)).all(), f"window {w} sample {i}"
+
+
+@pytest.mark.parametrize("N,M", [(17, 1), (17, 2), (70, 31), (70, 32), (70, 33), (130, 64),
+                                 (33, 255), (33, 256), (33, 257), (65, 511), (64, 513),
+                                 (20, 1025), (9, 2100)])
+def test_hmm_work_tables_match_oracle(orc, N, M):
+    # every stored posterior l * beta bit-equal to the oracle's, at site
+    # counts on either side of the 32-site blocks and 256-site transition
+    # chunks, and sample counts that leave workgroups part-filled
+    say(f"hmm_work tables N={N} M={M}")
+    from glphase_amd.engine import Resampler
+    p = helpers.synthetic(N, M, seed=3 * M + N)
+    rng = np.random.default_rng(M + 7 * N)
+    parents = rng.integers(0, p["H"], (N, 4)).astype(np.uint32)
+    job = job_with_parents(p, parents)
+    rs = Resampler(job, [het_aa_of(p)], p["pc"])
+    rs.run(pen=1.0, seed=1, gen=0)
+    rs.hnew()
+    check_tables(orc, rs, 0, p, parents, p["pc"])
+
+
+def test_hmm_work_tables_multi_window(orc):
+    say("hmm_work tables, several windows")
+    from glphase_amd.engine import Job, Resampler
+    N = 40
+    sizes = [33, 700, 1, 256, 300]
+    probs = [helpers.synthetic(N, M, seed=50 + w) for w, M in enumerate(sizes)]
+    cb, pc = probs[0]["codebook"], probs[0]["pc"]
+    for q in probs:
+        q["codebook"], q["pc"] = cb, pc
+    wins = [dict(panel=q["panel"], packed=q["packed"], tran=q["tran"]) for q in probs]
+    job = Job(wins, N, probs[0]["H"], cb, pc, bits=probs[0]["bits"], num_cycles=4,
+              precision=PREC_F32)
+    rng = np.random.default_rng(5)
+    parents = []
+    for w, q in enumerate(probs):
+        par = rng.integers(0, q["H"], (N, 4)).astype(np.uint32)
+        job.parents_d[w].copy_(job.torch.from_numpy(par.view(np.int32)).to(job.device))
+        parents.append(par)
+    rs = Resampler(job, [het_aa_of(q) for q in probs], pc)
+    rs.run(pen=0.7, seed=12, gen=4)
+    job.torch.cuda.synchronize(job.device)
+    for w, q in enumerate(probs):
+        check_tables(orc, rs, w, q, parents[w], pc)
+        got = rs.hnew_d[w].cpu().numpy().view(np.uint64)
+        want = oracle_work(orc, q["panel"], parents[w], het_aa_of(q), q["tran"], pc, 0.7, 12, w, 4)
+        assert (got == want).all(), f"window {w}"
+
+
+def test_emission_table_matches_oracle(orc):
+    # glp_work_emit's table equals Insti::initialize's emit arrays (the
+    # oracle's prob_emit) bit for bit, including GL pairs whose genotype
+    # probabilities clamp at 0 (negative entries, Het + AA > 1)
+    say("emission table")
+    from glphase_amd.engine import Resampler
+    N, M = 37, 301
+    p = helpers.synthetic(N, M, seed=8)
+    q = het_aa_of(p).copy()
+    rng = np.random.default_rng(2)
+    q[rng.random(q.shape) < 0.05] = -0.25
+    q[:, ::17, :] = (0.7, 0.6)
+    job = job_with_parents(p, np.zeros((N, 4), np.uint32))
+    rs = Resampler(job, [q], p["pc"])
+    got = rs.emit_d[0].cpu().numpy()
+    for i in range(N):
+        _, emit = orc.prob_emit(q[i], p["pc"])
+        assert (got[:, i, :].view(np.uint32) == emit.view(np.uint32)).all(), f"sample {i}"
+
+
+def test_hmm_work_without_emission_table(orc):
+    # a window given no table (emit NULL): glp_hmm_work builds it in the
+    # workspace, with the same tables and haplotypes as the prebuilt one
+    say("hmm_work, emission table built per call")
+    from glphase_amd.engine import Resampler
+    N, M = 45, 700
+    p = helpers.synthetic(N, M, seed=31)
+    rng = np.random.default_rng(32)
+    parents = rng.integers(0, p["H"], (N, 4)).astype(np.uint32)
+    job = job_with_parents(p, parents)
+    outs = []
+    for table in (True, False):
+        rs = Resampler(job, [het_aa_of(p)], p["pc"], emit_table=table)
+        assert rs.ws.numel() == (2 if table else 3) * M * N * 16
+        rs.run(pen=0.8, seed=4, gen=2)
+        outs.append((rs.hnew()[0], posteriors(rs, 0)))
+        check_tables(orc, rs, 0, p, parents, p["pc"])
+    assert (outs[0][0] == outs[1][0]).all()
+    assert (outs[0][1].view(np.uint32) == outs[1][1].view(np.uint32)).all()
+
+
 @pytest.mark.parametrize("S", [1.0, 0.49])
 def test_hmm_work_subnormal_sums(orc, S):
     # mutation-matrix entries near 2^-127.7 make every emission, and so most
@@ -81,6 +192,7 @@ def test_hmm_work_subnormal_sums(orc, S):
     got = rs.hnew()[0]
     want = oracle_work(orc, p["panel"], parents, het_aa_of(p), p["tran"], p["pc"], S, seed, 0, gen)
     assert (got == want).all()
+    check_tables(orc, rs, 0, p, parents, p["pc"])
 
 
 def test_generations_end_to_end(orc):
