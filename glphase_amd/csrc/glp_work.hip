@@ -46,6 +46,11 @@ __device__ __forceinline__ unsigned hbit64(uint64_t w, unsigned m) {
 }
 
 
+// Component j (a compile-time index) of v.
+__device__ __forceinline__ float f4c(const float4 &v, int j) {
+  return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+}
+
 // Genotype choice of impute.cpp:472-484 from the weights and the site's
 // uniform word.
 __device__ __forceinline__ unsigned work_choose(float c00, float c01, float c10, float c11,
@@ -161,11 +166,21 @@ __device__ __forceinline__ buf_rsrc raw_buffer(uintptr_t base) {
       reinterpret_cast<void *>(((uintptr_t)hi << 32) | lo), (short)0, 0x7fffffff, 0x00020000);
 }
 
+// The output tables hold, per group of 4 sites (g = m / 4), per sample and
+// per state, the 4 sites' values: float [G][N][4][4] with G = ceil(M / 4),
+// so that a lane stores 4 sites at once and a wavefront's store covers 16
+// samples' contiguous 1 KB.  Sites past M - 1 in the last group are padding.
+__host__ __device__ inline size_t pass_groups(unsigned M) { return (M + 3u) / 4u; }
+// float4 units of one output table
+__host__ __device__ inline size_t pass_table_f4(unsigned M, unsigned N) {
+  return 4 * pass_groups(M) * N;
+}
+
 // One lane's view of a sample for one recursion.
 struct PassLane {
-  uintptr_t emit, out;       // byte addresses: emission table and output table, [M][N] float4
-  uint32_t row;              // bytes per site row of both: 16 N
-  uint32_t e_off, out_off;   // the lane's offsets in a row: 16 i, 16 i + 4 st
+  uintptr_t emit, out;       // byte addresses: emission table [M][N] float4, output table
+  uint32_t row, grow;        // bytes per emission site row (16 N) and output group (64 N)
+  uint32_t e_off, out_off;   // the lane's offsets in a row: 16 i; 64 i + 16 st
   gptr_t<uint32_t> px, py;   // parents f_x, m_y: allele bits, 32 sites per word
   int M;
 };
@@ -174,20 +189,33 @@ struct PassBlock {
   float e[kPassB];  // the lane's emission at the block's sites
 };
 
+// The 16 low bits of v moved to the even bits of a 32-bit word.
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  return (v | (v << 1)) & 0x55555555u;
+}
+
 // Issues the gathers of block blk, whose parents' allele bits are (x, y).
 // FULL: all 32 sites lie in the window; otherwise site indices are clamped
 // to M - 1 (with the row bits of the unclamped site: any row of a valid
-// site is a valid address), so every gather reads inside the table.
+// site is a valid address), so every gather reads inside the table.  The
+// two bit words are interleaved once per block, so that a site's row
+// 2 x_k + y_k is one bit-field read.
 template <bool FULL>
 __device__ __forceinline__ void pass_gather(PassBlock &b, const PassLane &L, int blk, uint32_t x,
                                             uint32_t y) {
   const int m0 = kPassB * blk;
   const buf_rsrc r = raw_buffer(L.emit + (uintptr_t)m0 * L.row);
   const int last = L.M - 1 - m0;
+  const uint32_t z[2] = {(spread16(x) << 1) | spread16(y),
+                         (spread16(x >> 16) << 1) | spread16(y >> 16)};
 #pragma unroll
   for (int k = 0; k < kPassB; ++k) {
     const int kc = FULL ? k : (k < last ? k : last);
-    const uint32_t off = L.e_off + (((x >> k) & 1u) << 3) + (((y >> k) & 1u) << 2);
+    const uint32_t off = L.e_off + (((z[k >> 4] >> (2 * (k & 15))) & 3u) << 2);
     b.e[k] = __uint_as_float(
         __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, kc * (int)L.row, 0));
   }
@@ -206,20 +234,29 @@ __device__ __forceinline__ void pass_copy(PassBlock &dst, const PassBlock &src) 
   for (int k = 0; k < kPassB; ++k) dst.e[k] = src.e[k];
 }
 
-__device__ __forceinline__ void pass_store(float v, const buf_rsrc &r, const PassLane &L, int k) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)L.out_off,
-                                        k * (int)L.row, 0);
+// Stores the lane's values of the 4 sites of group j of the descriptor's
+// base (4 consecutive floats).
+typedef uint32_t glp_u4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void pass_store4(const float (&o)[4], const buf_rsrc &r,
+                                            const PassLane &L, int j) {
+  glp_u4s v;
+  v.x = __float_as_uint(o[0]);
+  v.y = __float_as_uint(o[1]);
+  v.z = __float_as_uint(o[2]);
+  v.w = __float_as_uint(o[3]);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)L.out_off, j * (int)L.grow, 0);
 }
 
 // Forward steps over block blk (impute.cpp:418-455); co: the block's first
 // site's coefficient vector for this lane's state (site k at co[4 k]).  Each
-// site's row of the table takes the scaled l.  FULL: sites 1 .. M-1 only;
-// otherwise site 0 starts the recursion and sites past M - 1 are skipped.
+// site's entry of the table takes the scaled l, stored per 4 sites from o.
+// FULL: sites 1 .. M-1 only; otherwise site 0 starts the recursion, sites
+// past M - 1 are skipped and the window's last group is stored whole.
 template <bool FAST, bool FULL>
 __device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
-                                               const float4 *co, float &l) {
+                                               const float4 *co, float &l, float (&o)[4]) {
   const int m0 = kPassB * blk;
-  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)m0 * L.row);
+  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)(m0 / 4) * L.grow);
   float4 cn = co[0];  // each site's LDS read is issued a site ahead of its use
 #pragma unroll
   for (int k = 0; k < kPassB; ++k) {
@@ -229,28 +266,35 @@ __device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBloc
     if (k + 1 < kPassB) cn = co[4 * (k + 1)];
     __builtin_amdgcn_sched_barrier(0);
     if (FULL || m) {
-      const float n = qperm<kQ0>(l) * c.x + qperm<kQ1>(l) * c.y + qperm<kQ2>(l) * c.z +
-                      qperm<kQ3>(l) * c.w;
+      float p0 = qperm<kQ0>(l) * c.x, p1 = qperm<kQ1>(l) * c.y, p2 = qperm<kQ2>(l) * c.z,
+            p3 = qperm<kQ3>(l) * c.w;
+      // the four products in four registers before the sums (reusing one for
+      // the next product costs a wait state)
+      asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3));
+      const float n = p0 + p1 + p2 + p3;
       l = n * b.e[k];
     } else {
       l = 0.25f * b.e[k];
     }
     l *= work_recip<FAST>(quad_sum(l));
-    pass_store(l, r, L, k);
+    o[k & 3] = l;
+    if ((k & 3) == 3) pass_store4(o, r, L, k >> 2);
   }
+  if (!FULL && (L.M & 3) && L.M - 1 - m0 < kPassB) pass_store4(o, r, L, (L.M - 1 - m0) >> 2);
 }
 
 // Backward steps over block blk, sites 32 blk + 31 down to 32 blk
 // (impute.cpp:398-416); tr: the block's first site's (t0, t1, t2).  Site m
-// turns beta_m into beta_{m-1}, written to row m - 1.  FULL: every site in
+// turns beta_m into beta_{m-1}, entry m - 1 of the table: slot (m - 1) & 3 of
+// o, whose group is stored when its lowest entry is in.  FULL: every site in
 // 1 .. M-1; otherwise the others are skipped.
 template <bool FAST, bool FULL>
 __device__ __forceinline__ void bwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
-                                               const float4 *tr, float &v) {
+                                               const float4 *tr, float &v, float (&o)[4]) {
   const int m0 = kPassB * blk;
-  // rows m0 - 1 + k; in block 0 no site k = 0 is stored, so the descriptor's
-  // base may lie one row before the table
-  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)m0 * L.row - L.row);
+  // entry m0 - 1 + k sits in group m0 / 4 + (k - 1) / 4 (the one of k = 0
+  // is stored from the block below)
+  const buf_rsrc r = raw_buffer(L.out + (uintptr_t)(m0 / 4) * L.grow);
   float4 tn = tr[kPassB - 1];  // each site's LDS read is issued a site ahead of its use
 #pragma unroll
   for (int k = kPassB - 1; k >= 0; --k) {
@@ -265,7 +309,8 @@ __device__ __forceinline__ void bwd_pass_block(const PassLane &L, const PassBloc
     const float cross = qperm<kQSwap1>(bb) + qperm<kQSwap2>(bb);
     const float n = bb * t.x + cross * t.y + qperm<kQOpp>(bb) * t.z;
     v = n * work_recip<FAST>(quad_sum(n));
-    pass_store(v, r, L, k);
+    o[(k + 3) & 3] = v;
+    if ((k & 3) == 1) pass_store4(o, r, L, k >> 2);
   }
 }
 
@@ -352,7 +397,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
       stage_enter<true>(S, stage, tran, ch, ch + 1, nch, M);
     }
   };
-  float l = 0;
+  float l = 0, o[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   PassBlock A, B;
   uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
   if (active) {
@@ -370,7 +415,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
         bx = L.px[2];
         by = L.py[2];
       }
-      fwd_pass_block<FAST, false>(L, A, 0, co(0), l);
+      fwd_pass_block<FAST, false>(L, A, 0, co(0), l, o);
     }
     int blk = 1;  // in B
     enter(blk);
@@ -385,7 +430,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
           bx = L.px[blk + 2];
           by = L.py[blk + 2];
         }
-        fwd_pass_block<FAST, true>(L, B, blk, co(blk), l);
+        fwd_pass_block<FAST, true>(L, B, blk, co(blk), l, o);
       }
       enter(++blk);  // in A
       if (blk == top) break;
@@ -395,12 +440,12 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
           bx = L.px[blk + 2];
           by = L.py[blk + 2];
         }
-        fwd_pass_block<FAST, true>(L, A, blk, co(blk), l);
+        fwd_pass_block<FAST, true>(L, A, blk, co(blk), l, o);
       }
       enter(++blk);  // in B
     }
   }
-  if (active) fwd_pass_block<FAST, false>(L, A, top, co(top), l);
+  if (active) fwd_pass_block<FAST, false>(L, A, top, co(top), l, o);
   return !active || quad_sum(l) > 0.0f;
 }
 
@@ -422,11 +467,14 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
       stage_enter<false>(S, stage, tran, ch, ch - 1, nch, M);
     }
   };
-  float v = 1;
+  // beta_{M-1} = 1: every slot starts at 1, so the group of entry M - 1 holds
+  // it (and padding past it) when stored; stored at once if M - 1 is its
+  // group's lowest entry
+  float v = 1, o[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   PassBlock A, B;
   uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
   if (active) {
-    pass_store(1.0f, raw_buffer(L.out + (uintptr_t)(M - 1) * L.row), L, 0);
+    if (((M - 1) & 3) == 0) pass_store4(o, raw_buffer(L.out), L, (M - 1) >> 2);
     const uint32_t xt = L.px[top], yt = L.py[top];
     if (top >= 1) {
       bx = L.px[top - 1];
@@ -441,7 +489,7 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
         bx = L.px[top - 2];
         by = L.py[top - 2];
       }
-      bwd_pass_block<FAST, false>(L, A, top, tr(top), v);
+      bwd_pass_block<FAST, false>(L, A, top, tr(top), v, o);
     }
     int blk = top - 1;  // in B
     enter(blk);
@@ -456,7 +504,7 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
           bx = L.px[blk - 2];
           by = L.py[blk - 2];
         }
-        bwd_pass_block<FAST, true>(L, B, blk, tr(blk), v);
+        bwd_pass_block<FAST, true>(L, B, blk, tr(blk), v, o);
       }
       enter(--blk);  // in A
       if (blk == 0) break;
@@ -466,31 +514,30 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
           bx = L.px[blk - 2];
           by = L.py[blk - 2];
         }
-        bwd_pass_block<FAST, true>(L, A, blk, tr(blk), v);
+        bwd_pass_block<FAST, true>(L, A, blk, tr(blk), v, o);
       }
       enter(--blk);  // in B
     }
   }
-  if (active) bwd_pass_block<FAST, false>(L, A, 0, tr(0), v);
+  if (active) bwd_pass_block<FAST, false>(L, A, 0, tr(0), v, o);
   return !active || quad_sum(v) > 0.0f;
 }
 
 // The two recursions of impute.cpp:398-470, side by side: workgroups of
 // even blockIdx.x run the backward recursion of 16 kPassWaves samples, odd
 // ones the forward recursion of the same samples; one lane per (sample,
-// state), lane s of a quad holding state s's value.  The cross-state terms come by quad DPP
-// moves in the reference's operand order, and the four lanes compute each
-// site's normaliser together.  The backward recursion writes beta to the
-// first table of the window's workspace, the forward one the scaled l to
-// the second ([M][N] float4 each, site-major: a quad stores one float4, a
-// wavefront 16 samples' contiguous float4s); the sampling step multiplies
-// them into the posteriors l * beta, the reference's product.  A workgroup
-// runs its recursion with the short reciprocal first; if some sample met a
-// sum outside its exact range, the workgroup redoes it with the IEEE
-// division, so every stored value is the division's.  Lanes past the last
-// sample repeat the last sample's work and store its (identical) values.
-// Parent indices past the panel are clamped (the MH update never produces
-// them).
+// state), lane s of a quad holding state s's value.  The cross-state terms
+// come by quad DPP moves in the reference's operand order, and the four
+// lanes compute each site's normaliser together.  The backward recursion
+// writes beta to the first table of the window's workspace, the forward one
+// the scaled l to the second (per group of 4 sites, sample and state the 4
+// sites' values); the sampling step multiplies them into the posteriors
+// l * beta, the reference's product.  A workgroup runs its recursion with
+// the short reciprocal first; if some sample met a sum outside its exact
+// range, the workgroup redoes it with the IEEE division, so every stored
+// value is the division's.  Lanes past the last sample repeat the last
+// sample's work and store its (identical) values.  Parent indices past the
+// panel are clamped (the MH update never produces them).
 __global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
     k_hmm_work_pass(const glp_work_window *__restrict__ wins, const size_t *__restrict__ ws_off,
                     float4 *__restrict__ ws, WorkArgs a) {
@@ -507,10 +554,12 @@ __global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_
   PassLane L;
   L.M = (int)W.num_sites;
   L.emit = reinterpret_cast<uintptr_t>(W.emit);
-  L.out = reinterpret_cast<uintptr_t>(ws + ws_off[w] + (fwd ? (size_t)W.num_sites * a.N : 0));
+  L.out = reinterpret_cast<uintptr_t>(ws + ws_off[w] +
+                                     (fwd ? pass_table_f4(W.num_sites, a.N) : 0));
   L.row = 16u * a.N;
+  L.grow = 64u * a.N;
   L.e_off = 16u * i;
-  L.out_off = 16u * i + 4u * st;
+  L.out_off = 64u * i + 16u * st;
   {
     const unsigned wn = W.words_per_hap;
     unsigned hx = gmem(W.parents)[4 * (size_t)i + (st >> 1)];
@@ -533,9 +582,9 @@ __global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_
 }
 
 // The sampling of impute.cpp:457-484, one lane per (window, sample, 64-site
-// word), sample fastest so a wavefront's reads of a site coalesce: from the
-// posteriors p = l * beta (the two tables k_hmm_work_pass wrote) and the GL
-// pair, the genotype
+// word), sample fastest so a wavefront's reads coalesce: from the posteriors
+// p = l * beta (the two tables k_hmm_work_pass wrote, read per group of 4
+// sites: each state's 4 values in one load) and the GL pair, the genotype
 // weights c_g = (t_g * sum_s p_s pc[s][g])^S, one Philox word per site, the
 // genotype choice, and the two new haplotype words.  POW: S != 1 (powf is
 // the identity at S = 1).
@@ -552,7 +601,7 @@ __global__ void __launch_bounds__(256)
   if (t >= (size_t)N * wn) return;
   const unsigned i = (unsigned)(t % N), wd = (unsigned)(t / N);
   const auto beta = gmem(reinterpret_cast<const float *>(ws + ws_off[w]));
-  const auto lfwd = gmem(reinterpret_cast<const float *>(ws + ws_off[w] + (size_t)M * N));
+  const auto lfwd = gmem(reinterpret_cast<const float *>(ws + ws_off[w] + pass_table_f4(M, N)));
   const auto q2 = gmem(W.het_aa);
   uint64_t hw[4];
 #pragma unroll
@@ -562,13 +611,25 @@ __global__ void __launch_bounds__(256)
     hw[q] = gmem(W.panel)[(size_t)h * wn + wd];
   }
   const float S = a.pen;
-  const unsigned m1 = 64u * wd + 64u < M ? 64u * wd + 64u : M;
+  const unsigned G = (unsigned)pass_groups(M);
+  const unsigned g1 = 16u * wd + 16u < G ? 16u * wd + 16u : G;
   uint64_t wa = 0, wb = 0;
-  U4 x{0, 0, 0, 0};
-  for (unsigned m = 64u * wd; m < m1; ++m) {
-    const float4 bq = load_f4(beta + 4 * ((size_t)m * N + i));
-    const float4 lq = load_f4(lfwd + 4 * ((size_t)m * N + i));
-    const float4 p = make_float4(lq.x * bq.x, lq.y * bq.y, lq.z * bq.z, lq.w * bq.w);
+  for (unsigned gi = 16u * wd; gi < g1; ++gi) {
+    // states 0..3 of the group's 4 sites, from each table
+    const size_t rec = 16 * ((size_t)gi * N + i);
+    float4 bs[4], ls[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bs[s] = load_f4(beta + rec + 4 * s);
+      ls[s] = load_f4(lfwd + rec + 4 * s);
+    }
+    const U4 x = draw(a.seed, kWorkTag | gi, i, w, a.gen);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+    const unsigned m = 4u * gi + (unsigned)j;
+    if (m >= M) break;
+    const float4 p = make_float4(f4c(ls[0], j) * f4c(bs[0], j), f4c(ls[1], j) * f4c(bs[1], j),
+                                 f4c(ls[2], j) * f4c(bs[2], j), f4c(ls[3], j) * f4c(bs[3], j));
     const float3 tq = work_prob(load_f2(q2 + 2 * ((size_t)m * N + i)));
     const unsigned f0 = hbit64(hw[0], m), f1 = hbit64(hw[1], m), m0 = hbit64(hw[2], m),
                    m1b = hbit64(hw[3], m);
@@ -584,11 +645,11 @@ __global__ void __launch_bounds__(256)
       c10 = (float)pow((double)c10, (double)S);
       c11 = (float)pow((double)c11, (double)S);
     }
-    if ((m & 3u) == 0) x = draw(a.seed, kWorkTag | (m >> 2), i, w, a.gen);
-    const uint32_t xm = (m & 3u) == 0 ? x.x : (m & 3u) == 1 ? x.y : (m & 3u) == 2 ? x.z : x.w;
+    const uint32_t xm = j == 0 ? x.x : j == 1 ? x.y : j == 2 ? x.z : x.w;
     const unsigned g = work_choose(c00, c01, c10, c11, xm);
     wa |= (uint64_t)(g >> 1) << (m & 63u);
     wb |= (uint64_t)(g & 1u) << (m & 63u);
+    }
   }
   const auto hnew = (gmut_t<uint64_t>)W.hnew;
   hnew[(size_t)(2 * i) * wn + wd] = wa;
@@ -622,14 +683,15 @@ int check_work_desc(const glp_work_desc *d) {
 }  // namespace
 
 // Workspace layout: per window, the beta table then the scaled-l table
-// ([M][N] float4 each), windows in order; after them the emission tables
-// of the windows given none, in window order.
+// (float [ceil(M / 4)][N][4][4] each), windows in order; after them the
+// emission tables ([M][N] float4) of the windows given none, in window order.
 extern "C" size_t glp_work_workspace_bytes(const glp_work_desc *d) {
   if (!d || !d->windows) return 0;
   size_t b = 0;
   for (unsigned w = 0; w < d->num_windows; ++w) {
-    const size_t cells = (size_t)d->windows[w].num_sites * d->num_samples;
-    b += (d->windows[w].emit ? 2 : 3) * cells * sizeof(float4);
+    const unsigned M = d->windows[w].num_sites;
+    b += 2 * pass_table_f4(M, d->num_samples) * sizeof(float4);
+    if (!d->windows[w].emit) b += (size_t)M * d->num_samples * sizeof(float4);
   }
   return b;
 }
@@ -687,7 +749,7 @@ extern "C" int glp_hmm_work(const glp_work_desc *d, void *stream) {
   size_t o = 0;
   for (unsigned w = 0; w < d->num_windows; ++w) {
     off[w] = o;
-    o += 2 * (size_t)d->windows[w].num_sites * d->num_samples;
+    o += 2 * pass_table_f4(d->windows[w].num_sites, d->num_samples);
   }
   // windows without an emission table get one built in the workspace
   for (unsigned w = 0; w < d->num_windows; ++w) {

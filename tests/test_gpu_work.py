@@ -61,18 +61,24 @@ def test_hmm_work_matches_oracle(orc, N, M, S):
         assert (got[:, -1] >> np.uint64(M % 64) == 0).all()
 
 
+def groups(M):
+    return (M + 3) // 4
+
+
 def posteriors(rs, w):
     """Window w's posteriors l * beta [N][M][4] from the resampling step's
-    workspace: per window a beta table, then a scaled-l table, [M][N] x 4
-    floats each, windows in order."""
+    workspace: per window a beta table, then a scaled-l table, each
+    [ceil(M / 4)][N][4 states][4 sites] floats, windows in order."""
     job = rs.job
     N = job.N
-    off = sum(2 * job.M[v] * N for v in range(w))
+    off = sum(2 * groups(job.M[v]) * N * 16 for v in range(w))
     M = job.M[w]
-    ws = rs.ws.cpu().numpy().view(np.float32).reshape(-1, 4)
-    beta = ws[off:off + M * N].reshape(M, N, 4)
-    lf = ws[off + M * N:off + 2 * M * N].reshape(M, N, 4)
-    return (lf * beta).transpose(1, 0, 2)
+    G = groups(M)
+    ws = rs.ws.cpu().numpy().view(np.float32)
+    beta = ws[off:off + G * N * 16].reshape(G, N, 4, 4)
+    lf = ws[off + G * N * 16:off + 2 * G * N * 16].reshape(G, N, 4, 4)
+    # [g][i][state][site j] -> [i][4 g + j][state]
+    return (lf * beta).transpose(1, 0, 3, 2).reshape(N, 4 * G, 4)[:, :M, :]
 
 
 def check_tables(orc, rs, w, p, parents, pc):
@@ -164,7 +170,7 @@ def test_hmm_work_without_emission_table(orc):
     outs = []
     for table in (True, False):
         rs = Resampler(job, [het_aa_of(p)], p["pc"], emit_table=table)
-        assert rs.ws.numel() == (2 if table else 3) * M * N * 16
+        assert rs.ws.numel() == 2 * groups(M) * N * 64 + (0 if table else M * N * 16)
         rs.run(pen=0.8, seed=4, gen=2)
         outs.append((rs.hnew()[0], posteriors(rs, 0)))
         check_tables(orc, rs, 0, p, parents, p["pc"])
