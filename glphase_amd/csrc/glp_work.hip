@@ -237,14 +237,34 @@ __device__ __forceinline__ void pass_copy(PassBlock &dst, const PassBlock &src) 
 // Stores the lane's values of the 4 sites of group j of the descriptor's
 // base (4 consecutive floats).
 typedef uint32_t glp_u4s __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void pass_store4(const float (&o)[4], const buf_rsrc &r,
-                                            const PassLane &L, int j) {
+__device__ __forceinline__ glp_u4s pack4(const float (&o)[4]) {
   glp_u4s v;
   v.x = __float_as_uint(o[0]);
   v.y = __float_as_uint(o[1]);
   v.z = __float_as_uint(o[2]);
   v.w = __float_as_uint(o[3]);
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)L.out_off, j * (int)L.grow, 0);
+  return v;
+}
+
+__device__ __forceinline__ void pass_store4(const float (&o)[4], const buf_rsrc &r,
+                                            const PassLane &L, int j) {
+  __builtin_amdgcn_raw_buffer_store_b128(pack4(o), r, (int)L.out_off, j * (int)L.grow, 0);
+}
+
+// The data registers of a store may not be overwritten before the store has
+// completed (the memory counter waits for it), so the data of the groups
+// stored last stay in two rotating register sets: set p (the group's
+// parity) is refilled only two groups, 8 sites, after its store.
+struct StoreRing {
+  glp_u4s v[2];
+};
+
+template <int P>
+__device__ __forceinline__ void pass_store4_ring(StoreRing &R, const float (&o)[4],
+                                                 const buf_rsrc &r, const PassLane &L, int j) {
+  asm volatile("" ::"v"(R.v[P]));  // the last store of set P keeps its registers until here
+  R.v[P] = pack4(o);
+  __builtin_amdgcn_raw_buffer_store_b128(R.v[P], r, (int)L.out_off, j * (int)L.grow, 0);
 }
 
 // Forward steps over block blk (impute.cpp:418-455); co: the block's first
@@ -254,7 +274,8 @@ __device__ __forceinline__ void pass_store4(const float (&o)[4], const buf_rsrc 
 // past M - 1 are skipped and the window's last group is stored whole.
 template <bool FAST, bool FULL>
 __device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
-                                               const float4 *co, float &l, float (&o)[4]) {
+                                               const float4 *co, float &l, float (&o)[4],
+                                               StoreRing &R) {
   const int m0 = kPassB * blk;
   const buf_rsrc r = raw_buffer(L.out + (uintptr_t)(m0 / 4) * L.grow);
   float4 cn = co[0];  // each site's LDS read is issued a site ahead of its use
@@ -278,7 +299,13 @@ __device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBloc
     }
     l *= work_recip<FAST>(quad_sum(l));
     o[k & 3] = l;
-    if ((k & 3) == 3) pass_store4(o, r, L, k >> 2);
+    if ((k & 3) == 3) {
+      // group k >> 2 of a block of 8 groups: its parity is its global one
+      if ((k >> 2) & 1)
+        pass_store4_ring<1>(R, o, r, L, k >> 2);
+      else
+        pass_store4_ring<0>(R, o, r, L, k >> 2);
+    }
   }
   if (!FULL && (L.M & 3) && L.M - 1 - m0 < kPassB) pass_store4(o, r, L, (L.M - 1 - m0) >> 2);
 }
@@ -290,7 +317,8 @@ __device__ __forceinline__ void fwd_pass_block(const PassLane &L, const PassBloc
 // 1 .. M-1; otherwise the others are skipped.
 template <bool FAST, bool FULL>
 __device__ __forceinline__ void bwd_pass_block(const PassLane &L, const PassBlock &b, int blk,
-                                               const float4 *tr, float &v, float (&o)[4]) {
+                                               const float4 *tr, float &v, float (&o)[4],
+                                               StoreRing &R) {
   const int m0 = kPassB * blk;
   // entry m0 - 1 + k sits in group m0 / 4 + (k - 1) / 4 (the one of k = 0
   // is stored from the block below)
@@ -310,7 +338,12 @@ __device__ __forceinline__ void bwd_pass_block(const PassLane &L, const PassBloc
     const float n = bb * t.x + cross * t.y + qperm<kQOpp>(bb) * t.z;
     v = n * work_recip<FAST>(quad_sum(n));
     o[(k + 3) & 3] = v;
-    if ((k & 3) == 1) pass_store4(o, r, L, k >> 2);
+    if ((k & 3) == 1) {
+      if ((k >> 2) & 1)
+        pass_store4_ring<1>(R, o, r, L, k >> 2);
+      else
+        pass_store4_ring<0>(R, o, r, L, k >> 2);
+    }
   }
 }
 
@@ -398,6 +431,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
     }
   };
   float l = 0, o[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  StoreRing R = {};
   PassBlock A, B;
   uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
   if (active) {
@@ -415,7 +449,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
         bx = L.px[2];
         by = L.py[2];
       }
-      fwd_pass_block<FAST, false>(L, A, 0, co(0), l, o);
+      fwd_pass_block<FAST, false>(L, A, 0, co(0), l, o, R);
     }
     int blk = 1;  // in B
     enter(blk);
@@ -430,7 +464,7 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
           bx = L.px[blk + 2];
           by = L.py[blk + 2];
         }
-        fwd_pass_block<FAST, true>(L, B, blk, co(blk), l, o);
+        fwd_pass_block<FAST, true>(L, B, blk, co(blk), l, o, R);
       }
       enter(++blk);  // in A
       if (blk == top) break;
@@ -440,12 +474,12 @@ __device__ bool pass_forward(const PassLane &L, bool active, gptr_t<float> tran,
           bx = L.px[blk + 2];
           by = L.py[blk + 2];
         }
-        fwd_pass_block<FAST, true>(L, A, blk, co(blk), l, o);
+        fwd_pass_block<FAST, true>(L, A, blk, co(blk), l, o, R);
       }
       enter(++blk);  // in B
     }
   }
-  if (active) fwd_pass_block<FAST, false>(L, A, top, co(top), l, o);
+  if (active) fwd_pass_block<FAST, false>(L, A, top, co(top), l, o, R);
   return !active || quad_sum(l) > 0.0f;
 }
 
@@ -471,6 +505,7 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
   // it (and padding past it) when stored; stored at once if M - 1 is its
   // group's lowest entry
   float v = 1, o[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+  StoreRing R = {};
   PassBlock A, B;
   uint32_t bx = 0, by = 0;  // allele bits of the next block to gather
   if (active) {
@@ -489,7 +524,7 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
         bx = L.px[top - 2];
         by = L.py[top - 2];
       }
-      bwd_pass_block<FAST, false>(L, A, top, tr(top), v, o);
+      bwd_pass_block<FAST, false>(L, A, top, tr(top), v, o, R);
     }
     int blk = top - 1;  // in B
     enter(blk);
@@ -504,7 +539,7 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
           bx = L.px[blk - 2];
           by = L.py[blk - 2];
         }
-        bwd_pass_block<FAST, true>(L, B, blk, tr(blk), v, o);
+        bwd_pass_block<FAST, true>(L, B, blk, tr(blk), v, o, R);
       }
       enter(--blk);  // in A
       if (blk == 0) break;
@@ -514,12 +549,12 @@ __device__ bool pass_backward(const PassLane &L, bool active, gptr_t<float> tran
           bx = L.px[blk - 2];
           by = L.py[blk - 2];
         }
-        bwd_pass_block<FAST, true>(L, A, blk, tr(blk), v, o);
+        bwd_pass_block<FAST, true>(L, A, blk, tr(blk), v, o, R);
       }
       enter(--blk);  // in B
     }
   }
-  if (active) bwd_pass_block<FAST, false>(L, A, 0, tr(0), v, o);
+  if (active) bwd_pass_block<FAST, false>(L, A, 0, tr(0), v, o, R);
   return !active || quad_sum(v) > 0.0f;
 }
 
