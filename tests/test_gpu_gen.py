@@ -70,7 +70,8 @@ def gpu_replace_rounds(haps_list, N, M):
     return hs.cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("N,M,rounds", [(1, 1, 3), (13, 130, 6), (200, 1000, 8), (5000, 4096, 3)])
+@pytest.mark.parametrize("N,M,rounds", [(1, 1, 3), (13, 130, 6), (200, 1000, 8), (9, 1025, 4),
+                                         (9, 2000, 3), (5000, 4096, 3), (7, 5000, 4)])
 def test_replace_matches_oracle(N, M, rounds):
     say(f"replace N={N} M={M}")
     rng = np.random.default_rng(N + M)
