@@ -60,10 +60,7 @@ __global__ void __launch_bounds__(256)
 // haplotypes (a, b) with its tallies (ha, hb) as they stand and swapped; the
 // tallies then count the pair in the orientation that agrees more (a to ha
 // when sis > tra, else b to ha).  Integer sums, so any order gives the
-// reference's values.  WN > 0: the window has at most WN words, and each
-// lane keeps its tallies in registers between the two passes, so they are
-// read once (WN = 0 reads them again).
-template <int WN>
+// reference's values.
 __global__ void __launch_bounds__(256)
     k_replace(const uint64_t *__restrict__ haps, unsigned N, unsigned M, unsigned wn,
               uint32_t *__restrict__ hsum) {
@@ -72,53 +69,29 @@ __global__ void __launch_bounds__(256)
   const uint64_t *oa = haps + (size_t)(2u * i) * wn, *ob = oa + wn;
   uint32_t *ha = hsum + (size_t)(2u * i) * M, *hb = ha + M;
   unsigned sis = 0, tra = 0;
-  constexpr int C = WN > 0 ? WN : 1;
-  uint32_t ca[C], cb[C];
-  auto pass1 = [&](unsigned w, uint32_t &sa, uint32_t &sb) {
+  for (unsigned w = 0; w < wn; ++w) {
     const unsigned m = 64u * w + lane;
     const uint64_t wa = oa[w], wb = ob[w];
     if (m < M) {
       const unsigned x = (unsigned)(wa >> lane) & 1u, y = (unsigned)(wb >> lane) & 1u;
-      sa = ha[m];
-      sb = hb[m];
+      const unsigned sa = ha[m], sb = hb[m];
       sis += x * sa + y * sb;
       tra += x * sb + y * sa;
     }
-  };
-  if constexpr (WN > 0) {
-#pragma unroll
-    for (int w = 0; w < WN; ++w) {
-      ca[w] = cb[w] = 0;
-      if ((unsigned)w < wn) pass1((unsigned)w, ca[w], cb[w]);
-    }
-  } else {
-    for (unsigned w = 0; w < wn; ++w) pass1(w, ca[0], cb[0]);
   }
   for (unsigned o = 32; o; o >>= 1) {
     sis += (unsigned)__shfl_xor((int)sis, (int)o, 64);
     tra += (unsigned)__shfl_xor((int)tra, (int)o, 64);
   }
   const bool keep = sis > tra;
-  auto pass2 = [&](unsigned w, uint32_t sa, uint32_t sb) {
+  for (unsigned w = 0; w < wn; ++w) {
     const unsigned m = 64u * w + lane;
     const uint64_t wa = oa[w], wb = ob[w];
     if (m < M) {
       const unsigned x = (unsigned)(wa >> lane) & 1u, y = (unsigned)(wb >> lane) & 1u;
-      if (WN > 0) {
-        ha[m] = sa + (keep ? x : y);
-        hb[m] = sb + (keep ? y : x);
-      } else {
-        ha[m] += keep ? x : y;
-        hb[m] += keep ? y : x;
-      }
+      ha[m] += keep ? x : y;
+      hb[m] += keep ? y : x;
     }
-  };
-  if constexpr (WN > 0) {
-#pragma unroll
-    for (int w = 0; w < WN; ++w)
-      if ((unsigned)w < wn) pass2((unsigned)w, ca[w], cb[w]);
-  } else {
-    for (unsigned w = 0; w < wn; ++w) pass2(w, 0, 0);
   }
 }
 
@@ -143,13 +116,7 @@ extern "C" int glp_replace(const uint64_t *haps, unsigned num_samples, unsigned 
   if (!haps || !hsum || num_samples == 0 || num_sites == 0)
     return set_error(GLP_ERR_ARG, "null pointer or empty window");
   const unsigned wn = (num_sites + 63) / 64;
-  const dim3 grid((num_samples + 3) / 4), block(256);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (wn <= 16)
-    hipLaunchKernelGGL(k_replace<16>, grid, block, 0, s, haps, num_samples, num_sites, wn, hsum);
-  else if (wn <= 64)
-    hipLaunchKernelGGL(k_replace<64>, grid, block, 0, s, haps, num_samples, num_sites, wn, hsum);
-  else
-    hipLaunchKernelGGL(k_replace<0>, grid, block, 0, s, haps, num_samples, num_sites, wn, hsum);
+  hipLaunchKernelGGL(k_replace, dim3((num_samples + 3) / 4), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), haps, num_samples, num_sites, wn, hsum);
   return hip_check(hipGetLastError(), "k_replace launch");
 }
