@@ -92,7 +92,7 @@ def check_tables(orc, rs, w, p, parents, pc):
 
 @pytest.mark.parametrize("N,M", [(17, 1), (17, 2), (70, 31), (70, 32), (70, 33), (130, 64),
                                  (33, 255), (33, 256), (33, 257), (65, 511), (64, 513),
-                                 (20, 1025), (9, 2100)])
+                                 (20, 1025), (9, 2100), (1, 100), (1, 33), (2, 64), (5, 8199)])
 def test_hmm_work_tables_match_oracle(orc, N, M):
     # every stored posterior l * beta bit-equal to the oracle's, at site
     # counts on either side of the 32-site blocks and 256-site transition
