@@ -671,12 +671,12 @@ static int plan_graph_build(glp_plan *p, int kind) {
   if (plan_fb(p)) {
     // the windows' forward-backward layouts, from the windows' current data
     const FbWin *jobs = p->fbjobs.as<FbWin>();
-    unsigned N = a.N, H = a.H, bits = a.bits;
-    void *margs[] = {&jobs, &N, &H, &bits};
+    unsigned N = a.N, H = a.H;
+    void *margs[] = {&jobs, &N, &H};
     unsigned gx = std::max(1u, std::min((p->fb_items_max + 255u) / 256u,
                                         std::max(1u, 4096u / std::max(1u, p->nwin))));
     hipKernelNodeParams mk{};
-    mk.func = const_cast<void *>(fb_mirror_kernel_fn());
+    mk.func = const_cast<void *>(fb_mirror_kernel_fn(a.bits));
     mk.gridDim = dim3(gx, p->nwin);
     mk.blockDim = dim3(256);
     mk.sharedMemBytes = 0;

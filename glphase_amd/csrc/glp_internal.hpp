@@ -75,7 +75,7 @@ int fb_check(unsigned M, unsigned bits, unsigned N, unsigned H);
 unsigned fb_items(unsigned M, unsigned bits, unsigned N, unsigned H);
 int launch_fb_mirror(const FbWin *d_fw, unsigned nwin, unsigned max_items, unsigned N,
                      unsigned H, unsigned bits, hipStream_t stream);
-const void *fb_mirror_kernel_fn();
+const void *fb_mirror_kernel_fn(unsigned bits);  // 4 or 8
 // One batched generation with every device buffer beyond the caller's in
 // the file-scope scratch array of glp_kernels.hip (glp_mh_run_noalloc).
 int run_noalloc(const KArgs &a, const glp_window *wins_h, unsigned nwin, unsigned Q,

@@ -1192,47 +1192,70 @@ __device__ __forceinline__ uint64_t sites64(const uint64_t *row, unsigned wn, un
   return x;
 }
 
+// Code word q of column i in the mirrored half: slot r (MSB-first) holds the
+// code of site hi - r, hi = M-1+p - q*cpw, 0 outside [0, M).  The word's
+// sites span at most two source words k0, k0 + 1; their 64 bits with the
+// fields reversed put slot r at field d + r, so one shift extracts it.
+template <int BITS>
+__device__ __forceinline__ uint32_t mirror_code_word(const uint32_t *__restrict__ packed,
+                                                     unsigned N, unsigned M, unsigned p,
+                                                     unsigned q, unsigned i) {
+  constexpr int CPW = 32 / BITS;
+  const unsigned nsrc = (M * BITS + 31u) >> 5;  // code words per column
+  const int hi = (int)(M - 1u + p) - (int)q * CPW;  // >= 1 for q < nw
+  const int lo = hi - CPW + 1;                      // >= 2 - CPW
+  const int k0 = (lo + CPW) / CPW - 1;              // floor(lo / CPW), >= -1
+  const uint32_t a = (k0 >= 0 && (unsigned)k0 < nsrc) ? packed[(size_t)k0 * N + i] : 0u;
+  const uint32_t b = (unsigned)(k0 + 1) < nsrc ? packed[(size_t)(k0 + 1) * N + i] : 0u;
+  uint64_t x = __builtin_bswap64(((uint64_t)a << 32) | b);  // bytes reversed
+  if (BITS == 4)
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+  // x's field f' holds source field 2 CPW - 1 - f'; slot r wants source
+  // field hi - r - k0 CPW, i.e. field d + r with d = 2 CPW - 1 - hi + k0 CPW
+  const unsigned d = (unsigned)(2 * CPW - 1 - hi + k0 * CPW);  // 1 .. CPW
+  uint32_t w = (uint32_t)((x << (BITS * d)) >> 32);
+  // slots r < r0 hold sites >= M, slots r > hi sites < 0
+  const unsigned r0 = hi >= (int)M ? (unsigned)(hi - (int)M + 1) : 0u;
+  if (r0) w &= 0xFFFFFFFFu >> (BITS * r0);
+  if (hi < CPW - 1) w &= ~(0xFFFFFFFFu >> (BITS * (hi + 1)));
+  return w;
+}
+
+// One work item per output word: panel words (both halves), code words
+// (both halves), tran floats.  Every index fits 32 bits (fb_check bounds the
+// panel and code arrays below 4 GiB).
+template <int BITS>
 __global__ void __launch_bounds__(256)
-    k_fb_mirror(const FbWin *__restrict__ fw, unsigned N, unsigned H, unsigned bits) {
+    k_fb_mirror(const FbWin *__restrict__ fw, unsigned N, unsigned H) {
   const FbWin f = fw[blockIdx.y];
   const unsigned M = f.M, p = M & 1u;
-  const FbGeom g(M, bits);
-  const unsigned cpw = 32u / bits, mask = (1u << bits) - 1u;
-  const size_t np = (size_t)H * g.wh, nc = (size_t)g.nw * N, ntf = 96u * g.nch;
-  const size_t ntb = 96u * g.nch + 128u;
-  const size_t total = np + nc + ntf + ntb;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
+  const FbGeom g(M, BITS);
+  const unsigned np = H * g.wh, nc = g.nw * N, ntf = 96u * g.nch;
+  const unsigned ntb = 96u * g.nch + 128u;
+  const unsigned total = np + nc + ntf + ntb;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += gridDim.x * blockDim.x) {
     if (t < np) {
-      const unsigned h = (unsigned)(t / g.wh), k = (unsigned)(t % g.wh);
+      const unsigned h = t / g.wh, k = t - h * g.wh;
       const uint64_t *row = f.panel + (size_t)h * f.wn;
       f.cpanel[t] = row[k];
       const long long lo = (long long)M + p - 64ll * (k + 1);
       f.cpanel[np + t] = __builtin_bitreverse64(sites64(row, f.wn, M, lo));
     } else if (t < np + nc) {
-      const size_t u = t - np;
-      const unsigned q = (unsigned)(u / N), i = (unsigned)(u % N);
-      f.ccodes[u] = f.packed[(size_t)q * N + i];
-      uint32_t w = 0;
-      for (unsigned r = 0; r < cpw; ++r) {
-        const long long s = (long long)M - 1 + p - ((long long)q * cpw + r);
-        if (s < 0 || s >= (long long)M) continue;
-        const uint32_t src = f.packed[(size_t)(s / cpw) * N + i];
-        const uint32_t code = (src >> (32u - bits - bits * (unsigned)(s % cpw))) & mask;
-        w |= code << (32u - bits - bits * r);
-      }
-      f.ccodes[nc + u] = w;
+      const unsigned u = t - np, q = u / N, i = u - q * N;
+      f.ccodes[u] = f.packed[u];  // rows of N words in both arrays
+      f.ccodes[nc + u] = mirror_code_word<BITS>(f.packed, N, M, p, q, i);
     } else {
-      const size_t u = t - np - nc;
+      const unsigned u = t - np - nc;
       const bool back = u >= ntf;
-      const unsigned fl = (unsigned)(back ? u - ntf : u), r = fl / 3u, c = fl % 3u;
+      const unsigned fl = back ? u - ntf : u, r = fl / 3u, c = fl - 3u * r;
       float v = 0.0f;
       if (!back) {
         if (r == 0u) v = c == 0u ? 1.0f : 0.0f;
         else if (r < M) v = f.tran[3u * r + c];
       } else if (r <= p) {
         v = c == 0u ? 1.0f : 0.0f;
-      } else if ((long long)M + p - r >= 1) {
+      } else if (M + p >= r + 1u) {
         v = f.tran[3u * (M + p - r) + c];
       }
       f.ctran[u] = v;
@@ -1258,11 +1281,16 @@ int launch_fb_mirror(const FbWin *d_fw, unsigned nwin, unsigned max_items, unsig
   unsigned gx = (max_items + 255u) / 256u;
   const unsigned cap = std::max(1u, 4096u / std::max(1u, nwin));
   gx = std::max(1u, std::min(gx, cap));
-  hipLaunchKernelGGL(k_fb_mirror, dim3(gx, nwin), dim3(256), 0, stream, d_fw, N, H, bits);
+  if (bits == 8)
+    hipLaunchKernelGGL(k_fb_mirror<8>, dim3(gx, nwin), dim3(256), 0, stream, d_fw, N, H);
+  else
+    hipLaunchKernelGGL(k_fb_mirror<4>, dim3(gx, nwin), dim3(256), 0, stream, d_fw, N, H);
   return hip_check(hipGetLastError(), "k_fb_mirror launch");
 }
 
-const void *fb_mirror_kernel_fn() { return (const void *)&k_fb_mirror; }
+const void *fb_mirror_kernel_fn(unsigned bits) {
+  return bits == 8 ? (const void *)&k_fb_mirror<8> : (const void *)&k_fb_mirror<4>;
+}
 
 unsigned fb_items(unsigned M, unsigned bits, unsigned N, unsigned H) {
   const FbGeom g(M, bits);
