@@ -560,21 +560,25 @@ def aux_vq(torch, w, steps):
                                           "identical": same_cb}}}
 
 
+TRAFFIC_FILES = ("k_batched_hbm_traffic.md", "k_batched_fb_hbm_traffic.md")
+
+
 def recorded_traffic(workload):
-    """HBM bytes per dispatch of the batched kernel for this workload, from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
-    copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "k_batched_hbm_traffic.md")
+    """HBM bytes per dispatch of the batched kernel for this workload (plan
+    key "NxMxC[ fb] qQ wW"), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summaries of the forward-only and forward-backward passes
+    (scaled by the copy-kernel calibration), or None."""
     prefix = "traffic_bytes_per_dispatch:"
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith(prefix) and "=" in line:
-                    key, val = line[len(prefix):].rsplit("=", 1)
-                    if key.strip() == workload:
-                        return float(val)
-    except (OSError, ValueError):
-        return None
+    for name in TRAFFIC_FILES:
+        try:
+            with open(os.path.join(ROOT, "profiles", name)) as f:
+                for line in f:
+                    if line.startswith(prefix) and "=" in line:
+                        key, val = line[len(prefix):].rsplit("=", 1)
+                        if key.strip() == workload:
+                            return float(val)
+        except (OSError, ValueError):
+            continue
     return None
 
 

@@ -34,3 +34,15 @@ def test_emission_table_and_fp64_forward_match_oracle(orc):
         emit = orc.sample_emit(p["packed"], int(samples[e]), p["M"], 8, E)
         want = orc.hmm_like_f64(p["panel"], parents[e].astype(np.uint32), emit, p["tran"])
         assert abs(got[e] - want) <= 1e-12 * abs(want)
+
+
+def test_recorded_traffic_reads_both_pass_forms():
+    # the headline plan (forward-backward form) and the forward-only plan
+    # each have a committed FETCH_SIZE + WRITE_SIZE record
+    fb = bench.recorded_traffic("5000x4096x64 fb q5 w8")
+    fwd = bench.recorded_traffic("5000x4096x64 q6 w4")
+    assert fb is not None and fwd is not None
+    # both lie between the algorithmic bytes (0.686 GB) and 4x that
+    for v in (fb, fwd):
+        assert 0.686e9 < v < 4 * 0.686e9
+    assert bench.recorded_traffic("5000x4096x64 fb q99 w8") is None
