@@ -410,6 +410,7 @@ def aux_batching(torch, steps, seed=21, N=2500, W=32, C=32, cpu=True):
                       "sites_min": int(sizes.min()), "sites_median": int(np.median(sizes)),
                       "sites_max": int(sizes.max()), "proposals": C, "precision": "fp32"},
            "lanes_per_window": tuned, "waves_per_workgroup": job.waves(),
+           "split_slots": job.split(),
            "unit": "site-evals/s"}
     for name, fn in (("batched", job.run_batched), ("batched_graph", job.run_batched_graph),
                      ("plain", job.run_plain)):
@@ -608,6 +609,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="lanes per chain (0 = auto)")
     ap.add_argument("--waves", type=int, default=0,
                     help="wavefronts per workgroup with --lanes (4 or 8; 0 = the plan's default)")
+    ap.add_argument("--split", type=int, choices=(0, 1), default=0,
+                    help="with --lanes and --waves 8: wavefronts 4-7 of each workgroup hold one "
+                         "chain slot fewer (glp_plan_set_split)")
     ap.add_argument("--precision", choices=("fb", "mixed"), default="fb",
                     help="fb: two half-window passes joined in FP64 (GLP_PREC_MIXED_FB); "
                          "mixed: the forward-only pass (GLP_PREC_MIXED)")
@@ -640,12 +644,14 @@ def main():
     lanes, grid = job.make_plan(args.lanes)
     if args.lanes and args.waves:
         job.set_waves(args.waves)
+        job.set_split(args.split)
         lanes, grid = job.lanes()
     if not args.lanes:
         t0 = time.time()
         tuned = job.autotune(gen=0, pen=pen, seed=seed)
         lanes, grid = job.lanes()
-        log(f"autotune: lanes/chain {list(tuned)}, {job.waves()} wavefronts per workgroup "
+        log(f"autotune: lanes/chain {list(tuned)}, {job.waves()} wavefronts per workgroup"
+            f"{' (split chain slots)' if job.split() else ''} "
             f"in {time.time() - t0:.2f}s")
     se_step = N * (C + 1) * M
     gen = [0]
@@ -759,7 +765,7 @@ def main():
 
     achieved = FLOP_PER_SE * se_step / (ms / 1e3) / 1e12
     workload_key = (f"{N}x{M}x{C}" + (" fb" if prec == PREC_MIXED_FB else "") +
-                    f" q{lanes} w{job.waves()}")
+                    f" q{lanes} w{job.waves()}" + (" split" if job.split() else ""))
     traffic = recorded_traffic(workload_key)
     out = {
         "metric": "HMM site-evals/s (sample x proposal x site), batched MH generation",
@@ -780,7 +786,7 @@ def main():
                                (", forward-backward form)" if prec == PREC_MIXED_FB else ")"),
                    "precision": args.precision,
                    "samples": N, "sites": M, "proposals": C, "lanes_per_chain": lanes,
-                   "waves_per_workgroup": job.waves(),
+                   "waves_per_workgroup": job.waves(), "split_slots": job.split(),
                    "vq_training_gls": N * M, "vq_iterations": w["vq_iters"]},
         "mh_iterations_per_s": 1e3 / ms,
         "proposals_per_s": N * C / (ms / 1e3),

@@ -136,6 +136,8 @@ SIGNATURES = [
     ("glp_plan_set_lanes", ctypes.c_int, [ctypes.c_void_p, c_u32p]),
     ("glp_plan_set_waves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     ("glp_plan_waves", ctypes.c_int, [ctypes.c_void_p, c_u32p]),
+    ("glp_plan_set_split", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("glp_plan_split", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     ("glp_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, c_u32p]),
     ("glp_plan_destroy", ctypes.c_int, [ctypes.c_void_p]),

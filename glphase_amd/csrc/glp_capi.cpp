@@ -164,10 +164,21 @@ static unsigned auto_lanes(unsigned nchains, unsigned C, int num_cu, unsigned nu
   return best;
 }
 
+// Chain slots of an 8-wavefront workgroup with the split, on a window of Q
+// passes per chain round: wavefronts 0-3 hold floor(units / Q) each,
+// wavefronts 4-7 one fewer (at least one).
+static unsigned split_wg_slots(unsigned nu, unsigned Q) {
+  const unsigned s = nu / Q;
+  return 4 * s + 4 * (s > 1 ? s - 1 : s);
+}
+
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
-  double waves = 0;
-  for (unsigned w : wins) waves += waves_for(p->a.N, p->q[w], units(p->a.precision));
-  const unsigned blocks = (unsigned)std::ceil(waves / (double)p->wpb);
+  const bool split = p->wpb == 8 && p->a.split;
+  double wgs = 0;
+  for (unsigned w : wins)
+    wgs += split ? (double)p->a.N / (double)split_wg_slots(units(p->a.precision), p->q[w])
+                 : waves_for(p->a.N, p->q[w], units(p->a.precision)) / (double)p->wpb;
+  const unsigned blocks = (unsigned)std::ceil(wgs);
   return std::max(1u, std::min((unsigned)(p->bpc * p->num_cu), blocks));
 }
 
@@ -246,6 +257,7 @@ static int fill_args(const glp_mh_desc *d, KArgs &a) {
   a.lut_entries = 16u << d->bits_per_code;
   a.precision = d->precision;
   a.kick = d->kickstart_from_ref ? 1 : 0;
+  a.split = 0;
   return GLP_OK;
 }
 
@@ -534,22 +546,30 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
   float best_ms = 1e30f;
   std::vector<unsigned> best_q = p->q;
   unsigned best_w = p->wpb;
-  // one depth for every window, at each workgroup size that fits
+  int best_split = p->a.split;
+  // one depth for every window, at each workgroup size that fits, and at 8
+  // wavefronts with and without the split
   for (unsigned w : {4u, 8u}) {
     if (rc != GLP_OK || p->bpc_of[w == 8 ? 1 : 0] < 1) continue;
     rc = plan_use_waves(p, w);
     for (unsigned Q : kTrials) {
       if (rc != GLP_OK || Q > a.C + 1 || Q > units(a.precision)) break;
       std::fill(p->q.begin(), p->q.end(), Q);
-      float ms = 0;
-      rc = trial(&ms);
-      if (rc == GLP_OK && ms < best_ms) {
-        best_ms = ms;
-        best_q = p->q;
-        best_w = w;
+      for (int sp = 0; sp < (w == 8 ? 2 : 1); ++sp) {
+        if (rc != GLP_OK || (sp && units(a.precision) / Q < 2)) break;
+        p->a.split = a.split = sp;
+        float ms = 0;
+        rc = trial(&ms);
+        if (rc == GLP_OK && ms < best_ms) {
+          best_ms = ms;
+          best_q = p->q;
+          best_w = w;
+          best_split = sp;
+        }
       }
     }
   }
+  p->a.split = a.split = best_split;
   if (rc == GLP_OK) rc = plan_use_waves(p, best_w);
   if (buckets.size() > 1) {
     for (auto &b : buckets) {
@@ -770,6 +790,21 @@ int glp_plan_set_waves(glp_plan *p, unsigned waves_per_workgroup) {
 int glp_plan_waves(glp_plan *p, unsigned *waves_per_workgroup) {
   if (!p || !waves_per_workgroup) return set_error(GLP_ERR_ARG, "null argument");
   *waves_per_workgroup = p->wpb;
+  return GLP_OK;
+}
+
+int glp_plan_set_split(glp_plan *p, int split) {
+  if (!p) return set_error(GLP_ERR_ARG, "null plan");
+  const int v = split ? 1 : 0;
+  if (v != p->a.split) plan_graph_reset(p);  // the graph's node holds the grid and arguments
+  p->a.split = v;
+  p->grid = plan_grid(p, p->order);
+  return GLP_OK;
+}
+
+int glp_plan_split(glp_plan *p, int *split) {
+  if (!p || !split) return set_error(GLP_ERR_ARG, "null argument");
+  *split = p->a.split;
   return GLP_OK;
 }
 

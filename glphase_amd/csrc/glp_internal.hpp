@@ -30,6 +30,7 @@ struct KArgs {
   unsigned bits;
   int precision;
   int kick;  // proposal 0 draws from the reference haplotypes [2N, H)
+  int split;  // batched kernel at 8 wavefronts per workgroup: wavefronts 4-7 hold one chain slot fewer
 };
 
 int set_error(int code, const char *msg);

@@ -832,7 +832,11 @@ __global__ void __launch_bounds__(256)
 // WAVES wavefronts per workgroup share one copy of the emission and spread
 // tables: 4 (one workgroup per CU) or 8 (two wavefronts per SIMD where the
 // LDS of the F32 ring or of fewer tables allows it; the compiler then keeps
-// the kernel within 256 VGPRs).
+// the kernel within 256 VGPRs).  With 8 and a.split, wavefronts 4-7 hold
+// one home slot fewer than wavefronts 0-3: the chains spread over more
+// workgroups, and where a SIMD runs one wavefront of each half, the one
+// with fewer chains finishes its rounds first and leaves the SIMD's issue
+// slots to the other.
 // FB (the forward-backward form): a pass takes a lane quad instead of a
 // pair, so the units below are quads and a wavefront holds 16 of them.
 template <int BITS, bool MIXED, int WAVES, bool FB>
@@ -907,7 +911,11 @@ __global__ void __launch_bounds__(64 * WAVES)
       Q = __builtin_amdgcn_readfirstlane(qtab[bw]);
       const unsigned slot = pair / Q;
       sbase = UL * Q * slot;
-      home = slot < NU / Q;
+      // a.split: the second wavefront of each SIMD (wavefronts 4-7 of the
+      // workgroup) holds one chain slot fewer than the first
+      unsigned nslots = NU / Q;
+      if (WAVES == 8 && a.split && threadIdx.x >= 256u && nslots > 1u) --nslots;
+      home = slot < nslots;
       W = wins[bw];
       continue;
     }

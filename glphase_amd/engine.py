@@ -144,6 +144,16 @@ class Job:
         check(lib().glp_plan_waves(self.plan, ptr(w, c_u32p)), "glp_plan_waves")
         return int(w[0])
 
+    def set_split(self, split):
+        """Uneven chain slots at 8 wavefronts per workgroup: wavefronts 4-7
+        hold one chain slot fewer than wavefronts 0-3 (results unchanged)."""
+        check(lib().glp_plan_set_split(self.plan, 1 if split else 0), "glp_plan_set_split")
+
+    def split(self):
+        v = ctypes.c_int()
+        check(lib().glp_plan_split(self.plan, ctypes.byref(v)), "glp_plan_split")
+        return bool(v.value)
+
     def lanes(self):
         q, g = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().glp_plan_lanes(self.plan, ctypes.byref(q), ctypes.byref(g)), "glp_plan_lanes")

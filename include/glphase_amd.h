@@ -217,12 +217,23 @@ int glp_plan_set_lanes(glp_plan *plan, const unsigned *lanes_per_window);
  * autotuner tries both. */
 int glp_plan_set_waves(glp_plan *plan, unsigned waves_per_workgroup);
 int glp_plan_waves(glp_plan *plan, unsigned *waves_per_workgroup);
+/* Uneven chain slots at 8 wavefronts per workgroup (split = 1): wavefronts
+ * 4-7 of each workgroup hold one chain slot fewer than wavefronts 0-3
+ * (floor(units / lanes) - 1, at least one), so a generation's chains spread
+ * over more workgroups and, where a SIMD runs one wavefront of each half,
+ * the one with fewer chains finishes first and leaves the SIMD's issue slots
+ * to the other.  Results do not depend on it; ignored at 4 wavefronts.  The
+ * autotuner tries both.  (No counterpart in hmmLike.h: the reference kernel
+ * runs one chain per thread.) */
+int glp_plan_set_split(glp_plan *plan, int split);
+int glp_plan_split(glp_plan *plan, int *split);
 /* Autotuner: times whole generations of the plan (every window, as a real
  * run shares the device) with HIP events on `stream`: first one
  * lanes-per-chain for every window from 1..8, 10, 12, 16, 32 (at most the
- * proposals + 1) at 4 and at 8 wavefronts per workgroup, then, with several
+ * proposals + 1) at 4 and at 8 wavefronts per workgroup (at 8 with and
+ * without the uneven chain slots of glp_plan_set_split), then, with several
  * window-size buckets (ceil(log2 sites)), each bucket's value in turn with
- * the others held and the workgroup size fixed; keeps the fastest.
+ * the others held and the workgroup size and split fixed; keeps the fastest.
  * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
  * trial.  lanes_out (optional) receives the choice per window.
  * Synchronises `stream`. */

@@ -78,6 +78,39 @@ def test_fb_eight_wave_workgroups_and_graphs(orc):
         assert (job.parents() == par).all()
 
 
+@pytest.mark.parametrize("lanes", [1, 3, 5, 8, 16])
+def test_fb_split_chain_slots(orc, lanes):
+    # 8-wave workgroups whose wavefronts 4-7 hold one chain slot fewer: the
+    # grid grows to hold every chain, and the chains equal the oracle's,
+    # eager and through graph replays with eager work between them
+    say(f"fb split chain slots lanes={lanes}")
+    p = helpers.synthetic(400, 600, seed=lanes + 500, depth=2.0)
+    C, pen, seed = 24, 0.6, 808
+    job = job_for(p, C, precision=FB, trace=True)
+    job.make_plan(lanes)
+    job.set_waves(8)
+    job.set_split(True)
+    assert job.split()
+    s = 16 // lanes
+    per_wg = 4 * s + 4 * (s - 1 if s > 1 else s)
+    assert job.lanes() == (lanes, -(-400 // per_wg))
+    job.run_batched(gen=4, pen=pen, seed=seed)
+    compare_run(orc, p, job, C, pen, seed, FB, gen=4)
+    par, like = job.parents(), job.like()
+    for _ in range(3):
+        job.run_batched_graph(gen=4, pen=pen, seed=seed)
+        assert (job.parents() == par).all() and (job.like() == like).all()
+        job.run_batched(gen=4, pen=pen, seed=seed)
+        assert (job.parents() == par).all()
+    ev, useful = job.stats()
+    assert useful == p["N"] * (C + 1) and ev >= useful
+    # switching the split off again rebuilds the graph with the smaller grid
+    job.set_split(False)
+    assert job.lanes() == (lanes, -(-(-(-400 // s)) // 8))
+    job.run_batched_graph(gen=4, pen=pen, seed=seed)
+    assert (job.parents() == par).all() and (job.like() == like).all()
+
+
 def test_fb_layout_follows_panel_changes(orc):
     # every run rebuilds the windows' layouts from their current data: a
     # changed panel between runs is seen by the next run, eager and graph

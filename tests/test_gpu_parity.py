@@ -109,18 +109,21 @@ def test_batched_matches_oracle(orc, lanes):
     assert ev >= useful
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("prec,lanes", [(PREC_F32, 1), (PREC_F32, 6), (PREC_MIXED, 3),
                                         (PREC_MIXED, 6)])
-def test_batched_eight_wave_workgroups(orc, prec, lanes):
-    # 8 wavefronts per workgroup sharing the LDS tables (two per SIMD): the
-    # same chains as the plain oracle, eager and through the plan's graph
-    say(f"batched, 8-wave workgroups prec={prec} lanes={lanes}")
+def test_batched_eight_wave_workgroups(orc, prec, lanes, split):
+    # 8 wavefronts per workgroup sharing the LDS tables (two per SIMD), with
+    # and without one chain slot fewer in wavefronts 4-7: the same chains as
+    # the plain oracle, eager and through the plan's graph
+    say(f"batched, 8-wave workgroups prec={prec} lanes={lanes} split={split}")
     p = helpers.synthetic(400, 600, seed=lanes + 10 * prec, depth=2.0)
     C, pen, seed = 24, 0.6, 808
     job = job_for(p, C, precision=prec, trace=True)
     job.make_plan(lanes)
     job.set_waves(8)
-    assert job.waves() == 8
+    job.set_split(split)
+    assert job.waves() == 8 and job.split() == split
     job.run_batched(gen=4, pen=pen, seed=seed)
     compare_run(orc, p, job, C, pen, seed, prec, gen=4)
     par, like = job.parents(), job.like()
