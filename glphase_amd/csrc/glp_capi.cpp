@@ -523,20 +523,27 @@ int glp_plan_autotune(glp_plan *p, unsigned gen, float pen, uint64_t seed, void 
   // real run): first one depth for every window, then each bucket's depth in
   // turn with the others held (coordinate descent on the measured time)
   int rc = GLP_OK;
+  // a trial's time is the mean over kTrialGens consecutive generations, each
+  // from the saved state: every plan sees the same draws, and a plan's time
+  // varies by several percent from one generation to the next (which chains
+  // accept most), more than the plans near the best differ
+  constexpr unsigned kTrialGens = 4;
   auto trial = [&](float *ms_out) -> int {
     GLP_TRY(plan_upload_q(p));
     const unsigned grid = plan_grid(p, p->order);
     float ms = 0;
-    for (int rep = 0; rep < 2; ++rep) {
+    for (unsigned rep = 0; rep < kTrialGens; ++rep) {
+      KArgs ar = a;
+      ar.gen = gen + rep;
       GLP_TRY(hip_check(hipEventRecord(e0, s), "event record"));
-      GLP_TRY(launch_batched(a, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
+      GLP_TRY(launch_batched(ar, p->wins.as<glp_window>(), p->order_d.as<unsigned>(),
                              p->qtab.as<unsigned>(), p->nwin, p->wctr.as<unsigned>(),
                              p->ctrs.as<unsigned long long>(), grid, p->wpb, s));
       GLP_TRY(hip_check(hipEventRecord(e1, s), "event record"));
       GLP_TRY(hip_check(hipEventSynchronize(e1), "event sync"));
       float t = 0;
       GLP_TRY(hip_check(hipEventElapsedTime(&t, e0, e1), "event time"));
-      ms = rep == 0 ? t : std::min(ms, t);
+      ms += t / kTrialGens;
       GLP_TRY(copy_all(false));
     }
     *ms_out = ms;

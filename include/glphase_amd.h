@@ -234,8 +234,9 @@ int glp_plan_split(glp_plan *plan, int *split);
  * without the uneven chain slots of glp_plan_set_split), then, with several
  * window-size buckets (ceil(log2 sites)), each bucket's value in turn with
  * the others held and the workgroup size and split fixed; keeps the fastest.
- * Window outputs (and inputs of GLP_PROP_HOST) are restored after every
- * trial.  lanes_out (optional) receives the choice per window.
+ * A trial's time is the mean over generations gen .. gen + 3, each run from
+ * the same saved state: window outputs (and inputs of GLP_PROP_HOST) are
+ * restored after every run.  lanes_out (optional) receives the choice per window.
  * Synchronises `stream`. */
 int glp_plan_autotune(glp_plan *plan, unsigned gen, float pen, uint64_t seed,
                       void *stream, unsigned *lanes_out);

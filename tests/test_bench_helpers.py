@@ -40,6 +40,7 @@ def test_recorded_traffic_reads_both_pass_forms():
     # the headline plan (forward-backward form) and the forward-only plan
     # each have a committed FETCH_SIZE + WRITE_SIZE record
     fb = bench.recorded_traffic("5000x4096x64 fb q5 w8")
+    assert bench.recorded_traffic("5000x4096x64 fb q5 w8 split") is not None
     fwd = bench.recorded_traffic("5000x4096x64 q6 w4")
     assert fb is not None and fwd is not None
     # both lie between the algorithmic bytes (0.686 GB) and 4x that
