@@ -172,6 +172,15 @@ static unsigned split_wg_slots(unsigned nu, unsigned Q) {
   return 4 * s + 4 * (s > 1 ? s - 1 : s);
 }
 
+// The deepest Q whose chains, in 8-wavefront workgroups with the split, fit
+// max_wgs resident workgroups at once (0: none does).
+static unsigned auto_lanes_split(unsigned nchains, unsigned C, unsigned max_wgs, unsigned nu) {
+  unsigned best = 0;
+  for (unsigned Q = 1; Q <= nu && Q <= C + 1; ++Q)
+    if (std::ceil((double)nchains / (double)split_wg_slots(nu, Q)) <= (double)max_wgs) best = Q;
+  return best;
+}
+
 static unsigned plan_grid(const glp_plan *p, const std::vector<unsigned> &wins) {
   const bool split = p->wpb == 8 && p->a.split;
   double wgs = 0;
@@ -351,8 +360,19 @@ int glp_plan_create(const glp_mh_desc *desc, unsigned lanes_per_chain, glp_plan 
   std::stable_sort(p->order.begin(), p->order.end(), [&](unsigned x, unsigned y) {
     return desc->windows[x].num_sites > desc->windows[y].num_sites;
   });
-  const unsigned Q =
-      lanes_per_chain ? lanes_per_chain : auto_lanes(a.N * p->nwin, a.C, p->num_cu, nu);
+  unsigned Q = lanes_per_chain ? lanes_per_chain : auto_lanes(a.N * p->nwin, a.C, p->num_cu, nu);
+  if (!lanes_per_chain && plan_fb(p) && p->bpc_of[1] >= 1) {
+    // the forward-backward form: 8-wavefront workgroups with split chain
+    // slots where they reach a deeper speculation that still fits the device
+    // (at 5,000 chains: depth 5 on 250 workgroups against depth 3 at one
+    // wavefront per SIMD)
+    const unsigned Q8 =
+        auto_lanes_split(a.N * p->nwin, a.C, (unsigned)(p->bpc_of[1] * p->num_cu), nu);
+    if (Q8 > Q && plan_use_waves(p, 8) == GLP_OK) {
+      p->a.split = 1;
+      Q = Q8;
+    }
+  }
   p->q.assign(p->nwin, Q);
   p->grid = plan_grid(p, p->order);
   rc = p->wins.ensure(sizeof(glp_window) * p->nwin);

@@ -192,7 +192,10 @@ int glp_plain_step(const glp_mh_desc *desc, unsigned c, void *stream);
  * serial MH result.  lanes_per_chain is 1..32 for every window (1..16 under
  * GLP_PREC_MIXED_FB: a wavefront holds floor(32 / lanes) chains, or
  * floor(16 / lanes); 0 = the deepest value whose wavefronts fit one per
- * SIMD; see glp_plan_autotune).  Under GLP_PREC_MIXED_FB the plan allocates
+ * SIMD, or under GLP_PREC_MIXED_FB the deepest whose chains fit the device
+ * in 8-wavefront workgroups with split chain slots (glp_plan_set_split)
+ * where that is deeper; see glp_plan_autotune).  Under GLP_PREC_MIXED_FB
+ * the plan allocates
  * each window's forward-backward layout (about its panel and codes once
  * more) and every run rebuilds it from the windows' current data first. */
 typedef struct glp_plan glp_plan;

@@ -111,6 +111,29 @@ def test_fb_split_chain_slots(orc, lanes):
     assert (job.parents() == par).all() and (job.like() == like).all()
 
 
+def test_fb_auto_plan_splits_when_deeper(orc):
+    # lanes = 0: the deepest depth at one wavefront per SIMD (4-wavefront
+    # workgroups), or 8-wavefront workgroups with split chain slots where
+    # their deepest depth that fits the device is deeper
+    import torch
+    say("fb automatic plan shape")
+    N, C = 5000, 8
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    q4 = max(Q for Q in range(1, min(16, C + 1) + 1) if -(-N // (16 // Q)) <= 4 * cus)
+    fits = [Q for Q in range(1, C + 2)
+            if -(-N // (4 * (16 // Q) + 4 * max(1, 16 // Q - 1))) <= cus]
+    q8 = max(fits) if fits else 0
+    p = helpers.synthetic(N, 40, seed=31, training_size=4096)
+    job = job_for(p, C, precision=FB, trace=True)
+    lanes, grid = job.make_plan(0)
+    if q8 > q4:
+        assert (lanes, job.waves(), job.split()) == (q8, 8, True)
+    else:
+        assert (lanes, job.waves(), job.split()) == (q4, 4, False)
+    job.run_batched(gen=1, pen=0.8, seed=17)
+    compare_run(orc, p, job, C, 0.8, 17, FB, gen=1)
+
+
 def test_fb_layout_follows_panel_changes(orc):
     # every run rebuilds the windows' layouts from their current data: a
     # changed panel between runs is seen by the next run, eager and graph
