@@ -60,7 +60,10 @@ __global__ void __launch_bounds__(256)
 // haplotypes (a, b) with its tallies (ha, hb) as they stand and swapped; the
 // tallies then count the pair in the orientation that agrees more (a to ha
 // when sis > tra, else b to ha).  Integer sums, so any order gives the
-// reference's values.
+// reference's values.  VEC (M a multiple of 4, tallies 16-byte aligned): a
+// lane takes 4 consecutive sites of each 256-site chunk through one 16-byte
+// load and store per tally row; otherwise one site per lane and 64-site word.
+template <bool VEC>
 __global__ void __launch_bounds__(256)
     k_replace(const uint64_t *__restrict__ haps, unsigned N, unsigned M, unsigned wn,
               uint32_t *__restrict__ hsum) {
@@ -69,6 +72,51 @@ __global__ void __launch_bounds__(256)
   const uint64_t *oa = haps + (size_t)(2u * i) * wn, *ob = oa + wn;
   uint32_t *ha = hsum + (size_t)(2u * i) * M, *hb = ha + M;
   unsigned sis = 0, tra = 0;
+  if (VEC) {
+    // chunk c, lane l: sites 256 c + 4 l .. + 3, bits 4 (l & 15) .. + 3 of
+    // allele word 4 c + l / 16
+    uint4 *ha4 = reinterpret_cast<uint4 *>(ha), *hb4 = reinterpret_cast<uint4 *>(hb);
+    const unsigned nc = (M + 255u) >> 8, sh = 4u * (lane & 15u);
+    for (unsigned c = 0; c < nc; ++c) {
+      const unsigned m0 = 256u * c + 4u * lane;
+      if (m0 < M) {
+        const unsigned w = 4u * c + (lane >> 4);
+        const unsigned xa = (unsigned)(oa[w] >> sh) & 15u, xb = (unsigned)(ob[w] >> sh) & 15u;
+        const uint4 sa = ha4[m0 >> 2], sb = hb4[m0 >> 2];
+        const unsigned a0 = xa & 1u, a1 = (xa >> 1) & 1u, a2 = (xa >> 2) & 1u, a3 = xa >> 3;
+        const unsigned b0 = xb & 1u, b1 = (xb >> 1) & 1u, b2 = (xb >> 2) & 1u, b3 = xb >> 3;
+        sis += a0 * sa.x + a1 * sa.y + a2 * sa.z + a3 * sa.w + b0 * sb.x + b1 * sb.y +
+               b2 * sb.z + b3 * sb.w;
+        tra += a0 * sb.x + a1 * sb.y + a2 * sb.z + a3 * sb.w + b0 * sa.x + b1 * sa.y +
+               b2 * sa.z + b3 * sa.w;
+      }
+    }
+    for (unsigned o = 32; o; o >>= 1) {
+      sis += (unsigned)__shfl_xor((int)sis, (int)o, 64);
+      tra += (unsigned)__shfl_xor((int)tra, (int)o, 64);
+    }
+    const bool keep = sis > tra;
+    for (unsigned c = 0; c < nc; ++c) {
+      const unsigned m0 = 256u * c + 4u * lane;
+      if (m0 < M) {
+        const unsigned w = 4u * c + (lane >> 4);
+        const unsigned xa = (unsigned)(oa[w] >> sh) & 15u, xb = (unsigned)(ob[w] >> sh) & 15u;
+        const unsigned ka = keep ? xa : xb, kb = keep ? xb : xa;
+        uint4 sa = ha4[m0 >> 2], sb = hb4[m0 >> 2];
+        sa.x += ka & 1u;
+        sa.y += (ka >> 1) & 1u;
+        sa.z += (ka >> 2) & 1u;
+        sa.w += ka >> 3;
+        sb.x += kb & 1u;
+        sb.y += (kb >> 1) & 1u;
+        sb.z += (kb >> 2) & 1u;
+        sb.w += kb >> 3;
+        ha4[m0 >> 2] = sa;
+        hb4[m0 >> 2] = sb;
+      }
+    }
+    return;
+  }
   for (unsigned w = 0; w < wn; ++w) {
     const unsigned m = 64u * w + lane;
     const uint64_t wa = oa[w], wb = ob[w];
@@ -116,7 +164,12 @@ extern "C" int glp_replace(const uint64_t *haps, unsigned num_samples, unsigned 
   if (!haps || !hsum || num_samples == 0 || num_sites == 0)
     return set_error(GLP_ERR_ARG, "null pointer or empty window");
   const unsigned wn = (num_sites + 63) / 64;
-  hipLaunchKernelGGL(k_replace, dim3((num_samples + 3) / 4), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), haps, num_samples, num_sites, wn, hsum);
+  const bool vec = num_sites % 4u == 0u && (reinterpret_cast<uintptr_t>(hsum) & 15u) == 0u;
+  if (vec)
+    hipLaunchKernelGGL(k_replace<true>, dim3((num_samples + 3) / 4), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), haps, num_samples, num_sites, wn, hsum);
+  else
+    hipLaunchKernelGGL(k_replace<false>, dim3((num_samples + 3) / 4), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), haps, num_samples, num_sites, wn, hsum);
   return hip_check(hipGetLastError(), "k_replace launch");
 }

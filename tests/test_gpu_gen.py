@@ -58,22 +58,29 @@ def test_init_haps_matches_oracle(N, M, seed, window):
     assert (got == want).all()
 
 
-def gpu_replace_rounds(haps_list, N, M):
+def gpu_replace_rounds(haps_list, N, M, offset=0):
+    # offset: the tallies start `offset` int32 into their buffer (offset 1
+    # leaves them 4 bytes off 16-byte alignment, the per-site form)
     import torch
     from glphase_amd._lib import check, lib
-    hs = torch.zeros((2 * N, M), dtype=torch.int32, device="cuda")
+    buf = torch.zeros(offset + 2 * N * M, dtype=torch.int32, device="cuda")
+    hs = buf[offset:]
     s = torch.cuda.current_stream().cuda_stream
     for h in haps_list:
         hd = torch.from_numpy(h.view(np.int64)).cuda()
         check(lib().glp_replace(hd.data_ptr(), N, M, hs.data_ptr(), s), "glp_replace")
     torch.cuda.synchronize()
-    return hs.cpu().numpy().view(np.uint32)
+    return hs.cpu().numpy().view(np.uint32).reshape(2 * N, M)
 
 
+# site counts that are multiples of 4 take the 16-byte form (4 sites per
+# lane), with chunks of 256 sites whole, partial (1000, 2000, 5000, 260) or
+# shorter than one (4, 100); the others the per-site form
 @pytest.mark.parametrize("N,M,rounds", [(1, 1, 3), (13, 130, 6), (200, 1000, 8), (9, 1025, 4),
-                                         (9, 2000, 3), (5000, 4096, 3), (7, 5000, 4)])
-def test_replace_matches_oracle(N, M, rounds):
-    say(f"replace N={N} M={M}")
+                                         (9, 2000, 3), (5000, 4096, 3), (7, 5000, 4), (3, 4, 5),
+                                         (11, 100, 4), (5, 260, 4)])
+def test_replace_matches_oracle(N, M, rounds, offset=0):
+    say(f"replace N={N} M={M} offset={offset}")
     rng = np.random.default_rng(N + M)
     wn = (M + 63) // 64
     hl = []
@@ -84,11 +91,17 @@ def test_replace_matches_oracle(N, M, rounds):
         if r % 2:
             h = h.reshape(N, 2, wn)[:, ::-1].reshape(2 * N, wn).copy()
         hl.append(h)
-    got = gpu_replace_rounds(hl, N, M)
+    got = gpu_replace_rounds(hl, N, M, offset)
     want = np.zeros((2 * N, M), np.uint32)
     for h in hl:
         orc.replace(h, N, M, want)
     assert (got == want).all()
+
+
+@pytest.mark.parametrize("N,M", [(13, 130), (11, 100), (5, 260), (9, 2000)])
+def test_replace_unaligned_tallies(N, M):
+    # tallies 4 bytes off 16-byte alignment: the per-site form at every size
+    test_replace_matches_oracle(N, M, 4, offset=1)
 
 
 def _truth(path):
